@@ -1,0 +1,177 @@
+/*
+ * updec.h -- C ABI of the MI355X-native decimal expression path.
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device memory
+ * owned by the caller; every call is ordered on the given HIP stream
+ * (NULL = the null stream).  Functions return UPDEC_OK (0) or an error code;
+ * updec_last_error() gives the message of the calling thread's last error.
+ * No C++ exception crosses this boundary.
+ *
+ * Each entry point names the reference interface it replaces.  Paths are
+ * relative to the reference repository.
+ */
+#ifndef UPDEC_H
+#define UPDEC_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UPDEC_OK 0
+#define UPDEC_EINVAL 1       /* bad argument */
+#define UPDEC_ECOMPILE 2     /* parse, rewrite or hipRTC compile failure */
+#define UPDEC_EHIP 3         /* HIP runtime error */
+#define UPDEC_EUNSUPPORTED 4 /* construct outside what the HIP path implements */
+
+#define UPDEC_MODE_AUTO 0     /* faithful inside the reference's envelope, else extended */
+#define UPDEC_MODE_FAITHFUL 1 /* 32-limb wrap and DivOrMod path quirks */
+#define UPDEC_MODE_EXTENDED 2 /* same rules, exact, no wrap */
+
+typedef void* updec_stream_t; /* hipStream_t */
+typedef struct updec_expr_s* updec_expr_t;
+typedef struct updec_agg_s* updec_agg_t;
+typedef struct updec_module_s* updec_module_t;
+
+/* Column of the compact decimal layout: DECIMAL(precision, scale), row
+ * stride updec_compact_bytes(precision).  `bits` is an optional statistic
+ * (every stored magnitude < 2^bits, 0 = unknown) from updec_column_maxbits. */
+typedef struct updec_column_desc {
+    const char* name;
+    int32_t precision;
+    int32_t scale;
+    int32_t bits;
+} updec_column_desc;
+
+const char* updec_last_error(void);
+int updec_version(void);
+
+/* Select the device (hipSetDevice) and warm the runtime. */
+int updec_init(int device);
+
+/* Compact bytes per row of DECIMAL(P, *): GetDecimalRealBytes
+ * (src/datatypes/decimal.cu:464-467, table at :92). */
+int updec_compact_bytes(int precision);
+
+/* Largest magnitude bit length over a compact column (synchronous). */
+int updec_column_maxbits(const uint8_t* d_col, int precision, int64_t rows, int32_t* bits_out, updec_stream_t stream);
+
+/* ---------------- expression path ----------------
+ * Replaces AriesEngineShell::AlignOptimize (src/AriesEngineWrapper/AriesEngineShell.cpp:646-685),
+ * AEExprCalcNode::GenerateCudaFunction (src/AriesEngine/AriesExprCalcNode.cpp:1285-1318) and
+ * AriesDynamicKernelManager::CompileKernels (src/CudaAcc/DynamicKernel.cu:60-158):
+ * parse `text` over the named columns, rewrite with the reference's
+ * alignment scheduling and constant pre-calculation, type every node,
+ * generate a gfx950 kernel and compile it with hipRTC (module cache keyed
+ * by source text). */
+int updec_expr_compile(const char* text, const updec_column_desc* cols, int ncols, int mode, updec_expr_t* out);
+/* result type of the materialized column, its compact bytes, and whether
+ * the faithful contract applies */
+int updec_expr_info(updec_expr_t e, int32_t* precision, int32_t* scale, int32_t* out_bytes, int32_t* faithful);
+/* canonical text of the rewritten, typed plan (NUL-terminated; returns the
+ * needed size in *needed) */
+int updec_expr_plan(updec_expr_t e, char* buf, size_t cap, size_t* needed);
+int updec_expr_source(updec_expr_t e, char* buf, size_t cap, size_t* needed);
+/* Evaluate every row into compact output rows of out_bytes each:
+ * AEExprDynKernelNode::Process / RunKernelFunction
+ * (src/AriesEngine/AriesExprCalcNode.cpp:543-575) and
+ * AriesDynamicKernelManager::CallKernel (src/CudaAcc/DynamicKernel.cu:281-334).
+ * d_cols[i] is column i of the compile-time descriptor list; d_out must be
+ * 4-byte aligned. */
+int updec_expr_calc(updec_expr_t e, const uint8_t* const* d_cols, int64_t rows, uint8_t* d_out, updec_stream_t stream);
+void updec_expr_free(updec_expr_t e);
+
+/* ---------------- fused aggregation ----------------
+ * Replaces AEExprAggFunctionNode::RunKernelFunction
+ * (src/AriesEngine/AriesExprCalcNode.cpp:1623-1700) with
+ * AggregateColumnData(SUM/COUNT) (src/CudaAcc/AriesSqlOperator_group.cu:360-424)
+ * for the case the planner can fuse: SUM(expr_k) for k < nexprs over rows
+ * passing an optional int32 filter, grouped by up to two 1-byte key columns.
+ * Group discovery runs first; groups are numbered in ascending key order
+ * (the order SortColumns gives, AriesSqlOperator_group.cu:14-54).
+ * Output: images[g * nexprs + k] is the 133-byte nullable_type<Decimal>
+ * image of SUM(expr_k) in group g; counts[g] = COUNT(*) of group g;
+ * keys_out[g] = key0 | key1 << 8 of group g (host memory). */
+typedef struct updec_agg_desc {
+    const updec_column_desc* cols;
+    int32_t ncols;
+    const char* const* exprs;
+    int32_t nexprs;
+    int32_t has_filter; /* filter on the int32 column passed to updec_agg_run */
+    int32_t filter_op;  /* 0 <=, 1 <, 2 >=, 3 >, 4 ==, 5 != */
+    int64_t filter_value;
+    int32_t nkeys; /* 0, 1 or 2 one-byte key columns */
+    int32_t mode;
+} updec_agg_desc;
+
+int updec_agg_compile(const updec_agg_desc* desc, updec_agg_t* out);
+int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_filter, const uint8_t* d_key0,
+                  const uint8_t* d_key1, int64_t rows, uint8_t* d_images, int64_t* d_counts, int32_t max_groups,
+                  int32_t* ngroups_out, uint16_t* keys_out, updec_stream_t stream);
+/* device time of the last run's main aggregation kernel, in milliseconds
+ * (HIP events on the run's stream; synchronizes) */
+int updec_agg_last_kernel_ms(updec_agg_t a, float* ms);
+/* generated source for a given group count (no device needed) */
+int updec_agg_source(updec_agg_t a, int32_t ngroups, char* buf, size_t cap, size_t* needed);
+void updec_agg_free(updec_agg_t a);
+
+/* AVG = SUM / COUNT per group: DivisionInt64 (src/CudaAcc/AriesSqlOperator_helpers.cu:717-764),
+ * division_data_int64 (src/CudaAcc/AriesEngineAlgorithm.cu:9476-9535), do_div
+ * (src/CudaAcc/algorithm/kernel_util.hxx:427-438): Decimal(sum) /= Decimal((int64)count).
+ * d_sums: n images of 133 bytes at a stride of stride_bytes. */
+int updec_div_int64(const uint8_t* d_sums, int64_t stride_bytes, const int64_t* d_counts, uint8_t* d_out, int64_t n,
+                    int mode, updec_stream_t stream);
+
+/* Segmented SUM of a compact column in `associated` order with segment
+ * starts `groups`: aggregate_column<CompactDecimal> SUM
+ * (src/CudaAcc/AriesEngineAlgorithm.h:832-857) through segreduce
+ * (src/CudaAcc/algorithm/kernel_segreduce.hxx:535-612).  d_associated may
+ * be NULL (identity).  d_out: ngroups 133-byte images. */
+int updec_segsum_compact(const uint8_t* d_col, int precision, int scale, int64_t rows, const int32_t* d_associated,
+                         const int32_t* d_groups, int32_t ngroups, uint8_t* d_out, int mode, updec_stream_t stream);
+
+/* ---------------- dynamic-code interface ----------------
+ * AriesDynamicKernelManager::CompileKernels / CallKernel
+ * (src/CudaAcc/DynamicKernel.h:50-99): compile caller-supplied kernel
+ * source (which may #include "dec_device.h") for gfx950 with hipRTC and
+ * launch an extern "C" kernel by name with one by-value argument block. */
+int updec_compile(const char* src, size_t len, updec_module_t* out);
+int updec_launch(updec_module_t m, const char* fn, uint32_t grid, uint32_t block, const void* args, size_t args_size,
+                 updec_stream_t stream);
+void updec_module_free(updec_module_t m);
+/* hipRTC-compile a source for gfx950 without loading it (no device needed);
+ * returns UPDEC_ECOMPILE with the compiler log on failure */
+int updec_check_compile(const char* src, size_t len);
+
+/* ---------------- seeded synthetic columns ----------------
+ * Counter-based generator (splitmix64 keyed by seed, column id and row), so
+ * any row range can be produced independently and identically on host and
+ * device.  kind:
+ *   UPDEC_GEN_RANGE:  magnitude = (lo + r mod (hi - lo + 1)) * step   (64-bit)
+ *   UPDEC_GEN_DIGITS: magnitude uniform with exactly `digits` decimal digits
+ *                     (leading digit nonzero: uniform in [10^(digits-1), 10^digits))
+ * neg_permille: probability of a negative sign in 1/1000. */
+#define UPDEC_GEN_RANGE 0
+#define UPDEC_GEN_DIGITS 1
+typedef struct updec_gen_desc {
+    int32_t kind;
+    int32_t precision;
+    int32_t digits;
+    int32_t neg_permille;
+    uint64_t lo, hi, step;
+    uint64_t seed;
+    uint32_t column_id;
+} updec_gen_desc;
+int updec_gen_compact(const updec_gen_desc* g, int64_t row0, int64_t rows, uint8_t* out, int on_device,
+                      updec_stream_t stream);
+/* int32 uniform in [lo, hi] and 1-byte values drawn from a set */
+int updec_gen_int32(uint64_t seed, uint32_t column_id, int32_t lo, int32_t hi, int64_t row0, int64_t rows, int32_t* out,
+                    int on_device, updec_stream_t stream);
+int updec_gen_byte(uint64_t seed, uint32_t column_id, const uint8_t* choices, int32_t nchoices, int64_t row0,
+                   int64_t rows, uint8_t* out, int on_device, updec_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

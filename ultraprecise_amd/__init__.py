@@ -1,0 +1,289 @@
+"""MI355X-native arbitrary-precision DECIMAL expression path.
+
+Python host layer over the C ABI of ``libupdec.so`` (include/updec.h).
+The compute path is HIP only: when the library is missing, importing this
+package raises instead of falling back to anything else.  PyTorch is used
+for device memory and streams.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libupdec.so")
+
+MODE_AUTO, MODE_FAITHFUL, MODE_EXTENDED = 0, 1, 2
+GEN_RANGE, GEN_DIGITS = 0, 1
+IMAGE_BYTES = 133  # nullable_type<Decimal>: flag, sign, prec, frac, error, v[32]
+
+
+class UpdecError(RuntimeError):
+    pass
+
+
+class ColumnDesc(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("precision", ctypes.c_int32), ("scale", ctypes.c_int32),
+                ("bits", ctypes.c_int32)]
+
+
+class AggDesc(ctypes.Structure):
+    _fields_ = [("cols", ctypes.POINTER(ColumnDesc)), ("ncols", ctypes.c_int32),
+                ("exprs", ctypes.POINTER(ctypes.c_char_p)), ("nexprs", ctypes.c_int32),
+                ("has_filter", ctypes.c_int32), ("filter_op", ctypes.c_int32), ("filter_value", ctypes.c_int64),
+                ("nkeys", ctypes.c_int32), ("mode", ctypes.c_int32)]
+
+
+class GenDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("precision", ctypes.c_int32), ("digits", ctypes.c_int32),
+                ("neg_permille", ctypes.c_int32), ("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64),
+                ("step", ctypes.c_uint64), ("seed", ctypes.c_uint64), ("column_id", ctypes.c_uint32)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or ultraprecise_amd/build.py")
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I32, I64, U64, SZ = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_size_t
+    sig = {
+        "updec_last_error": (ctypes.c_char_p, []),
+        "updec_version": (ctypes.c_int, []),
+        "updec_init": (ctypes.c_int, [ctypes.c_int]),
+        "updec_compact_bytes": (ctypes.c_int, [ctypes.c_int]),
+        "updec_column_maxbits": (ctypes.c_int, [P, ctypes.c_int, I64, ctypes.POINTER(I32), P]),
+        "updec_expr_compile": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ColumnDesc), ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(P)]),
+        "updec_expr_info": (ctypes.c_int, [P, ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(I32),
+                                           ctypes.POINTER(I32)]),
+        "updec_expr_plan": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
+        "updec_expr_source": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
+        "updec_expr_calc": (ctypes.c_int, [P, ctypes.POINTER(P), I64, P, P]),
+        "updec_expr_free": (None, [P]),
+        "updec_agg_compile": (ctypes.c_int, [ctypes.POINTER(AggDesc), ctypes.POINTER(P)]),
+        "updec_agg_run": (ctypes.c_int, [P, ctypes.POINTER(P), P, P, P, I64, P, P, I32, ctypes.POINTER(I32),
+                                         ctypes.POINTER(ctypes.c_uint16), P]),
+        "updec_agg_last_kernel_ms": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float)]),
+        "updec_agg_source": (ctypes.c_int, [P, I32, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
+        "updec_agg_free": (None, [P]),
+        "updec_div_int64": (ctypes.c_int, [P, I64, P, P, I64, ctypes.c_int, P]),
+        "updec_segsum_compact": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I64, P, P, I32, P, ctypes.c_int, P]),
+        "updec_compile": (ctypes.c_int, [ctypes.c_char_p, SZ, ctypes.POINTER(P)]),
+        "updec_launch": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, P, SZ, P]),
+        "updec_module_free": (None, [P]),
+        "updec_check_compile": (ctypes.c_int, [ctypes.c_char_p, SZ]),
+        "updec_gen_compact": (ctypes.c_int, [ctypes.POINTER(GenDesc), I64, I64, P, ctypes.c_int, P]),
+        "updec_gen_int32": (ctypes.c_int, [U64, ctypes.c_uint32, I32, I32, I64, I64, P, ctypes.c_int, P]),
+        "updec_gen_byte": (ctypes.c_int, [U64, ctypes.c_uint32, P, I32, I64, I64, P, ctypes.c_int, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_lib = _load()
+
+# C ABI symbols declared in include/updec.h (checked by the CPU tests)
+EXPORTED = ["updec_last_error", "updec_version", "updec_init", "updec_compact_bytes", "updec_column_maxbits",
+            "updec_expr_compile", "updec_expr_info", "updec_expr_plan", "updec_expr_source", "updec_expr_calc",
+            "updec_expr_free", "updec_agg_compile", "updec_agg_run", "updec_agg_last_kernel_ms", "updec_agg_source",
+            "updec_agg_free", "updec_div_int64", "updec_segsum_compact", "updec_compile", "updec_launch",
+            "updec_module_free", "updec_check_compile", "updec_gen_compact", "updec_gen_int32", "updec_gen_byte"]
+
+
+def lib():
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise UpdecError(f"updec error {rc}: {_lib.updec_last_error().decode(errors='replace')}")
+
+
+def _text(fn, h, *extra):
+    need = ctypes.c_size_t(0)
+    _check(fn(h, *extra, None, 0, ctypes.byref(need)))
+    buf = ctypes.create_string_buffer(need.value)
+    _check(fn(h, *extra, buf, need.value, ctypes.byref(need)))
+    return buf.value.decode()
+
+
+def compact_bytes(precision):
+    return _lib.updec_compact_bytes(precision)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return ctypes.c_void_p(stream)
+
+
+def _col_array(cols):
+    arr = (ColumnDesc * len(cols))()
+    keep = []
+    for i, c in enumerate(cols):
+        name, p, s = c[0], c[1], c[2]
+        bits = c[3] if len(c) > 3 else 0
+        b = name.encode()
+        keep.append(b)
+        arr[i] = ColumnDesc(b, p, s, bits)
+    return arr, keep
+
+
+def init(device=0):
+    _check(_lib.updec_init(device))
+
+
+class Expr:
+    """A compiled decimal expression: parse, the reference's alignment
+    rewrite, typing, HIP code generation and (lazily) hipRTC compilation.
+    ``cols`` is a list of (name, precision, scale[, bits])."""
+
+    def __init__(self, text, cols, mode=MODE_AUTO):
+        self.cols = [tuple(c) for c in cols]
+        arr, keep = _col_array(self.cols)
+        h = ctypes.c_void_p()
+        _check(_lib.updec_expr_compile(text.encode(), arr, len(self.cols), mode, ctypes.byref(h)))
+        self._h = h
+        P, S, B, F = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(_lib.updec_expr_info(h, ctypes.byref(P), ctypes.byref(S), ctypes.byref(B), ctypes.byref(F)))
+        self.precision, self.scale, self.out_bytes, self.faithful = P.value, S.value, B.value, bool(F.value)
+
+    @property
+    def plan(self):
+        return _text(_lib.updec_expr_plan, self._h)
+
+    @property
+    def source(self):
+        return _text(_lib.updec_expr_source, self._h)
+
+    def calc(self, columns, rows, out, stream=None):
+        """columns: device uint8 tensors in descriptor order; out: device
+        uint8 tensor of rows * out_bytes bytes."""
+        ptrs = (ctypes.c_void_p * max(1, len(columns)))(*[c.data_ptr() for c in columns])
+        _check(_lib.updec_expr_calc(self._h, ptrs, rows, _ptr(out), _stream(stream)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.updec_expr_free(self._h)
+            self._h = None
+
+
+FILTER_OPS = {"<=": 0, "<": 1, ">=": 2, ">": 3, "==": 4, "!=": 5}
+
+
+class Agg:
+    """Fused SUM(expr_k) / COUNT(*) with an optional int32 filter and up to
+    two one-byte group keys."""
+
+    def __init__(self, exprs, cols, filter=None, nkeys=0, mode=MODE_AUTO):
+        self.cols = [tuple(c) for c in cols]
+        self.exprs = list(exprs)
+        arr, keep = _col_array(self.cols)
+        ex = (ctypes.c_char_p * len(self.exprs))(*[e.encode() for e in self.exprs])
+        d = AggDesc(arr, len(self.cols), ex, len(self.exprs), 0, 0, 0, nkeys, mode)
+        if filter is not None:
+            d.has_filter, d.filter_op, d.filter_value = 1, FILTER_OPS[filter[0]], int(filter[1])
+        h = ctypes.c_void_p()
+        _check(_lib.updec_agg_compile(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self.nkeys = nkeys
+
+    def source(self, ngroups=1):
+        return _text(_lib.updec_agg_source, self._h, ctypes.c_int32(ngroups))
+
+    def run(self, columns, rows, images, counts, filter_col=None, key0=None, key1=None, max_groups=255, stream=None):
+        """Returns (ngroups, keys).  images: device uint8 tensor of at least
+        max_groups * nexprs * 133 bytes; counts: device int64 tensor."""
+        ptrs = (ctypes.c_void_p * max(1, len(columns)))(*[c.data_ptr() for c in columns])
+        ng = ctypes.c_int32(0)
+        keys = (ctypes.c_uint16 * max(1, max_groups))()
+        _check(_lib.updec_agg_run(self._h, ptrs, _ptr(filter_col), _ptr(key0), _ptr(key1), rows, _ptr(images),
+                                  _ptr(counts), max_groups, ctypes.byref(ng), keys, _stream(stream)))
+        return ng.value, [keys[i] for i in range(ng.value)]
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_float(0)
+        _check(_lib.updec_agg_last_kernel_ms(self._h, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.updec_agg_free(self._h)
+            self._h = None
+
+
+def div_int64(sums, stride, counts, out, n, mode=MODE_AUTO, stream=None):
+    _check(_lib.updec_div_int64(_ptr(sums), stride, _ptr(counts), _ptr(out), n, mode, _stream(stream)))
+
+
+def segsum_compact(col, precision, scale, rows, associated, groups, ngroups, out, mode=MODE_AUTO, stream=None):
+    _check(_lib.updec_segsum_compact(_ptr(col), precision, scale, rows, _ptr(associated), _ptr(groups), ngroups,
+                                     _ptr(out), mode, _stream(stream)))
+
+
+def column_maxbits(col, precision, rows, stream=None):
+    b = ctypes.c_int32(0)
+    _check(_lib.updec_column_maxbits(_ptr(col), precision, rows, ctypes.byref(b), _stream(stream)))
+    return b.value
+
+
+def check_compile(src):
+    b = src.encode()
+    _check(_lib.updec_check_compile(b, len(b)))
+
+
+def gen_desc(kind, precision, seed, column_id, digits=0, lo=0, hi=0, step=1, neg_permille=0):
+    return GenDesc(kind, precision, digits, neg_permille, lo, hi, step, seed, column_id)
+
+
+def gen_compact_host(desc, rows, row0=0):
+    """Generate a compact column on the host; returns a numpy uint8 array."""
+    import numpy as np
+    nb = compact_bytes(desc.precision)
+    out = np.empty(rows * nb, dtype=np.uint8)
+    _check(_lib.updec_gen_compact(ctypes.byref(desc), row0, rows, out.ctypes.data_as(ctypes.c_void_p), 0, None))
+    return out
+
+
+def gen_compact_device(desc, rows, row0=0, stream=None):
+    import torch
+    nb = compact_bytes(desc.precision)
+    out = torch.empty(max(1, rows * nb), dtype=torch.uint8, device="cuda")
+    _check(_lib.updec_gen_compact(ctypes.byref(desc), row0, rows, _ptr(out), 1, _stream(stream)))
+    return out
+
+
+def gen_int32_host(seed, column_id, lo, hi, rows, row0=0):
+    import numpy as np
+    out = np.empty(rows, dtype=np.int32)
+    _check(_lib.updec_gen_int32(seed, column_id, lo, hi, row0, rows, out.ctypes.data_as(ctypes.c_void_p), 0, None))
+    return out
+
+
+def gen_int32_device(seed, column_id, lo, hi, rows, row0=0, stream=None):
+    import torch
+    out = torch.empty(max(1, rows), dtype=torch.int32, device="cuda")
+    _check(_lib.updec_gen_int32(seed, column_id, lo, hi, row0, rows, _ptr(out), 1, _stream(stream)))
+    return out
+
+
+def gen_byte_host(seed, column_id, choices, rows, row0=0):
+    import numpy as np
+    ch = (ctypes.c_uint8 * len(choices))(*choices)
+    out = np.empty(rows, dtype=np.uint8)
+    _check(_lib.updec_gen_byte(seed, column_id, ch, len(choices), row0, rows, out.ctypes.data_as(ctypes.c_void_p), 0,
+                               None))
+    return out
+
+
+def gen_byte_device(seed, column_id, choices, rows, row0=0, stream=None):
+    import torch
+    ch = (ctypes.c_uint8 * len(choices))(*choices)
+    out = torch.empty(max(1, rows), dtype=torch.uint8, device="cuda")
+    _check(_lib.updec_gen_byte(seed, column_id, ch, len(choices), row0, rows, _ptr(out), 1, _stream(stream)))
+    return out

@@ -1,0 +1,57 @@
+"""Build step for the HIP library: embeds dec_device.h for hipRTC and
+compiles libupdec.so for gfx950 in-tree (the built library travels with the
+source tree; nothing is installed)."""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libupdec.so")
+
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def embed_device_header():
+    src = open(os.path.join(CSRC, "dec_device.h")).read()
+    delim = "UPDEC_EMBED"
+    assert ")" + delim + '"' not in src
+    out = ("// generated from dec_device.h by build.py; do not edit\n"
+           "static const char kDecDeviceSource[] = R\"" + delim + "(" + src + ")" + delim + "\";\n")
+    path = os.path.join(CSRC, "dec_device_embed.inc")
+    if not os.path.exists(path) or open(path).read() != out:
+        with open(path, "w") as f:
+            f.write(out)
+
+
+def build(verbose=False):
+    embed_device_header()
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    objs = []
+    for src in ("updec_api.hip", "planner.cpp"):
+        obj = os.path.join(CSRC, src.rsplit(".", 1)[0] + ".o")
+        cmd = [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if src.endswith(".cpp"):
+            cmd = [hipcc, "-x", "hip"] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    cmd = [hipcc, "-shared", "-fPIC", "--offload-arch=gfx950"] + objs + ["-lhiprtc", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB)

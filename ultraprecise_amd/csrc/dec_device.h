@@ -1,0 +1,599 @@
+// dec_device.h -- decimal limb arithmetic for gfx950 (CDNA4), header-only.
+//
+// Compiled twice: by hipcc into the static kernels of libupdec.so, and by
+// hipRTC into every per-query expression module (the text of this file is
+// embedded in the library and handed to hipRTC as a header).
+//
+// Values are sign-magnitude, as the reference's Decimal
+// (src/datatypes/decimal.hxx:130-141): a base-2^32 little-endian magnitude
+// plus a sign flag.  Unlike the reference's fixed 32-limb array, every value
+// here has a limb count fixed at code-generation time from a bit-width bound,
+// so one lane holds exactly the limbs its value can need and all limb
+// indices are compile-time constants (the arrays live in VGPRs, never in
+// scratch).  Scales are compile-time too: alignment is a multiply by a
+// constant 10^n row.
+#ifndef UPDEC_DEC_DEVICE_H
+#define UPDEC_DEC_DEVICE_H
+
+#ifndef __HIPCC_RTC__
+#include <hip/hip_runtime.h>
+#endif
+
+#define UPD_DEV __device__ __forceinline__
+
+namespace updec {
+
+typedef unsigned char u8;
+typedef unsigned int u32;
+typedef int i32;
+typedef unsigned long long u64;
+typedef long long i64;
+typedef __SIZE_TYPE__ usize;
+
+template <int W> struct Mag { u32 w[W]; };
+template <int W> struct Dec { Mag<W> m; u32 neg; };
+
+// ---------------------------------------------------------------- constants
+
+// Exact 10^n as 32-bit words, built at compile time (n <= 460).
+struct BigConst {
+    u32 w[48];
+    int n;
+    constexpr BigConst(int p) : w(), n(0)
+    {
+        w[0] = 1;
+        for (int k = 0; k < p; k++) {
+            u64 c = 0;
+            for (int i = 0; i < 48; i++) {
+                u64 t = (u64)w[i] * 10u + c;
+                w[i] = (u32)t;
+                c = t >> 32;
+            }
+        }
+        int t = 47;
+        while (t > 0 && w[t] == 0) t--;
+        n = t + 1;
+    }
+    constexpr int bits() const
+    {
+        int b = 0;
+        u32 x = w[n - 1];
+        while (x) { b++; x >>= 1; }
+        return 32 * (n - 1) + b;
+    }
+};
+
+template <int N> struct P10 {
+    static constexpr BigConst c = BigConst(N);
+    static constexpr int W = c.n;
+    static constexpr int BITS = c.bits();
+};
+
+// The reference's estimate of bitlen(10^shift) used to pick the uint64
+// division path (decimal.cu:2252-2256).
+constexpr int est_shift_bits(int s) { return (s / 3) * 10 + (s % 3 == 1 ? 4 : (s % 3 == 2 ? 7 : 0)); }
+
+// ---------------------------------------------------------------- magnitude helpers
+
+template <int WO, int WI> UPD_DEV Mag<WO> resize(const Mag<WI>& a)
+{
+    Mag<WO> r;
+#pragma unroll
+    for (int i = 0; i < WO; i++) r.w[i] = (i < WI) ? a.w[i] : 0u;
+    return r;
+}
+
+template <int W> UPD_DEV Mag<W> mzero()
+{
+    Mag<W> r;
+#pragma unroll
+    for (int i = 0; i < W; i++) r.w[i] = 0u;
+    return r;
+}
+
+template <int W> UPD_DEV bool is_zero(const Mag<W>& a)
+{
+    u32 o = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) o |= a.w[i];
+    return o == 0;
+}
+
+template <int W> UPD_DEV u32 bitlen(const Mag<W>& a)
+{
+    u32 b = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) b = a.w[i] ? (u32)(32 * i + 32 - __builtin_clz(a.w[i])) : b;
+    return b;
+}
+
+// r = a + b, returns carry out
+template <int W> UPD_DEV u32 add_to(Mag<W>& r, const Mag<W>& a, const Mag<W>& b)
+{
+    u64 c = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        u64 t = (u64)a.w[i] + b.w[i] + c;
+        r.w[i] = (u32)t;
+        c = t >> 32;
+    }
+    return (u32)c;
+}
+
+// r = a - b, returns borrow out (1 when a < b)
+template <int W> UPD_DEV u32 sub_to(Mag<W>& r, const Mag<W>& a, const Mag<W>& b)
+{
+    u64 br = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        u64 t = (u64)a.w[i] - b.w[i] - br;
+        r.w[i] = (u32)t;
+        br = (t >> 63) & 1u;
+    }
+    return (u32)br;
+}
+
+// -1, 0, 1
+template <int W> UPD_DEV int cmp(const Mag<W>& a, const Mag<W>& b)
+{
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) c = (a.w[i] != b.w[i]) ? (a.w[i] > b.w[i] ? 1 : -1) : c;
+    return c;
+}
+
+// low WO limbs of a * b (schoolbook, 32x32->64 multiply-accumulate)
+template <int WO, int WA, int WB> UPD_DEV Mag<WO> mul_lo(const Mag<WA>& a, const Mag<WB>& b)
+{
+    Mag<WO> r = mzero<WO>();
+#pragma unroll
+    for (int i = 0; i < WA; i++) {
+        if (i >= WO) break;
+        u64 c = 0;
+#pragma unroll
+        for (int j = 0; j < WB; j++) {
+            if (i + j < WO) {
+                u64 t = (u64)a.w[i] * b.w[j] + r.w[i + j] + c;
+                r.w[i + j] = (u32)t;
+                c = t >> 32;
+            }
+        }
+        if (i + WB < WO) r.w[i + WB] = (u32)c;
+    }
+    return r;
+}
+
+// low WO limbs of a * 10^N (constant row, zero limbs of 10^N skipped)
+template <int WO, int WA, int N> UPD_DEV Mag<WO> mul_p10(const Mag<WA>& a)
+{
+    if constexpr (N == 0) {
+        return resize<WO>(a);
+    } else {
+        constexpr BigConst p = P10<N>::c;
+        constexpr int WP = P10<N>::W;
+        Mag<WO> r = mzero<WO>();
+#pragma unroll
+        for (int i = 0; i < WA; i++) {
+            if (i >= WO) break;
+            u64 c = 0;
+#pragma unroll
+            for (int j = 0; j < WP; j++) {
+                if (i + j < WO) {
+                    u64 t = (u64)a.w[i] * p.w[j] + r.w[i + j] + c;
+                    r.w[i + j] = (u32)t;
+                    c = t >> 32;
+                }
+            }
+            if (i + WP < WO) r.w[i + WP] = (u32)c;
+        }
+        return r;
+    }
+}
+
+// shift left by k bits (0 <= k < 32*W), bits shifted past the top are lost.
+// Word moves are a log-depth select network so all indices stay static.
+template <int W> UPD_DEV Mag<W> shl_var(const Mag<W>& a, u32 k)
+{
+    u32 ws = k >> 5, bs = k & 31u;
+    Mag<W> t = a;
+#pragma unroll
+    for (int st = 1; st < W; st <<= 1) {
+        bool on = (ws & (u32)st) != 0;
+        Mag<W> u;
+#pragma unroll
+        for (int i = 0; i < W; i++) u.w[i] = (i >= st) ? t.w[i - st] : 0u;
+#pragma unroll
+        for (int i = 0; i < W; i++) t.w[i] = on ? u.w[i] : t.w[i];
+    }
+    Mag<W> r;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        u64 hi = t.w[i], lo = (i > 0) ? t.w[i - 1] : 0u;
+        r.w[i] = (u32)((((hi << 32) | lo) << bs) >> 32);
+    }
+    return r;
+}
+
+// shift right by k bits (0 <= k < 32*W)
+template <int W> UPD_DEV Mag<W> shr_var(const Mag<W>& a, u32 k)
+{
+    u32 ws = k >> 5, bs = k & 31u;
+    Mag<W> t = a;
+#pragma unroll
+    for (int st = 1; st < W; st <<= 1) {
+        bool on = (ws & (u32)st) != 0;
+        Mag<W> u;
+#pragma unroll
+        for (int i = 0; i < W; i++) u.w[i] = (i + st < W) ? t.w[i + st] : 0u;
+#pragma unroll
+        for (int i = 0; i < W; i++) t.w[i] = on ? u.w[i] : t.w[i];
+    }
+    Mag<W> r;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        u64 lo = t.w[i], hi = (i + 1 < W) ? t.w[i + 1] : 0u;
+        r.w[i] = (u32)(((hi << 32) | lo) >> bs);
+    }
+    return r;
+}
+
+// Reciprocal of a normalized divisor word d (top bit set):
+// v = floor((2^64 - 1) / d) - 2^32  (Moller & Granlund, "Improved division
+// by invariant integers", IEEE Trans. Computers 60(2), 2011, section 3).
+UPD_DEV u32 recip_word(u32 d)
+{
+    return (u32)((~0ull) / (u64)d - (1ull << 32));
+}
+
+// 2-by-1 division with a precomputed reciprocal (Moller & Granlund 2011,
+// Algorithm 4).  Requires u1 < d, d normalized.  q = floor((u1*2^32+u0)/d).
+UPD_DEV u32 div_2by1(u32 u1, u32 u0, u32 d, u32 v, u32& r)
+{
+    u64 q = (u64)v * u1;
+    q += ((u64)(u1 + 1u) << 32) | u0;
+    u32 q1 = (u32)(q >> 32), q0 = (u32)q;
+    u32 rr = u0 - q1 * d;
+    if (rr > q0) { q1 -= 1u; rr += d; }
+    if (rr >= d) { q1 += 1u; rr -= d; }
+    r = rr;
+    return q1;
+}
+
+// Q = floor(X / B), R = X mod B, for B != 0.  Knuth, TAOCP vol. 2, 4.3.1,
+// Algorithm D, with the divisor normalized to fill all NB limbs (so the
+// quotient loop has a static trip count of NX+1 digits) and the digit
+// estimate from div_2by1 refined by the second divisor limb.
+template <int NX, int NB> UPD_DEV void divmod(const Mag<NX>& X, const Mag<NB>& B, Mag<NX + 1>& Q, Mag<NB>& R)
+{
+    constexpr int NU = NX + NB + 1;
+    const u32 k = 32u * NB - bitlen(B);
+    const Mag<NB> vn = shl_var<NB>(B, k);
+    Mag<NU> un = shl_var<NU>(resize<NU>(X), k);
+    const u32 d1 = vn.w[NB - 1];
+    const u32 d0 = (NB >= 2) ? vn.w[NB - 2] : 0u;
+    const u32 v = recip_word(d1);
+#pragma unroll
+    for (int j = NX; j >= 0; j--) {
+        const u32 u2 = un.w[j + NB], u1 = un.w[j + NB - 1];
+        u32 qhat;
+        u64 rhat;
+        if (u2 >= d1) {
+            qhat = 0xffffffffu;
+            rhat = (u64)u1 + d1;
+        } else {
+            u32 r32;
+            qhat = div_2by1(u2, u1, d1, v, r32);
+            rhat = r32;
+        }
+        if constexpr (NB >= 2) {
+            const u32 u0 = un.w[j + NB - 2];
+#pragma unroll
+            for (int it = 0; it < 2; it++) {
+                bool fix = (rhat < (1ull << 32)) && ((u64)qhat * d0 > ((rhat << 32) | u0));
+                qhat -= fix ? 1u : 0u;
+                rhat += fix ? d1 : 0u;
+            }
+        }
+        // un[j .. j+NB] -= qhat * vn
+        u64 kk = 0;
+        i64 t;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            u64 p = (u64)qhat * vn.w[i];
+            t = (i64)un.w[i + j] - (i64)kk - (i64)(p & 0xffffffffull);
+            un.w[i + j] = (u32)t;
+            kk = (p >> 32) - (u64)(t >> 32);
+        }
+        t = (i64)un.w[j + NB] - (i64)kk;
+        un.w[j + NB] = (u32)t;
+        // rare add-back when the estimate was one too large
+        const u32 neg = (t < 0) ? 0xffffffffu : 0u;
+        qhat += neg;  // qhat - 1 when negative
+        u64 c = 0;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            u64 s = (u64)un.w[i + j] + (vn.w[i] & neg) + c;
+            un.w[i + j] = (u32)s;
+            c = s >> 32;
+        }
+        un.w[j + NB] += (u32)c & neg;
+        Q.w[j] = qhat;
+    }
+    R = shr_var<NB>(resize<NB>(un), k);
+}
+
+// ---------------------------------------------------------------- signed operations
+
+template <int WO, int WI> UPD_DEV Dec<WO> dresize(const Dec<WI>& a)
+{
+    Dec<WO> r;
+    r.m = resize<WO>(a.m);
+    r.neg = a.neg;
+    return r;
+}
+
+// unary minus flips the sign, also of zero (decimal.cu:578-586)
+template <int W> UPD_DEV Dec<W> dneg(const Dec<W>& a)
+{
+    Dec<W> r = a;
+    r.neg = a.neg ^ 1u;
+    return r;
+}
+
+// a * 10^N at the same sign (scale alignment, decimal.cu:2905-2920)
+template <int WO, int WI, int N> UPD_DEV Dec<WO> dscale(const Dec<WI>& a)
+{
+    Dec<WO> r;
+    r.m = mul_p10<WO, WI, N>(a.m);
+    r.neg = a.neg;
+    return r;
+}
+
+// Add of equally scaled operands with the reference's sign rules
+// (decimal.cu:1284-1323): same sign -> magnitudes add and the sign is kept
+// (so -0 + -0 = -0); different signs -> |larger| - |smaller| with the sign
+// of the larger, and +0 on a tie.
+template <int WO, int WA, int WB> UPD_DEV Dec<WO> dadd(const Dec<WA>& a, const Dec<WB>& b)
+{
+    const Mag<WO> x = resize<WO>(a.m);
+    const Mag<WO> y = resize<WO>(b.m);
+    const bool same = a.neg == b.neg;
+    const u32 m = same ? 0u : 0xffffffffu;
+    Mag<WO> t;
+    u64 c = same ? 0u : 1u;
+#pragma unroll
+    for (int i = 0; i < WO; i++) {
+        u64 s = (u64)x.w[i] + (y.w[i] ^ m) + c;
+        t.w[i] = (u32)s;
+        c = s >> 32;
+    }
+    // different signs and no carry out: x < y, t = x - y + 2^(32 WO) -> negate
+    const bool flip = !same && c == 0;
+    const u32 fm = flip ? 0xffffffffu : 0u;
+    Dec<WO> r;
+    u64 cc = flip ? 1u : 0u;
+    u32 any = 0;
+#pragma unroll
+    for (int i = 0; i < WO; i++) {
+        u64 s = (u64)(t.w[i] ^ fm) + cc;
+        r.m.w[i] = (u32)s;
+        cc = s >> 32;
+        any |= r.m.w[i];
+    }
+    r.neg = same ? a.neg : (flip ? b.neg : (any ? a.neg : 0u));
+    return r;
+}
+
+// a - b == a + (-b) under these rules (decimal.cu:1575-1613)
+template <int WO, int WA, int WB> UPD_DEV Dec<WO> dsub(const Dec<WA>& a, const Dec<WB>& b)
+{
+    return dadd<WO, WA, WB>(a, dneg<WB>(b));
+}
+
+// MulDecimalOnCpu (decimal.cu:1325-1351): sign xor (so -0 is possible), magnitude product
+template <int WO, int WA, int WB> UPD_DEV Dec<WO> dmul(const Dec<WA>& a, const Dec<WB>& b)
+{
+    Dec<WO> r;
+    r.m = mul_lo<WO, WA, WB>(a.m, b.m);
+    r.neg = a.neg ^ b.neg;
+    return r;
+}
+
+// DivOrMod, isMod = false (decimal.cu:2201-2333): q = round-half-up of
+// |a| * 10^S / |b| with S = scale(b) + 4; result scale = scale(a) + 4 and
+// sign = xor; 0 / x = +0; x / 0 keeps |a| (the reference flags
+// ERR_DIV_BY_ZERO in a field the compact layout does not store).
+// FAITH selects the reference's path quirks (see od_div in the oracle):
+//   (i)  bitlen(a) + est(10^S) - 1 <= 64 and bitlen(b) <= 64: uint64 numerator
+//        and doubled remainder, both wrapping mod 2^64;
+//   (ii) |b| < 2^32: the round-up test is (uint32)(2R) >= |b|;
+//   (iii) otherwise: when bitlen(X) == bitlen(b) - 1 the result is 1.
+template <int WO, int WA, int WB, int S, int FAITH>
+UPD_DEV Dec<WO> ddiv(const Dec<WA>& a, const Dec<WB>& b, u32* err = nullptr)
+{
+    constexpr int WX = WA + P10<S>::W;
+    const bool azero = is_zero(a.m);
+    const bool bzero = is_zero(b.m);
+    Mag<WB> bs = b.m;
+    if (bzero) bs.w[0] = 1u;
+    const Mag<WX> X = mul_p10<WX, WA, S>(a.m);
+    Mag<WX + 1> Q;
+    Mag<WB> R;
+    divmod<WX, WB>(X, bs, Q, R);
+    Mag<WB> T;
+    sub_to<WB>(T, bs, R);
+    u32 rnd = cmp<WB>(R, T) >= 0 ? 1u : 0u;
+    if constexpr (FAITH) {
+        const u32 bla = bitlen(a.m), blb = bitlen(bs);
+        const bool path_i = (bla + (u32)est_shift_bits(S) - 1u <= 64u) && blb <= 64u;
+        bool top0 = true;
+#pragma unroll
+        for (int i = 1; i < WB; i++) top0 = top0 && bs.w[i] == 0u;
+        const bool path_ii = !path_i && top0;
+        const bool path_iii_one = !path_i && !path_ii && (bitlen(X) + 1u == blb);
+        if (path_ii) rnd = ((u32)(R.w[0] * 2u) >= bs.w[0]) ? 1u : 0u;
+        if (path_iii_one) {
+            Q = mzero<WX + 1>();
+            Q.w[0] = 1u;
+            rnd = 0u;
+        }
+        if (path_i) {
+            // uint64 emulation of DivByInt64 (decimal.cu:2431-2446)
+            Mag<2> a2 = resize<2>(a.m);
+            Mag<2> dvs = mul_p10<2, 2, S>(a2);
+            Mag<2> dvt = resize<2>(bs);
+            Mag<3> q3;
+            Mag<2> r2;
+            divmod<2, 2>(dvs, dvt, q3, r2);
+            Mag<2> r2d;
+            r2d.w[0] = r2.w[0] << 1;
+            r2d.w[1] = (r2.w[1] << 1) | (r2.w[0] >> 31);
+            rnd = cmp<2>(r2d, dvt) >= 0 ? 1u : 0u;
+            Q = resize<WX + 1>(resize<2>(q3));
+        }
+    }
+    // Q += rnd
+    u64 c = rnd;
+#pragma unroll
+    for (int i = 0; i < WX + 1; i++) {
+        u64 s = (u64)Q.w[i] + c;
+        Q.w[i] = (u32)s;
+        c = s >> 32;
+    }
+    Dec<WO> r;
+    if (azero) {
+        r.m = mzero<WO>();
+        r.neg = 0u;
+    } else if (bzero) {
+        r.m = resize<WO>(a.m);
+        r.neg = a.neg ^ b.neg;
+        if (err) *err = 2u;
+    } else {
+        r.m = resize<WO>(Q);
+        r.neg = a.neg ^ b.neg;
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------- compact codec
+
+// Decode one compact DECIMAL row (decimal.cu:168-176): B little-endian bytes
+// at row*B, sign in bit 7 of the last byte.  Loads are the aligned dwords
+// covering the row; the last rows of a column, whose covering dwords would
+// run past the column, take a byte path.
+template <int B> UPD_DEV Dec<(B + 3) / 4> load_compact(const u8* __restrict__ col, usize row, usize colbytes)
+{
+    constexpr int ND = (B + 6) / 4;
+    constexpr int NW = (B + 3) / 4;
+    const usize off = row * (usize)B;
+    const usize d0 = off >> 2;
+    const u32 sh = (u32)(off & 3u);
+    u32 dw[ND + 1];
+    if ((d0 + ND) * 4u <= colbytes) {
+        const u32* p = reinterpret_cast<const u32*>(col) + d0;
+#pragma unroll
+        for (int i = 0; i < ND; i++) dw[i] = __builtin_nontemporal_load(p + i);
+    } else {
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+            u32 x = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                usize o = (d0 + i) * 4u + b;
+                u32 byte = (o < colbytes) ? (u32)col[o] : 0u;
+                x |= byte << (8 * b);
+            }
+            dw[i] = x;
+        }
+    }
+    dw[ND] = 0u;
+    Dec<NW> r;
+#pragma unroll
+    for (int i = 0; i < NW; i++) r.m.w[i] = __builtin_amdgcn_alignbyte(dw[i + 1], dw[i], sh);
+    if constexpr ((B % 4) != 0) r.m.w[NW - 1] &= (1u << (8 * (B % 4))) - 1u;
+    constexpr int SB = 8 * ((B - 1) % 4) + 7;
+    r.neg = (r.m.w[NW - 1] >> SB) & 1u;
+    r.m.w[NW - 1] &= ~(1u << SB);
+    return r;
+}
+
+// Encode (decimal.cu:337-341): the low B bytes of the magnitude with the sign
+// OR-ed into bit 7 of the last byte.  Returns the bytes as words.
+template <int B, int W> UPD_DEV Mag<(B + 3) / 4> encode_compact(const Dec<W>& a)
+{
+    constexpr int NW = (B + 3) / 4;
+    Mag<NW> r;
+#pragma unroll
+    for (int i = 0; i < NW; i++) r.w[i] = (i < W) ? a.m.w[i] : 0u;
+    if constexpr ((B % 4) != 0) r.w[NW - 1] &= (1u << (8 * (B % 4))) - 1u;
+    constexpr int SW = (B - 1) / 4, SB = 8 * ((B - 1) % 4) + 7;
+    r.w[SW] |= a.neg << SB;
+    return r;
+}
+
+// ---------------------------------------------------------------- two's complement accumulators
+
+// acc += (-1)^neg * m, sign-extended to AW limbs
+template <int AW, int W> UPD_DEV void acc_add(Mag<AW>& acc, const Dec<W>& v)
+{
+    const u32 m = v.neg ? 0xffffffffu : 0u;
+    u64 c = v.neg ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < AW; i++) {
+        u32 x = (i < W) ? (v.m.w[i] ^ m) : m;
+        u64 s = (u64)acc.w[i] + x + c;
+        acc.w[i] = (u32)s;
+        c = s >> 32;
+    }
+}
+
+template <int AW> UPD_DEV void acc_merge(Mag<AW>& acc, const Mag<AW>& o)
+{
+    add_to<AW>(acc, acc, o);
+}
+
+// wave64 butterfly reduction of a two's complement multi-limb value
+template <int AW> UPD_DEV void wave_reduce(Mag<AW>& acc)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        Mag<AW> o;
+#pragma unroll
+        for (int i = 0; i < AW; i++) o.w[i] = (u32)__shfl_xor((int)acc.w[i], off, 64);
+        add_to<AW>(acc, acc, o);
+    }
+}
+
+// Convert a two's complement sum into the 133-byte nullable_type<Decimal>
+// image {flag, sign, prec, frac, error, v[32]} (aries_types.hxx:298-310,
+// decimal.hxx:130-141).  Zero is +0 (a +0 seed never yields -0).
+template <int AW> UPD_DEV void write_sum_image(u8* out, const Mag<AW>& acc, u32 prec, u32 frac, u32 error)
+{
+    const bool neg = (acc.w[AW - 1] >> 31) != 0;
+    Mag<AW> m;
+    const u32 fm = neg ? 0xffffffffu : 0u;
+    u64 c = neg ? 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < AW; i++) {
+        u64 s = (u64)(acc.w[i] ^ fm) + c;
+        m.w[i] = (u32)s;
+        c = s >> 32;
+    }
+    out[0] = 1;
+    out[1] = neg ? 1 : 0;
+    out[2] = (u8)prec;
+    out[3] = (u8)frac;
+    out[4] = (u8)error;
+#pragma unroll
+    for (int i = 0; i < 32; i++) {
+        u32 x = (i < AW) ? m.w[i] : 0u;
+        out[5 + 4 * i + 0] = (u8)x;
+        out[5 + 4 * i + 1] = (u8)(x >> 8);
+        out[5 + 4 * i + 2] = (u8)(x >> 16);
+        out[5 + 4 * i + 3] = (u8)(x >> 24);
+    }
+}
+
+} // namespace updec
+
+#endif

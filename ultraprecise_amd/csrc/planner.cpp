@@ -1,0 +1,1181 @@
+// planner.cpp -- expression parsing, the reference's AlignOptimize rewrite,
+// result typing, limb-width analysis and HIP source generation.
+#include "planner.h"
+
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <queue>
+#include <sstream>
+
+namespace updec {
+
+// ------------------------------------------------------------------ format constants
+
+// __PREC_BYTE_ARRAY (decimal.cu:92): compact bytes per precision, a format
+// constant of the column layout, valid for P <= 299.
+static const uint8_t kPrecBytes[300] = {
+    0,   1,   1,   2,   2,   3,   3,   4,   4,   4,   5,   5,   6,   6,   6,   7,   7,   8,   8,   9,   9,   9,   10,  10,  11,
+    11,  11,  12,  12,  13,  13,  14,  14,  14,  15,  15,  16,  16,  16,  17,  17,  18,  18,  19,  19,  19,  20,  20,  21,  21,
+    21,  22,  22,  23,  23,  24,  24,  24,  25,  25,  26,  26,  26,  27,  27,  28,  28,  29,  29,  29,  30,  30,  31,  31,  31,
+    32,  32,  33,  33,  34,  34,  34,  35,  35,  36,  36,  36,  37,  37,  38,  38,  38,  39,  39,  40,  40,  41,  41,  41,  42,
+    42,  43,  43,  43,  44,  44,  45,  45,  46,  46,  46,  47,  47,  48,  48,  48,  49,  49,  50,  50,  51,  51,  51,  52,  52,
+    53,  53,  53,  54,  54,  55,  55,  56,  56,  56,  57,  57,  58,  58,  58,  59,  59,  60,  60,  61,  61,  61,  62,  62,  63,
+    63,  63,  64,  64,  65,  65,  66,  66,  66,  67,  67,  68,  68,  68,  69,  69,  70,  70,  71,  71,  71,  72,  72,  73,  73,
+    73,  74,  74,  75,  75,  76,  76,  76,  77,  77,  78,  78,  78,  79,  79,  80,  80,  81,  81,  81,  82,  82,  83,  83,  83,
+    84,  84,  85,  85,  86,  86,  86,  87,  87,  88,  88,  88,  89,  89,  90,  90,  91,  91,  91,  92,  92,  93,  93,  93,  94,
+    94,  95,  95,  96,  96,  96,  97,  97,  98,  98,  98,  99,  99,  100, 100, 101, 101, 101, 102, 102, 103, 103, 103, 104, 104,
+    105, 105, 105, 106, 106, 107, 107, 108, 108, 108, 109, 109, 110, 110, 110, 111, 111, 112, 112, 113, 113, 113, 114, 114, 115,
+    115, 115, 116, 116, 117, 117, 118, 118, 118, 119, 119, 120, 120, 120, 121, 121, 122, 122, 123, 123, 123, 124, 124, 125, 125};
+
+static int pow10_bits(int n)
+{
+    if (n <= 0) return n == 0 ? 1 : 0;
+    std::vector<uint32_t> v(n / 9 + 2, 0);
+    v[0] = 1;
+    for (int k = 0; k < n; k++) {
+        uint64_t c = 0;
+        for (auto& w : v) { uint64_t t = (uint64_t)w * 10u + c; w = (uint32_t)t; c = t >> 32; }
+    }
+    int t = (int)v.size() - 1;
+    while (t > 0 && !v[t]) t--;
+    return 32 * t + (32 - __builtin_clz(v[t]));
+}
+
+int compact_bytes(int P)
+{
+    if (P < 0) return 0;
+    if (P < 300) return kPrecBytes[P];
+    // ceil((bitlen(10^P - 1) + 1) / 8); 10^P is never a power of two, so
+    // bitlen(10^P - 1) == bitlen(10^P)
+    return (pow10_bits(P) + 1 + 7) / 8;
+}
+
+// ------------------------------------------------------------------ HostDec
+
+static const uint32_t kPow9[10] = {1u, 10u, 100u, 1000u, 10000u, 100000u, 1000000u, 10000000u, 100000000u, 1000000000u};
+
+static void hd_reduce(HostDec& d, Mode m)
+{
+    if (m == Mode::Faithful)
+        for (int i = 32; i < HostDec::NW; i++) d.v[i] = 0;
+}
+
+static uint32_t hd_mul_small(uint32_t* v, uint32_t k, uint32_t add)
+{
+    uint64_t c = add;
+    for (int i = 0; i < HostDec::NW; i++) { uint64_t t = (uint64_t)v[i] * k + c; v[i] = (uint32_t)t; c = t >> 32; }
+    return (uint32_t)c;
+}
+
+static void hd_mul10n(HostDec& d, int n, Mode m)
+{
+    while (n > 0) { int k = n > 9 ? 9 : n; hd_mul_small(d.v, kPow9[k], 0); n -= k; }
+    hd_reduce(d, m);
+}
+
+HostDec HostDec::from_string(const std::string& s, Mode m)
+{
+    HostDec d;
+    size_t i = 0;
+    int dot = 0, digits = 0;
+    bool dotflag = false;
+    if (!s.empty() && s[0] == '-') { d.sign = 1; i = 1; }
+    for (; i < s.size(); i++) {
+        if (s[i] == '.') { dotflag = true; continue; }
+        if (dotflag) dot++;
+        digits++;
+        hd_mul_small(d.v, 10u, (uint32_t)(s[i] - '0'));
+    }
+    d.prec = digits;
+    d.frac = dot;
+    hd_reduce(d, m);
+    return d;
+}
+
+HostDec HostDec::from_int(int64_t t, int prec)
+{
+    HostDec d;
+    d.prec = prec;
+    uint64_t u = t < 0 ? (uint64_t)(-(t + 1)) + 1u : (uint64_t)t;
+    if (t < 0) d.sign = 1;
+    d.v[0] = (uint32_t)u;
+    d.v[1] = (uint32_t)(u >> 32);
+    return d;
+}
+
+HostDec HostDec::from_int_scale(int32_t t, int32_t scale, Mode m)
+{
+    HostDec d;
+    int prec = 10;
+    if (scale > prec) prec += scale;
+    d.prec = prec;
+    d.frac = scale;
+    int64_t tt = t;
+    if (tt < 0) { d.sign = 1; tt = -tt; }
+    d.v[0] = (uint32_t)tt;
+    hd_mul10n(d, scale, m);
+    return d;
+}
+
+HostDec HostDec::rescaled_up(int scale, Mode m) const
+{
+    HostDec d = *this;
+    int gap = scale - frac;
+    if (gap > 0) hd_mul10n(d, gap, m);
+    d.frac = scale;
+    d.prec = prec + gap;
+    return d;
+}
+
+static int hd_cmp(const uint32_t* a, const uint32_t* b)
+{
+    for (int i = HostDec::NW - 1; i >= 0; i--)
+        if (a[i] != b[i]) return a[i] > b[i] ? 1 : -1;
+    return 0;
+}
+
+void HostDec::add(const HostDec& d, Mode m)
+{
+    HostDec t = d;
+    if (t.frac < frac) { hd_mul10n(t, frac - t.frac, m); t.frac = frac; }
+    else if (t.frac > frac) { hd_mul10n(*this, t.frac - frac, m); frac = t.frac; }
+    if (sign == t.sign) {
+        uint64_t c = 0;
+        for (int i = 0; i < NW; i++) { uint64_t s = (uint64_t)v[i] + t.v[i] + c; v[i] = (uint32_t)s; c = s >> 32; }
+    } else {
+        int r = hd_cmp(v, t.v);
+        const uint32_t* x = r >= 0 ? v : t.v;
+        const uint32_t* y = r >= 0 ? t.v : v;
+        uint32_t out[NW];
+        uint64_t br = 0;
+        for (int i = 0; i < NW; i++) { uint64_t s = (uint64_t)x[i] - y[i] - br; out[i] = (uint32_t)s; br = (s >> 63) & 1; }
+        memcpy(v, out, sizeof out);
+        sign = (r > 0 && !d.sign) || (r < 0 && d.sign);
+    }
+    hd_reduce(*this, m);
+}
+
+void HostDec::mul(const HostDec& d, Mode m)
+{
+    sign = d.sign ^ sign;
+    frac = frac + d.frac;
+    uint32_t r[NW] = {0};
+    for (int i = 0; i < NW; i++) {
+        if (!v[i]) continue;
+        uint64_t c = 0;
+        for (int j = 0; i + j < NW; j++) {
+            uint64_t t = (uint64_t)v[i] * d.v[j] + r[i + j] + c;
+            r[i + j] = (uint32_t)t;
+            c = t >> 32;
+        }
+    }
+    memcpy(v, r, sizeof r);
+    hd_reduce(*this, m);
+}
+
+bool HostDec::is_zero() const
+{
+    for (int i = 0; i < NW; i++)
+        if (v[i]) return false;
+    return true;
+}
+
+// Decimal == Decimal(1): sign 0 and |M| == 10^frac after alignment
+bool HostDec::equals_one() const
+{
+    if (sign) return false;
+    HostDec one;
+    one.v[0] = 1;
+    hd_mul10n(one, frac, Mode::Extended);
+    return hd_cmp(v, one.v) == 0;
+}
+
+int HostDec::bitlen() const
+{
+    for (int i = NW - 1; i >= 0; i--)
+        if (v[i]) return 32 * i + 32 - __builtin_clz(v[i]);
+    return 0;
+}
+
+// CheckAndSetRealPrecision (decimal.cu:2060-2088), double arithmetic included
+void HostDec::check_and_set_real_precision()
+{
+    int i = 31;
+    for (; i >= 0; i--)
+        if (v[i] != 0) break;
+    if (i == -1) { prec = 1; frac = 0; return; }
+    prec = i * 9;
+    double maxUint32 = 4.294967295;
+    double tmpMul = 1;
+    for (int j = 0; j < i; j++) tmpMul *= maxUint32;
+    tmpMul *= v[i];
+    int tt = (int)tmpMul;
+    while (tt > 0) { tt /= 10; prec++; }
+}
+
+std::string HostDec::hex() const
+{
+    int t = NW - 1;
+    while (t > 0 && !v[t]) t--;
+    char buf[16];
+    std::string s;
+    snprintf(buf, sizeof buf, "%x", v[t]);
+    s += buf;
+    for (int i = t - 1; i >= 0; i--) { snprintf(buf, sizeof buf, "%08x", v[i]); s += buf; }
+    return s;
+}
+
+// ------------------------------------------------------------------ parser
+
+namespace {
+struct Lexer {
+    const std::string& s;
+    size_t i = 0;
+    explicit Lexer(const std::string& x) : s(x) {}
+    void ws() { while (i < s.size() && isspace((unsigned char)s[i])) i++; }
+    bool eat(char c) { ws(); if (i < s.size() && s[i] == c) { i++; return true; } return false; }
+    char peek() { ws(); return i < s.size() ? s[i] : 0; }
+};
+
+struct Parser {
+    Lexer lx;
+    const std::vector<std::string>& names;
+    const std::vector<ColumnType>& types;
+    Mode mode;
+    std::string err;
+    Parser(const std::string& t, const std::vector<std::string>& n, const std::vector<ColumnType>& ty, Mode m)
+        : lx(t), names(n), types(ty), mode(m) {}
+
+    ExprP calc(Op op, ExprP a, ExprP b)
+    {
+        auto e = std::make_shared<Expr>();
+        e->kind = NK::Calc;
+        e->op = op;
+        e->ch = {a, b};
+        return e;
+    }
+    ExprP expr()
+    {
+        ExprP l = term();
+        if (!l) return nullptr;
+        for (;;) {
+            char c = lx.peek();
+            if (c == '+' || c == '-') {
+                lx.i++;
+                ExprP r = term();
+                if (!r) return nullptr;
+                l = calc(c == '+' ? Op::Add : Op::Sub, l, r);
+            } else return l;
+        }
+    }
+    ExprP term()
+    {
+        ExprP l = unary();
+        if (!l) return nullptr;
+        for (;;) {
+            char c = lx.peek();
+            if (c == '*' || c == '/' || c == '%') {
+                lx.i++;
+                ExprP r = unary();
+                if (!r) return nullptr;
+                l = calc(c == '*' ? Op::Mul : c == '/' ? Op::Div : Op::Mod, l, r);
+            } else return l;
+        }
+    }
+    ExprP unary()
+    {
+        if (lx.eat('-')) {
+            ExprP e = unary();
+            if (!e) return nullptr;
+            if (e->kind == NK::Int) e->ival = -e->ival;
+            else if (e->kind == NK::Dec) e->dval.sign ^= 1u;
+            else e->reverse = !e->reverse;
+            return e;
+        }
+        return primary();
+    }
+    ExprP primary()
+    {
+        char c = lx.peek();
+        if (c == '(') {
+            lx.i++;
+            ExprP e = expr();
+            if (!e) return nullptr;
+            if (!lx.eat(')')) { err = "expected ')'"; return nullptr; }
+            return e;
+        }
+        if (isdigit((unsigned char)c) || c == '.') {
+            size_t st = lx.i;
+            bool dot = false;
+            while (lx.i < lx.s.size() && (isdigit((unsigned char)lx.s[lx.i]) || lx.s[lx.i] == '.')) {
+                if (lx.s[lx.i] == '.') dot = true;
+                lx.i++;
+            }
+            std::string lit = lx.s.substr(st, lx.i - st);
+            auto e = std::make_shared<Expr>();
+            if (dot) {
+                e->kind = NK::Dec;
+                e->dval = HostDec::from_string(lit, mode);
+            } else {
+                e->kind = NK::Int;
+                e->ival = std::stoll(lit);
+            }
+            return e;
+        }
+        if (isalpha((unsigned char)c) || c == '_') {
+            size_t st = lx.i;
+            while (lx.i < lx.s.size() && (isalnum((unsigned char)lx.s[lx.i]) || lx.s[lx.i] == '_')) lx.i++;
+            std::string id = lx.s.substr(st, lx.i - st);
+            for (size_t k = 0; k < names.size(); k++) {
+                if (names[k] == id) {
+                    auto e = std::make_shared<Expr>();
+                    e->kind = NK::Column;
+                    e->col = (int)k;
+                    e->P = types[k].P;
+                    e->S = types[k].S;
+                    return e;
+                }
+            }
+            err = "unknown column '" + id + "'";
+            return nullptr;
+        }
+        err = std::string("unexpected character '") + (c ? c : '?') + "'";
+        return nullptr;
+    }
+};
+}  // namespace
+
+ExprP parse_expr(const std::string& text, const std::vector<std::string>& names, const std::vector<ColumnType>& types,
+                 Mode mode, std::string* err)
+{
+    Parser p(text, names, types, mode);
+    ExprP e = p.expr();
+    if (e && p.lx.peek() != 0) { p.err = "trailing input"; e = nullptr; }
+    if (!e && err) *err = p.err.empty() ? "parse error" : p.err;
+    if (e) infer_frontend_types(*e, mode);
+    return e;
+}
+
+static int digits_of(int64_t v)
+{
+    HostDec d = HostDec::from_int(v, 10);
+    d.check_and_set_real_precision();
+    return d.prec;
+}
+
+void infer_frontend_types(Expr& e, Mode mode)
+{
+    switch (e.kind) {
+    case NK::Column: return;
+    case NK::Int: e.P = digits_of(e.ival); e.S = 0; return;
+    case NK::Dec: {
+        HostDec t = e.dval;
+        t.check_and_set_real_precision();
+        e.P = t.prec;
+        e.S = t.frac;
+        return;
+    }
+    case NK::Calc: break;
+    }
+    for (auto& c : e.ch) infer_frontend_types(*c, mode);
+    int Pa = e.ch[0]->P, Sa = e.ch[0]->S, Pb = e.ch[1]->P, Sb = e.ch[1]->S;
+    int P = 0, S = 0;
+    switch (e.op) {
+    case Op::Add:
+    case Op::Sub: P = std::max(Pa - Sa, Pb - Sb) + 1; S = std::max(Sa, Sb); P += S; break;
+    case Op::Mul: S = Sa + Sb; P = Pa + Pb; break;
+    case Op::Div: P = Pa - Pb + Sb + 4 + 1; S = Sa + 4; break;
+    case Op::Mod: P = Pb; S = 0; break;
+    }
+    if (mode == Mode::Faithful) { P &= 0xff; S &= 0xff; }  // Decimal::prec/frac are uint8
+    e.P = P;
+    e.S = S;
+}
+
+// ------------------------------------------------------------------ AlignOptimize
+
+namespace {
+
+struct BT;
+struct MT;
+struct Arena {
+    std::vector<std::unique_ptr<BT>> bts;
+    std::vector<std::unique_ptr<MT>> mts;
+    BT* bt();
+    MT* mt();
+};
+
+struct BT {
+    NK kind = NK::Column;
+    Op op = Op::Add;
+    int col = -1;
+    int64_t ival = 0;
+    HostDec dval;
+    int P = 0, S = 0;
+    bool reverse = false;
+    std::vector<BT*> ch;
+};
+
+struct MT {
+    NK kind = NK::Column;
+    Op op = Op::Add;
+    int col = -1;
+    int64_t ival = 0;
+    HostDec dval;
+    int P = 0, S = 0;
+    bool reverse = false;
+    BT* root = nullptr;
+    std::vector<MT*> ch;
+};
+
+BT* Arena::bt() { bts.emplace_back(new BT()); return bts.back().get(); }
+MT* Arena::mt() { mts.emplace_back(new MT()); return mts.back().get(); }
+
+BT* clone_bt(Arena& A, const Expr& e)
+{
+    BT* b = A.bt();
+    b->kind = e.kind; b->op = e.op; b->col = e.col; b->ival = e.ival; b->dval = e.dval;
+    b->P = e.P; b->S = e.S; b->reverse = e.reverse;
+    for (auto& c : e.ch) b->ch.push_back(clone_bt(A, *c));
+    return b;
+}
+
+MT* mt_from_bt(Arena& A, BT* b)
+{
+    MT* m = A.mt();
+    m->root = b; m->kind = b->kind; m->op = b->op; m->col = b->col; m->ival = b->ival; m->dval = b->dval;
+    m->P = b->P; m->S = b->S; m->reverse = b->reverse;
+    return m;
+}
+
+// ModifySubOperation (AriesEngineShell.cpp:244-281)
+void modify_sub(BT* t)
+{
+    std::queue<BT*> q;
+    q.push(t);
+    while (!q.empty()) {
+        BT* n = q.front();
+        if (n->kind == NK::Calc) {
+            if (n->op == Op::Mul && n->reverse) n->ch[0]->reverse = !n->ch[0]->reverse;
+            else if (n->op == Op::Div && n->reverse) n->ch[0]->reverse = !n->ch[0]->reverse;
+            else if (n->op == Op::Add && n->reverse) {
+                n->ch[0]->reverse = !n->ch[0]->reverse;
+                n->ch[1]->reverse = !n->ch[1]->reverse;
+            } else if (n->op == Op::Sub) {
+                if (!n->reverse) n->ch[1]->reverse = !n->ch[1]->reverse;
+                if (n->reverse) n->ch[0]->reverse = !n->ch[0]->reverse;
+                n->op = Op::Add;
+            }
+            if (n->ch.size() == 2) { q.push(n->ch[0]); q.push(n->ch[1]); }
+        }
+        q.pop();
+    }
+}
+
+// getChild (AriesEngineShell.cpp:284-322)
+std::vector<MT*> get_child(Arena& A, BT* b, Op sym)
+{
+    std::vector<MT*> out;
+    if (sym == Op::Div) {
+        out.push_back(mt_from_bt(A, b->ch[0]));
+        out.push_back(mt_from_bt(A, b->ch[1]));
+        return out;
+    }
+    for (int k = 0; k < 2; k++) {
+        BT* c = b->ch[k];
+        if (c->kind != NK::Calc) out.push_back(mt_from_bt(A, c));
+        else if (c->op == sym) {
+            auto nx = get_child(A, c, sym);
+            out.insert(out.end(), nx.begin(), nx.end());
+        } else out.push_back(mt_from_bt(A, c));
+    }
+    return out;
+}
+
+// BinToMultiTree (AriesEngineShell.cpp:325-342)
+void bin_to_multi(Arena& A, BT* b, MT* m)
+{
+    m->ch = get_child(A, b, m->root->op);
+    for (MT* c : m->ch)
+        if (c->root && c->root->kind == NK::Calc && c->root->ch.size() == 2) bin_to_multi(A, c->root, c);
+}
+
+// CmpByScale (AriesEngineShell.cpp:344-358)
+bool cmp_by_scale(const MT* a, const MT* b)
+{
+    if (a->kind == NK::Int) return true;
+    if (b->kind == NK::Int) return false;
+    if (a->kind == NK::Dec && b->kind == NK::Dec) return a->S < b->S;
+    if (a->kind == NK::Dec) return true;
+    if (b->kind == NK::Dec) return false;
+    return a->S < b->S;
+}
+
+// std::sort on <= 16 elements is an insertion sort (libstdc++
+// __insertion_sort); reproduced so that equal-scale operands keep the order
+// the reference gives them.
+void insertion_sort(std::vector<MT*>& v)
+{
+    if (v.size() > 16) { std::stable_sort(v.begin(), v.end(), cmp_by_scale); return; }
+    for (size_t i = 1; i < v.size(); i++) {
+        MT* val = v[i];
+        if (cmp_by_scale(val, v[0])) {
+            for (size_t k = i; k > 0; k--) v[k] = v[k - 1];
+            v[0] = val;
+        } else {
+            size_t k = i;
+            while (cmp_by_scale(val, v[k - 1])) { v[k] = v[k - 1]; k--; }
+            v[k] = val;
+        }
+    }
+}
+
+bool is_decimal_calc(const MT* m)
+{
+    // only CALC nodes of decimal result type are rewritten; a tree with a
+    // division or a decimal operand anywhere below is decimal-typed
+    std::function<bool(const BT*)> has_dec = [&](const BT* b) -> bool {
+        if (b->kind == NK::Column || b->kind == NK::Dec) return true;
+        if (b->kind == NK::Calc && b->op == Op::Div) return true;
+        for (auto* c : b->ch)
+            if (has_dec(c)) return true;
+        return false;
+    };
+    return m->kind == NK::Calc && m->root && has_dec(m->root);
+}
+
+// AlignOptMultiTree (AriesEngineShell.cpp:360-597)
+bool align_opt(Arena& A, MT* top, Mode mode, std::string* err)
+{
+    std::queue<MT*> q;
+    q.push(top);
+    while (!q.empty()) {
+        MT* node = q.front();
+        if (node->kind == NK::Calc && is_decimal_calc(node)) {
+            if (node->op == Op::Add) {
+                insertion_sort(node->ch);
+                int nodeInt = 0;
+                HostDec nodeDec = HostDec::from_int(0, 10);
+                size_t index = 0;
+                for (index = 0; index < node->ch.size(); index++) {
+                    MT* c = node->ch[index];
+                    if (c->kind == NK::Int) {
+                        int t = (int)c->ival;
+                        if (c->reverse) t = -t;
+                        nodeInt += t;
+                    } else if (c->kind == NK::Dec) {
+                        HostDec d = c->dval;
+                        if (c->reverse) d.sign = 1;
+                        nodeDec.add(d, mode);
+                    } else break;
+                }
+                if (index == 0)
+                    for (MT* c : node->ch)
+                        if (c->kind == NK::Calc) q.push(c);
+                std::vector<MT*> nl;
+                if (nodeInt == 0 && nodeDec.is_zero()) {
+                    if (index == node->ch.size()) { *err = "addition of constants only"; return false; }
+                    if (index == node->ch.size() - 1) {
+                        // CopyNodeInfo: the single remaining operand replaces the node
+                        MT* c = node->ch[index];
+                        node->kind = c->kind; node->op = c->op; node->col = c->col; node->ival = c->ival;
+                        node->dval = c->dval; node->P = c->P; node->S = c->S; node->reverse = c->reverse;
+                        node->ch = c->ch;
+                        node->root = c->root;
+                    } else {
+                        for (; index < node->ch.size(); index++) {
+                            nl.push_back(node->ch[index]);
+                            if (node->ch[index]->kind == NK::Calc) q.push(node->ch[index]);
+                        }
+                        node->ch = nl;
+                    }
+                } else {
+                    if (nodeDec.is_zero()) {
+                        if (index >= node->ch.size()) { *err = "addition of constants only"; return false; }
+                        MT* nn = A.mt();
+                        int nearScale = node->ch[index]->S;
+                        nn->kind = NK::Dec;
+                        nn->S = nearScale;
+                        nn->P = 0;
+                        nn->dval = HostDec::from_int_scale(nodeInt, nearScale, mode);
+                        nl.push_back(nn);
+                        for (; index < node->ch.size(); index++) {
+                            nl.push_back(node->ch[index]);
+                            if (node->ch[index]->kind == NK::Calc) q.push(node->ch[index]);
+                        }
+                    } else {
+                        bool optInsert = true;
+                        if (nodeInt != 0) nodeDec.add(HostDec::from_int(nodeInt, 10), mode);
+                        for (; index < node->ch.size(); index++) {
+                            if (node->ch[index]->S > nodeDec.frac && optInsert) {
+                                optInsert = false;
+                                HostDec d = nodeDec.rescaled_up(node->ch[index]->S, mode);
+                                MT* nn = A.mt();
+                                nn->kind = NK::Dec;
+                                nn->S = d.frac;
+                                nn->P = 0;
+                                nn->dval = d;
+                                nl.push_back(nn);
+                            }
+                            nl.push_back(node->ch[index]);
+                            if (node->ch[index]->kind == NK::Calc) q.push(node->ch[index]);
+                        }
+                        if (optInsert) {
+                            MT* nn = A.mt();
+                            nn->kind = NK::Dec;
+                            nn->P = nodeDec.frac;
+                            nn->S = nodeDec.frac;
+                            nn->dval = nodeDec;
+                            nl.push_back(nn);
+                        }
+                    }
+                    node->ch = nl;
+                }
+            } else if (node->op == Op::Mul) {
+                int nodeInt = 1;
+                HostDec nodeDec = HostDec::from_int(1, 10);
+                std::vector<MT*> nl;
+                for (MT* c : node->ch) {
+                    if (c->kind == NK::Int) nodeInt *= (int)c->ival;
+                    else if (c->kind == NK::Dec) nodeDec.mul(c->dval, mode);
+                    else nl.push_back(c);
+                }
+                if (nodeInt != 1 || !nodeDec.equals_one()) {
+                    if (nodeInt == 0 || nodeDec.is_zero()) {
+                        node->kind = NK::Int;
+                        node->ival = 0;
+                    } else {
+                        nodeDec.mul(HostDec::from_int(nodeInt, 10), mode);
+                        if (!nodeDec.equals_one()) {
+                            MT* nn = A.mt();
+                            nn->kind = NK::Dec;
+                            nn->S = nodeDec.frac;
+                            nn->P = 0;
+                            nn->dval = nodeDec;
+                            nl.push_back(nn);
+                        }
+                        node->ch = nl;
+                    }
+                }
+                if (node->kind == NK::Calc)
+                    for (MT* c : node->ch)
+                        if (c->kind == NK::Calc) q.push(c);
+            } else if (node->op == Op::Div || node->op == Op::Mod) {
+                if (node->ch[0]->kind == NK::Int && node->ch[0]->ival == 0) {
+                    node->kind = NK::Int;
+                    node->ival = 0;
+                } else {
+                    for (size_t i = 0; i < node->ch.size(); i++) {
+                        MT* c = node->ch[i];
+                        if (c->kind == NK::Calc) q.push(c);
+                        if (c->kind == NK::Int) {
+                            bool fits32 = c->ival >= INT32_MIN && c->ival <= INT32_MAX;
+                            HostDec d = HostDec::from_int(c->ival, fits32 ? 10 : 19);
+                            MT* nn = A.mt();
+                            nn->kind = NK::Dec;
+                            nn->P = d.prec;
+                            nn->S = d.frac;
+                            nn->dval = d;
+                            node->ch[i] = nn;
+                        }
+                    }
+                }
+            } else {
+                for (MT* c : node->ch)
+                    if (c->kind == NK::Calc) q.push(c);
+            }
+        }
+        q.pop();
+    }
+    return true;
+}
+
+// RemoveZeroNode (AriesEngineShell.cpp:184-219)
+MT* remove_zero(MT* m)
+{
+    std::vector<MT*> nl;
+    for (size_t i = 0; i < m->ch.size(); i++) {
+        MT* c = m->ch[i];
+        if (c->kind == NK::Int) {
+            if (m->op == Op::Mul) { m->kind = NK::Int; m->ival = 0; return m; }
+        } else if (c->kind == NK::Calc) {
+            m->ch[i] = remove_zero(c);
+            if (m->ch[i]->kind == NK::Int) {
+                if (m->op == Op::Mul) { m->kind = NK::Int; m->ival = 0; return m; }
+            } else nl.push_back(m->ch[i]);
+        } else nl.push_back(c);
+    }
+    m->ch = nl;
+    return m;
+}
+
+BT* leaf_from_mt(Arena& A, const MT* m)
+{
+    BT* b = A.bt();
+    b->kind = m->kind; b->op = m->op; b->col = m->col; b->ival = m->ival; b->dval = m->dval;
+    b->P = m->P; b->S = m->S; b->reverse = m->reverse;
+    return b;
+}
+
+// ListToBinTree (AriesEngineShell.cpp:90-144): left-deep chain with the
+// per-node result types.  Types are uint16 like AriesDataType.
+BT* list_to_bin(Arena& A, std::vector<BT*>& L, int index, Op op)
+{
+    if (index == 0) return L[0];
+    BT* l = list_to_bin(A, L, index - 1, op);
+    BT* r = L[index];
+    BT* t = A.bt();
+    t->kind = NK::Calc;
+    int P0 = l->P, S0 = l->S, P1 = r->P, S1 = r->S;
+    if (op == Op::Add) {
+        if (r->reverse) { r->reverse = false; t->op = Op::Sub; }
+        else t->op = Op::Add;
+        int P = (uint16_t)(std::max(P0 - S0, P1 - S1) + 1);
+        int S = std::max(S0, S1);
+        t->P = (uint16_t)(P + S);
+        t->S = S;
+    } else if (op == Op::Mul) {
+        t->op = Op::Mul;
+        int P = (uint16_t)((P0 - S0) + (P1 - S1));
+        int S = (uint16_t)(S0 + S1);
+        t->P = (uint16_t)(P + S);
+        t->S = S;
+    } else if (op == Op::Div) {
+        t->op = Op::Div;
+        t->P = (uint16_t)(P0 - P1 + S1 + 4 + 1);
+        t->S = (uint16_t)(S0 + 4);
+    } else {
+        t->op = Op::Mod;
+        t->P = P1;
+        t->S = 0;
+    }
+    t->ch = {l, r};
+    return t;
+}
+
+// MultiToBinTree (AriesEngineShell.cpp:146-182)
+BT* multi_to_bin(Arena& A, MT* m)
+{
+    std::vector<BT*> tmp;
+    for (MT* c : m->ch) {
+        if (c->kind != NK::Calc) tmp.push_back(leaf_from_mt(A, c));
+        else tmp.push_back(multi_to_bin(A, c));
+    }
+    if (m->ch.empty()) return leaf_from_mt(A, m);
+    if (m->kind == NK::Calc && m->ch.size() == 1) {
+        if (m->ch[0]->kind == NK::Calc) return list_to_bin(A, tmp, (int)tmp.size() - 1, m->op);
+        return leaf_from_mt(A, m->ch[0]);
+    }
+    return list_to_bin(A, tmp, (int)tmp.size() - 1, m->op);
+}
+
+ExprP to_expr(const BT* b)
+{
+    auto e = std::make_shared<Expr>();
+    e->kind = b->kind; e->op = b->op; e->col = b->col; e->ival = b->ival; e->dval = b->dval;
+    e->P = b->P; e->S = b->S; e->reverse = b->reverse;
+    for (auto* c : b->ch) e->ch.push_back(to_expr(c));
+    return e;
+}
+
+}  // namespace
+
+ExprP align_optimize(const ExprP& e, Mode mode)
+{
+    if (!e || e->kind != NK::Calc) return e;
+    Arena A;
+    BT* bt = clone_bt(A, *e);
+    modify_sub(bt);
+    MT* mt = mt_from_bt(A, bt);
+    bin_to_multi(A, bt, mt);
+    std::string err;
+    if (!align_opt(A, mt, mode, &err)) return nullptr;
+    mt = remove_zero(mt);
+    if (mt->kind == NK::Int) return nullptr;
+    BT* out = multi_to_bin(A, mt);
+    // AlignOptimizeNode (AriesEngineShell.cpp:628-644): root precision capped at 288
+    if (mode == Mode::Faithful && out->P > 288) out->P = 288;
+    return to_expr(out);
+}
+
+// ------------------------------------------------------------------ plan
+
+static int words_for(int bits) { return std::max(1, (bits + 31) / 32); }
+
+bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode, Plan* out, std::string* err)
+{
+    Plan p;
+    p.mode = mode;
+    std::function<int(const Expr&)> rec = [&](const Expr& e) -> int {
+        PNode n{};
+        int id = -1;
+        if (e.kind == NK::Column) {
+            n.kind = PK::Col;
+            n.col = e.col;
+            n.P = cols[e.col].P;
+            n.S = cols[e.col].S;
+            n.bits = 8 * compact_bytes(n.P) - 1;
+            if (cols[e.col].bits > 0 && cols[e.col].bits < n.bits) n.bits = cols[e.col].bits;
+        } else if (e.kind == NK::Dec) {
+            n.kind = PK::Const;
+            n.k = e.dval;
+            n.P = e.P;
+            n.S = e.dval.frac;
+            if (e.S != e.dval.frac) { *err = "decimal constant whose scale differs from its type"; return -1; }
+            n.bits = std::max(1, e.dval.bitlen());
+        } else if (e.kind == NK::Int) {
+            *err = "integer operand left in a decimal calculation after rewrite";
+            return -1;
+        } else {
+            int a = rec(*e.ch[0]);
+            if (a < 0) return -1;
+            int b = rec(*e.ch[1]);
+            if (b < 0) return -1;
+            const PNode& A = p.n[a];
+            const PNode& B = p.n[b];
+            n.a = a;
+            n.b = b;
+            n.P = e.P;
+            switch (e.op) {
+            case Op::Add:
+            case Op::Sub: {
+                n.kind = e.op == Op::Add ? PK::Add : PK::Sub;
+                n.S = std::max(A.S, B.S);
+                int ba = A.bits + (n.S > A.S ? pow10_bits(n.S - A.S) : 0);
+                int bb = B.bits + (n.S > B.S ? pow10_bits(n.S - B.S) : 0);
+                n.bits = std::max(ba, bb) + 1;
+                break;
+            }
+            case Op::Mul:
+                n.kind = PK::Mul;
+                n.S = A.S + B.S;
+                n.bits = A.bits + B.bits;
+                break;
+            case Op::Div:
+                n.kind = PK::Div;
+                n.S = A.S + 4;
+                n.bits = std::max(64, A.bits + pow10_bits(B.S + 4) + 1);
+                break;
+            case Op::Mod: *err = "MOD is not implemented on the HIP path"; return -1;
+            }
+            if (n.S != e.S) { *err = "internal: runtime scale differs from node type"; return -1; }
+        }
+        if (mode == Mode::Faithful) n.bits = std::min(n.bits, 1024);
+        n.W = words_for(n.bits);
+        p.n.push_back(n);
+        id = (int)p.n.size() - 1;
+        if (e.reverse) {
+            PNode g{};
+            g.kind = PK::Neg;
+            g.a = id;
+            g.P = n.P;
+            g.S = n.S;
+            g.bits = n.bits;
+            g.W = n.W;
+            p.n.push_back(g);
+            id = (int)p.n.size() - 1;
+        }
+        return id;
+    };
+    if (!rw) { *err = "expression rewrites to a constant or integer"; return false; }
+    int r = rec(*rw);
+    if (r < 0) return false;
+    p.root = r;
+    p.outP = rw->P;
+    p.outS = rw->S;
+    p.outB = compact_bytes(p.outP);
+    *out = std::move(p);
+    return true;
+}
+
+std::string Plan::text() const
+{
+    std::function<std::string(int)> rec = [&](int id) -> std::string {
+        const PNode& x = n[id];
+        char buf[64];
+        switch (x.kind) {
+        case PK::Col: snprintf(buf, sizeof buf, "c%d:%d:%d", x.col, x.P, x.S); return buf;
+        case PK::Const: return std::string("k") + (x.k.sign ? "-" : "+") + x.k.hex() + ":" + std::to_string(x.S);
+        case PK::Neg: return "(NEG " + rec(x.a) + ")";
+        default: break;
+        }
+        const char* op = x.kind == PK::Add ? "ADD" : x.kind == PK::Sub ? "SUB" : x.kind == PK::Mul ? "MUL" : "DIV";
+        snprintf(buf, sizeof buf, "(%s %d %d ", op, x.P, x.S);
+        return buf + rec(x.a) + " " + rec(x.b) + ")";
+    };
+    return rec(root);
+}
+
+bool plan_is_faithful_envelope(const Plan& p, const std::vector<ColumnType>& cols)
+{
+    for (auto& c : cols)
+        if (c.P > 299 || c.P < 0 || c.S > 255) return false;
+    for (const PNode& x : p.n) {
+        if (x.S > 255) return false;
+        if (x.kind == PK::Add || x.kind == PK::Sub) {
+            if (x.S - p.n[x.a].S > 38 || x.S - p.n[x.b].S > 38) return false;
+        }
+        if (x.kind == PK::Div) {
+            int s = p.n[x.b].S + 4;
+            if (s > 38) return false;
+            // the reference divides in one chunk when the dividend leaves room
+            // for s more decimal digits (decimal.cu:2283-2300)
+            int bitsA = std::min(p.n[x.a].bits, 1024);
+            int dtz = (bitsA - 1) / 32;
+            if ((32 - dtz) * 9 - 10 < s) return false;
+        }
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ code generation
+
+namespace {
+
+struct Emit {
+    std::ostringstream os;
+    int tmp = 0;
+    std::string fresh(const char* p) { return std::string(p) + std::to_string(tmp++); }
+};
+
+// Emit straight-line code for one plan; returns the variable of the root.
+// Column loads are shared across plans through `loaded`.
+std::string emit_plan(Emit& E, const Plan& p, const std::vector<ColumnType>& cols, std::vector<std::string>& loaded,
+                      const std::string& ind, const std::string& prefix)
+{
+    const bool faith = p.mode == Mode::Faithful;
+    std::vector<std::string> var(p.n.size());
+    for (size_t i = 0; i < p.n.size(); i++) {
+        const PNode& x = p.n[i];
+        std::string v = prefix + "n" + std::to_string(i);
+        var[i] = v;
+        switch (x.kind) {
+        case PK::Col: {
+            int c = x.col;
+            int B = compact_bytes(cols[c].P);
+            if (loaded[c].empty()) {
+                loaded[c] = "c" + std::to_string(c);
+                E.os << ind << "const Dec<" << (B + 3) / 4 << "> " << loaded[c] << " = load_compact<" << B
+                     << ">(a.col[" << c << "], row, a.colbytes[" << c << "]);\n";
+            }
+            E.os << ind << "const Dec<" << x.W << "> " << v << " = dresize<" << x.W << ">(" << loaded[c] << ");\n";
+            break;
+        }
+        case PK::Const: {
+            E.os << ind << "Dec<" << x.W << "> " << v << ";\n";
+            for (int w = 0; w < x.W; w++) E.os << ind << v << ".m.w[" << w << "] = 0x" << std::hex << x.k.v[w] << std::dec << "u;\n";
+            E.os << ind << v << ".neg = " << x.k.sign << "u;\n";
+            break;
+        }
+        case PK::Neg:
+            E.os << ind << "const Dec<" << x.W << "> " << v << " = dneg<" << x.W << ">(dresize<" << x.W << ">(" << var[x.a]
+                 << "));\n";
+            break;
+        case PK::Add:
+        case PK::Sub: {
+            const PNode& A = p.n[x.a];
+            const PNode& B = p.n[x.b];
+            auto aligned = [&](const PNode& c, const std::string& cv) -> std::pair<std::string, int> {
+                if (c.S == x.S) return {cv, c.W};
+                int sh = x.S - c.S;
+                int bits = c.bits + pow10_bits(sh);
+                if (faith) bits = std::min(bits, 1024);
+                int w = words_for(bits);
+                std::string t = E.fresh((prefix + "al").c_str());
+                E.os << ind << "const Dec<" << w << "> " << t << " = dscale<" << w << ", " << c.W << ", " << sh << ">(" << cv
+                     << ");\n";
+                return {t, w};
+            };
+            auto pa = aligned(A, var[x.a]);
+            auto pb = aligned(B, var[x.b]);
+            E.os << ind << "const Dec<" << x.W << "> " << v << " = " << (x.kind == PK::Add ? "dadd" : "dsub") << "<" << x.W
+                 << ", " << pa.second << ", " << pb.second << ">(" << pa.first << ", " << pb.first << ");\n";
+            break;
+        }
+        case PK::Mul:
+            E.os << ind << "const Dec<" << x.W << "> " << v << " = dmul<" << x.W << ", " << p.n[x.a].W << ", " << p.n[x.b].W
+                 << ">(" << var[x.a] << ", " << var[x.b] << ");\n";
+            break;
+        case PK::Div:
+            E.os << ind << "const Dec<" << x.W << "> " << v << " = ddiv<" << x.W << ", " << p.n[x.a].W << ", " << p.n[x.b].W
+                 << ", " << (p.n[x.b].S + 4) << ", " << (faith ? 1 : 0) << ">(" << var[x.a] << ", " << var[x.b] << ");\n";
+            break;
+        }
+    }
+    return var[p.root];
+}
+
+const char* kArgsStruct =
+    "#define UPD_MAXCOLS 16\n"
+    "struct ColArgs { const u8* col[UPD_MAXCOLS]; usize colbytes[UPD_MAXCOLS]; usize rows; u8* out; };\n"
+    "struct AggArgs { const u8* col[UPD_MAXCOLS]; usize colbytes[UPD_MAXCOLS]; usize rows;\n"
+    "                 const i32* fcol; const u8* key0; const u8* key1; const u8* lut; u32* partials; };\n";
+
+}  // namespace
+
+std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, const std::string& name, const GenOptions& o)
+{
+    Emit E;
+    const int T = o.block;
+    const int OB = p.outB;
+    E.os << "#include \"dec_device.h\"\nusing namespace updec;\n" << kArgsStruct;
+    E.os << "// plan: " << p.text() << "\n";
+    E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name << "(ColArgs a)\n{\n";
+    E.os << "  __shared__ u32 tile[(" << T << " * " << OB << " + 3) / 4];\n";
+    E.os << "  u8* t8 = reinterpret_cast<u8*>(tile);\n";
+    E.os << "  for (usize base = (usize)blockIdx.x * " << T << "; base < a.rows; base += (usize)gridDim.x * " << T << ") {\n";
+    E.os << "    const usize row = base + threadIdx.x;\n";
+    E.os << "    if (row < a.rows) {\n";
+    std::vector<std::string> loaded(cols.size());
+    std::string rv = emit_plan(E, p, cols, loaded, "      ", "");
+    E.os << "      const Mag<" << (OB + 3) / 4 << "> enc = encode_compact<" << OB << ", " << p.n[p.root].W << ">(" << rv << ");\n";
+    E.os << "#pragma unroll\n      for (int b = 0; b < " << OB << "; b++) t8[threadIdx.x * " << OB
+         << " + b] = (u8)(enc.w[b >> 2] >> (8 * (b & 3)));\n";
+    E.os << "    }\n    __syncthreads();\n";
+    E.os << "    const usize n = (a.rows - base < " << T << " ? a.rows - base : " << T << ") * " << OB << ";\n";
+    E.os << "    u8* dst = a.out + base * " << OB << ";\n";
+    E.os << "    for (usize i = threadIdx.x; i < n / 4; i += " << T << ") reinterpret_cast<u32*>(dst)[i] = tile[i];\n";
+    E.os << "    for (usize i = (n / 4) * 4 + threadIdx.x; i < n; i += " << T << ") dst[i] = t8[i];\n";
+    E.os << "    __syncthreads();\n  }\n}\n";
+    return E.os.str();
+}
+
+std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& cols, const std::string& name,
+                            const GenOptions& o, int* acc_words)
+{
+    Emit E;
+    const int T = o.block;
+    const int NP = (int)s.plans.size();
+    const int NG = s.ngroups;
+    // value bound after the compact round trip of the materialized result
+    int vbits = 1;
+    std::vector<int> pb(NP);
+    for (int i = 0; i < NP; i++) {
+        const Plan& p = s.plans[i];
+        pb[i] = std::min(p.n[p.root].bits, 8 * p.outB - 1);
+        vbits = std::max(vbits, pb[i]);
+    }
+    // per-lane accumulators hold at most 2^16 rows (the launch guarantees
+    // it); block and grid partials hold up to 2^40 rows
+    const int AW1 = words_for(vbits + 16 + 1);
+    const int AW2 = words_for(vbits + 40 + 1);
+    *acc_words = AW2;
+    const bool lds_slots = NG > 1;
+    const int PW = NP * NG * AW2 + 2 * NG;  // partial words per block
+    const char* fops[] = {"<=", "<", ">=", ">", "==", "!="};
+
+    E.os << "#include \"dec_device.h\"\nusing namespace updec;\n" << kArgsStruct;
+    for (int i = 0; i < NP; i++) E.os << "// plan " << i << ": " << s.plans[i].text() << "\n";
+    E.os << "#define NP " << NP << "\n#define NG " << NG << "\n#define AW1 " << AW1 << "\n#define AW2 " << AW2 << "\n#define T "
+         << T << "\n#define PW " << PW << "\n";
+    // round trip through the materialized compact column (the reference sums
+    // the stored compact result, AriesExprCalcNode.cpp:1623-1700)
+    E.os << "template <int B, int W> UPD_DEV Dec<W> stored(const Dec<W>& v) {\n"
+            "  Dec<W> r = v;\n"
+            "  constexpr int TOPBIT = 8 * B - 1;\n"
+            "#pragma unroll\n"
+            "  for (int i = 0; i < W; i++) {\n"
+            "    if (32 * i >= TOPBIT + 1) r.m.w[i] = 0u;\n"
+            "    else if (32 * i + 32 > TOPBIT) {\n"
+            "      const int bit = TOPBIT - 32 * i;\n"
+            "      r.neg |= (v.m.w[i] >> bit) & 1u;\n"
+            "      r.m.w[i] &= (bit == 0) ? 0u : (0xffffffffu >> (32 - bit));\n"
+            "    }\n"
+            "  }\n"
+            "  return r;\n"
+            "}\n";
+    E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name << "_main(AggArgs a)\n{\n";
+    E.os << "  const u32 tid = threadIdx.x;\n";
+    if (lds_slots) {
+        E.os << "  __shared__ u32 slot[NG * NP * AW1 * T];\n  __shared__ u32 cslot[NG * T];\n";
+        E.os << "  for (u32 i = tid; i < NG * NP * AW1 * T; i += T) slot[i] = 0u;\n";
+        E.os << "  for (u32 i = tid; i < NG * T; i += T) cslot[i] = 0u;\n";
+        E.os << "  __syncthreads();\n";
+    } else {
+        for (int i = 0; i < NP; i++) E.os << "  Mag<AW1> acc" << i << " = mzero<AW1>();\n";
+        E.os << "  u32 cnt = 0;\n";
+    }
+    E.os << "  for (usize row = (usize)blockIdx.x * T + tid; row < a.rows; row += (usize)gridDim.x * T) {\n";
+    if (s.filter_col >= 0) E.os << "    if (!(a.fcol[row] " << fops[s.filter_op] << " (i64)" << s.filter_val << "LL)) continue;\n";
+    if (s.nkeys == 2) E.os << "    const u32 g = a.lut[((u32)a.key0[row] << 8) | a.key1[row]];\n";
+    else if (s.nkeys == 1) E.os << "    const u32 g = a.lut[a.key0[row]];\n";
+    else E.os << "    const u32 g = 0;\n";
+    std::vector<std::string> loaded(cols.size());
+    for (int i = 0; i < NP; i++) {
+        const Plan& p = s.plans[i];
+        std::string rv = emit_plan(E, p, cols, loaded, "    ", "p" + std::to_string(i) + "_");
+        E.os << "    {\n      const Dec<" << p.n[p.root].W << "> v = stored<" << p.outB << ", " << p.n[p.root].W << ">(" << rv
+             << ");\n";
+        if (lds_slots) {
+            E.os << "      u32* sp = slot + ((g * NP + " << i << ") * AW1) * T + tid;\n";
+            E.os << "      Mag<AW1> s;\n#pragma unroll\n      for (int w = 0; w < AW1; w++) s.w[w] = sp[w * T];\n";
+            E.os << "      acc_add<AW1, " << p.n[p.root].W << ">(s, v);\n";
+            E.os << "#pragma unroll\n      for (int w = 0; w < AW1; w++) sp[w * T] = s.w[w];\n";
+        } else {
+            E.os << "      acc_add<AW1, " << p.n[p.root].W << ">(acc" << i << ", v);\n";
+        }
+        E.os << "    }\n";
+    }
+    if (lds_slots) E.os << "    cslot[g * T + tid] += 1u;\n";
+    else E.os << "    cnt += 1u;\n";
+    E.os << "  }\n";
+    // block reduction: each wave reduces, wave leaders combine through LDS
+    E.os << "  __shared__ u32 red[(T / 64) * (NP * NG * AW2 + 2 * NG)];\n";
+    E.os << "  const u32 wave = tid >> 6, lane = tid & 63u;\n";
+    if (lds_slots) E.os << "  __syncthreads();\n";
+    E.os << "#pragma unroll\n  for (int gg = 0; gg < NG; gg++) {\n";
+    E.os << "#pragma unroll\n    for (int pp = 0; pp < NP; pp++) {\n";
+    E.os << "      Mag<AW1> s1;\n";
+    if (lds_slots)
+        E.os << "#pragma unroll\n      for (int w = 0; w < AW1; w++) s1.w[w] = slot[((gg * NP + pp) * AW1 + w) * T + tid];\n";
+    else {
+        E.os << "      s1 = mzero<AW1>();\n";
+        for (int i = 0; i < NP; i++) E.os << "      if (pp == " << i << ") s1 = acc" << i << ";\n";
+    }
+    E.os << "      Mag<AW2> s2;\n      const u32 ext = (s1.w[AW1 - 1] >> 31) ? 0xffffffffu : 0u;\n";
+    E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) s2.w[w] = w < AW1 ? s1.w[w] : ext;\n";
+    E.os << "      wave_reduce<AW2>(s2);\n";
+    E.os << "      if (lane == 0) {\n#pragma unroll\n        for (int w = 0; w < AW2; w++) red[wave * PW + (gg * NP + pp) * AW2 + w] = s2.w[w];\n      }\n";
+    E.os << "    }\n";
+    if (lds_slots) E.os << "    u64 c = cslot[gg * T + tid];\n";
+    else E.os << "    u64 c = cnt;\n";
+    E.os << "#pragma unroll\n    for (int off = 32; off >= 1; off >>= 1) c += (u64)__shfl_xor((long long)c, off, 64);\n";
+    E.os << "    if (lane == 0) { red[wave * PW + NP * NG * AW2 + 2 * gg] = (u32)c; red[wave * PW + NP * NG * AW2 + 2 * gg + 1] = (u32)(c >> 32); }\n";
+    E.os << "  }\n  __syncthreads();\n";
+    E.os << "  // combine waves: one thread per partial value\n";
+    E.os << "  for (u32 k = tid; k < NP * NG + NG; k += T) {\n";
+    E.os << "    u32* dst = a.partials + (usize)blockIdx.x * PW;\n";
+    E.os << "    if (k < NP * NG) {\n      Mag<AW2> s = mzero<AW2>();\n";
+    E.os << "      for (int wv = 0; wv < T / 64; wv++) {\n        Mag<AW2> o;\n#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = red[wv * PW + k * AW2 + w];\n        add_to<AW2>(s, s, o);\n      }\n";
+    E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) dst[k * AW2 + w] = s.w[w];\n";
+    E.os << "    } else {\n      const u32 gg = k - NP * NG;\n      u64 c = 0;\n";
+    E.os << "      for (int wv = 0; wv < T / 64; wv++) c += (u64)red[wv * PW + NP * NG * AW2 + 2 * gg] | ((u64)red[wv * PW + NP * NG * AW2 + 2 * gg + 1] << 32);\n";
+    E.os << "      dst[NP * NG * AW2 + 2 * gg] = (u32)c; dst[NP * NG * AW2 + 2 * gg + 1] = (u32)(c >> 32);\n    }\n  }\n}\n";
+
+    // final: one block per (plan, group) sum and one per group count
+    E.os << "extern \"C\" __global__ void __launch_bounds__(256) " << name
+         << "_final(const u32* __restrict__ partials, u32 nblocks, u8* __restrict__ images, u64* __restrict__ counts)\n{\n";
+    E.os << "  const u32 k = blockIdx.x, tid = threadIdx.x;\n";
+    E.os << "  __shared__ u32 red[4 * AW2 + 8];\n";
+    E.os << "  if (k < NP * NG) {\n    Mag<AW2> s = mzero<AW2>();\n";
+    E.os << "    for (u32 b = tid; b < nblocks; b += 256) {\n      Mag<AW2> o;\n#pragma unroll\n      for (int w = 0; w < AW2; w++) o.w[w] = partials[(usize)b * PW + k * AW2 + w];\n      add_to<AW2>(s, s, o);\n    }\n";
+    E.os << "    wave_reduce<AW2>(s);\n";
+    E.os << "    if ((tid & 63u) == 0) {\n#pragma unroll\n      for (int w = 0; w < AW2; w++) red[(tid >> 6) * AW2 + w] = s.w[w];\n    }\n";
+    E.os << "    __syncthreads();\n    if (tid == 0) {\n      Mag<AW2> t = mzero<AW2>();\n      for (int wv = 0; wv < 4; wv++) {\n        Mag<AW2> o;\n#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = red[wv * AW2 + w];\n        add_to<AW2>(t, t, o);\n      }\n";
+    E.os << "      const u32 pp = k % NP;\n      u32 prec = 0, frac = 0;\n";
+    for (int i = 0; i < NP; i++)
+        E.os << "      if (pp == " << i << ") { prec = " << (s.plans[i].outP & 0xff) << "u; frac = " << (s.plans[i].outS & 0xff)
+             << "u; }\n";
+    E.os << "      write_sum_image<AW2>(images + (usize)k * 133u, t, prec, frac, 0u);\n    }\n";
+    E.os << "  } else {\n    const u32 gg = k - NP * NG;\n    u64 c = 0;\n";
+    E.os << "    for (u32 b = tid; b < nblocks; b += 256) c += (u64)partials[(usize)b * PW + NP * NG * AW2 + 2 * gg] | ((u64)partials[(usize)b * PW + NP * NG * AW2 + 2 * gg + 1] << 32);\n";
+    E.os << "#pragma unroll\n    for (int off = 32; off >= 1; off >>= 1) c += (u64)__shfl_xor((long long)c, off, 64);\n";
+    E.os << "    if ((tid & 63u) == 0) { red[2 * (tid >> 6)] = (u32)c; red[2 * (tid >> 6) + 1] = (u32)(c >> 32); }\n";
+    E.os << "    __syncthreads();\n    if (tid == 0) { u64 t = 0; for (int wv = 0; wv < 4; wv++) t += (u64)red[2 * wv] | ((u64)red[2 * wv + 1] << 32); counts[gg] = t; }\n  }\n}\n";
+    return E.os.str();
+}
+
+}  // namespace updec

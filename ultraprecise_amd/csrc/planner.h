@@ -1,0 +1,134 @@
+// planner.h -- host side of the decimal expression path: parse, the
+// reference's alignment scheduling / constant pre-calculation rewrite,
+// per-node result types, limb-width analysis, and HIP source generation.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace updec {
+
+enum class Mode { Faithful = 0, Extended = 1 };
+
+// Host decimal used for constant pre-calculation (the reference folds
+// constants with Decimal::AddDecimalOnCpu / MulDecimalOnCpu on the host,
+// AriesEngineShell.cpp:360-597).  Exact limbs; faithful mode reduces every
+// result mod 2^1024 like the reference's 32-limb Decimal.
+struct HostDec {
+    static constexpr int NW = 96;
+    uint32_t sign = 0;
+    int32_t prec = 0;
+    int32_t frac = 0;
+    uint32_t v[NW] = {0};
+
+    static HostDec from_string(const std::string& s, Mode m);     // decimal.cu:2980-3025
+    static HostDec from_int(int64_t t, int prec);                  // decimal.cu:281-309
+    static HostDec from_int_scale(int32_t t, int32_t scale, Mode m);// decimal.cu:230-260
+    HostDec rescaled_up(int scale, Mode m) const;                  // decimal.cu:263-268
+    void add(const HostDec& d, Mode m);                            // decimal.cu:1284-1323
+    void mul(const HostDec& d, Mode m);                            // decimal.cu:1325-1351
+    bool is_zero() const;
+    bool equals_one() const;                                        // operator!=(Decimal, int)
+    void check_and_set_real_precision();                           // decimal.cu:2060-2088
+    int bitlen() const;
+    std::string hex() const;
+};
+
+struct ColumnType {
+    int P = 0, S = 0;
+    // Optional statistic: every stored magnitude is < 2^bits (0 = unknown,
+    // then the layout bound 8*B(P)-1 is used).  Narrower bounds give
+    // narrower limb arrays; a wrong bound gives wrong results.
+    int bits = 0;
+};
+
+// Expression tree, mirroring the reference's BinaryTree/MultipleTree
+// (AriesEngineShell.cpp:19-95) closely enough to apply the same rewrite.
+enum class NK { Column, Int, Dec, Calc };
+enum class Op { Add = 0, Sub = 1, Mul = 2, Div = 3, Mod = 4 };
+
+struct Expr {
+    NK kind = NK::Column;
+    Op op = Op::Add;
+    int col = -1;
+    int64_t ival = 0;
+    HostDec dval;
+    int P = 0, S = 0;       // value_type.DataType.{Precision,Scale}
+    bool reverse = false;    // pending negation (value_reverse)
+    std::vector<std::shared_ptr<Expr>> ch;
+};
+using ExprP = std::shared_ptr<Expr>;
+
+// Parse infix text: identifiers (column names), integer and decimal
+// literals, + - * / %, parentheses, unary minus.
+ExprP parse_expr(const std::string& text, const std::vector<std::string>& names,
+                 const std::vector<ColumnType>& types, Mode mode, std::string* err);
+
+// Frontend result types (CommonBiaodashi.cpp:3338-3377 with
+// Calc*TargetPrecision, decimal.cu:1353-1358,1568-1572,1783-1786,1997-2000).
+void infer_frontend_types(Expr& e, Mode mode);
+
+// The reference's AlignOptimize (AriesEngineShell.cpp:646-685) applied to
+// one calculation tree.  Returns the rewritten tree whose nodes carry the
+// per-node (P,S) of ListToBinTree (AriesEngineShell.cpp:90-144).
+ExprP align_optimize(const ExprP& e, Mode mode);
+
+// Typed evaluation plan
+enum class PK { Col, Const, Neg, Add, Sub, Mul, Div };
+struct PNode {
+    PK kind;
+    int P = 0, S = 0;
+    int col = -1;
+    HostDec k;               // Const
+    int a = -1, b = -1;      // children
+    int bits = 0;            // magnitude bound: value < 2^bits
+    int W = 1;               // limbs
+};
+struct Plan {
+    std::vector<PNode> n;
+    int root = -1;
+    Mode mode = Mode::Faithful;
+    int outP = 0, outS = 0, outB = 0;   // materialized result type and compact bytes
+    std::string text() const;           // canonical plan text (see oracle's od_plan_parse)
+};
+
+int compact_bytes(int P);   // __PREC_BYTE_ARRAY (decimal.cu:92), closed form past 299
+
+// Build a plan from a rewritten tree.  Column types give the operand
+// layout.  Returns false with *err set when the tree uses something the
+// HIP path does not implement (MOD).
+bool build_plan(const ExprP& rewritten, const std::vector<ColumnType>& cols, Mode requested, Plan* out,
+                std::string* err);
+
+// Classification: true when the plan stays inside the reference's defined
+// envelope (every shift <= 38, scales <= 255, precisions <= 299, divisions
+// finish in one chunk), i.e. the reference-faithful contract applies.
+bool plan_is_faithful_envelope(const Plan& p, const std::vector<ColumnType>& cols);
+
+// ---- code generation ----
+struct GenOptions {
+    int block = 256;
+};
+
+// calc kernel: one compact output row per input row (the reference's
+// calc_expr_<id>, AriesExprCalcNode.cpp:1285-1318)
+std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, const std::string& name,
+                            const GenOptions& o);
+
+// fused aggregation kernel for several plans over the same rows:
+//   optional int32 filter column (value <= / < / >= / > / == / != constant),
+//   optional group id from up to two 1-byte key columns through a LUT,
+//   per plan a SUM accumulator per group, plus a row count per group.
+struct AggSpec {
+    std::vector<Plan> plans;
+    int filter_col = -1;      // index into the int32 column list
+    int filter_op = 0;        // 0 <=, 1 <, 2 >=, 3 >, 4 ==, 5 !=
+    int64_t filter_val = 0;
+    int nkeys = 0;            // 0, 1 or 2 key byte columns
+    int ngroups = 1;          // groups addressed by the LUT (max)
+};
+std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& cols, const std::string& name,
+                            const GenOptions& o, int* acc_words);
+
+}  // namespace updec

@@ -1,0 +1,873 @@
+// updec_api.hip -- the C ABI (include/updec.h): hipRTC module cache,
+// expression / aggregation drivers, and the static kernels (generator,
+// statistics, group discovery, AVG division).
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/updec.h"
+#include "dec_device.h"
+#include "gen_common.h"
+#include "planner.h"
+
+// text of dec_device.h, generated at build time, handed to hipRTC
+#include "dec_device_embed.inc"
+
+using namespace updec;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                 \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess) return fail(UPDEC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+hipStream_t S(updec_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ------------------------------------------------------------------ hipRTC module cache
+
+struct Module {
+    hipModule_t mod = nullptr;
+    std::map<std::string, hipFunction_t> fns;
+    std::mutex mu;
+    ~Module()
+    {
+        if (mod) (void)hipModuleUnload(mod);
+    }
+    hipFunction_t fn(const std::string& name, std::string* err)
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = fns.find(name);
+        if (it != fns.end()) return it->second;
+        hipFunction_t f = nullptr;
+        hipError_t e = hipModuleGetFunction(&f, mod, name.c_str());
+        if (e != hipSuccess) {
+            *err = "hipModuleGetFunction(" + name + "): " + hipGetErrorString(e);
+            return nullptr;
+        }
+        fns[name] = f;
+        return f;
+    }
+};
+using ModuleP = std::shared_ptr<Module>;
+
+// LRU of compiled modules keyed by the full source text, as the
+// reference's dynamic kernel manager (DynamicKernel.cu:10-58, LRU_COUNT 100)
+class ModuleCache {
+  public:
+    static ModuleCache& get()
+    {
+        static ModuleCache c;
+        return c;
+    }
+    ModuleP compile(const std::string& src, std::string* err)
+    {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto it = map_.find(src);
+            if (it != map_.end()) {
+                lru_.splice(lru_.begin(), lru_, it->second.second);
+                return it->second.first;
+            }
+        }
+        std::vector<char> code;
+        if (!rtc(src, &code, err)) return nullptr;
+        auto m = std::make_shared<Module>();
+        hipError_t e = hipModuleLoadData(&m->mod, code.data());
+        if (e != hipSuccess) {
+            *err = std::string("hipModuleLoadData: ") + hipGetErrorString(e);
+            m->mod = nullptr;
+            return nullptr;
+        }
+        std::lock_guard<std::mutex> g(mu_);
+        lru_.push_front(src);
+        map_[src] = {m, lru_.begin()};
+        while (map_.size() > kCap) {
+            map_.erase(lru_.back());
+            lru_.pop_back();
+        }
+        return m;
+    }
+    // compile only (no device needed): used to check generated sources
+    static bool rtc(const std::string& src, std::vector<char>* code, std::string* err)
+    {
+        hiprtcProgram prog;
+        const char* hdr_src[] = {kDecDeviceSource};
+        const char* hdr_name[] = {"dec_device.h"};
+        if (hiprtcCreateProgram(&prog, src.c_str(), "updec_kernel.hip", 1, hdr_src, hdr_name) != HIPRTC_SUCCESS) {
+            *err = "hiprtcCreateProgram failed";
+            return false;
+        }
+        const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                              "-packed-fp32-ops"};
+        hiprtcResult r = hiprtcCompileProgram(prog, sizeof(opts) / sizeof(opts[0]), opts);
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls + 1, '\0');
+        if (ls) hiprtcGetProgramLog(prog, &log[0]);
+        if (r != HIPRTC_SUCCESS) {
+            *err = "hipRTC compile failed: " + log;
+            hiprtcDestroyProgram(&prog);
+            return false;
+        }
+        size_t cs = 0;
+        hiprtcGetCodeSize(prog, &cs);
+        code->resize(cs);
+        hiprtcGetCode(prog, code->data());
+        hiprtcDestroyProgram(&prog);
+        return true;
+    }
+
+  private:
+    static constexpr size_t kCap = 100;
+    std::mutex mu_;
+    std::list<std::string> lru_;
+    std::unordered_map<std::string, std::pair<ModuleP, std::list<std::string>::iterator>> map_;
+};
+
+int device_cus()
+{
+    static std::atomic<int> cus{0};
+    int c = cus.load();
+    if (c) return c;
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    cus.store(v);
+    return v;
+}
+
+std::vector<ColumnType> to_types(const updec_column_desc* cols, int n, std::vector<std::string>* names)
+{
+    std::vector<ColumnType> t(n);
+    for (int i = 0; i < n; i++) {
+        t[i].P = cols[i].precision;
+        t[i].S = cols[i].scale;
+        t[i].bits = cols[i].bits;
+        if (names) names->push_back(cols[i].name ? cols[i].name : ("c" + std::to_string(i)));
+    }
+    return t;
+}
+
+// parse -> AlignOptimize -> plan; mode AUTO picks faithful inside the envelope
+bool make_plan(const std::string& text, const std::vector<std::string>& names, const std::vector<ColumnType>& types,
+               int mode, Plan* out, std::string* err)
+{
+    for (int attempt = 0; attempt < 2; attempt++) {
+        Mode m;
+        if (mode == UPDEC_MODE_FAITHFUL) m = Mode::Faithful;
+        else if (mode == UPDEC_MODE_EXTENDED) m = Mode::Extended;
+        else m = attempt == 0 ? Mode::Faithful : Mode::Extended;
+        ExprP e = parse_expr(text, names, types, m, err);
+        if (!e) return false;
+        ExprP rw = align_optimize(e, m);
+        if (!rw) { *err = "expression rewrites to a constant or integer"; return false; }
+        Plan p;
+        if (!build_plan(rw, types, m, &p, err)) return false;
+        bool inside = plan_is_faithful_envelope(p, types);
+        if (mode == UPDEC_MODE_AUTO && m == Mode::Faithful && !inside) continue;
+        if (mode == UPDEC_MODE_FAITHFUL && !inside) {
+            *err = "expression leaves the reference's defined envelope; use extended mode";
+            return false;
+        }
+        *out = std::move(p);
+        return true;
+    }
+    return false;
+}
+
+int copy_text(const std::string& s, char* buf, size_t cap, size_t* needed)
+{
+    if (needed) *needed = s.size() + 1;
+    if (buf && cap) {
+        size_t n = std::min(cap - 1, s.size());
+        memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return UPDEC_OK;
+}
+
+std::atomic<uint64_t> g_kernel_id{0};
+
+}  // namespace
+
+// ------------------------------------------------------------------ static kernels
+
+#define UPD_MAXCOLS 16
+struct ColArgs {
+    const u8* col[UPD_MAXCOLS];
+    usize colbytes[UPD_MAXCOLS];
+    usize rows;
+    u8* out;
+};
+struct AggArgs {
+    const u8* col[UPD_MAXCOLS];
+    usize colbytes[UPD_MAXCOLS];
+    usize rows;
+    const i32* fcol;
+    const u8* key0;
+    const u8* key1;
+    const u8* lut;
+    u32* partials;
+};
+
+__global__ void __launch_bounds__(256) k_gen_compact(updec_gen::Desc d, uint64_t key, int64_t row0, int64_t rows,
+                                                     u8* out, int nbytes)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    u8 buf[128];
+    updec_gen::gen_row(d, key, (uint64_t)(row0 + i), buf, nbytes);
+    for (int b = 0; b < nbytes; b++) out[i * nbytes + b] = buf[b];
+}
+
+__global__ void __launch_bounds__(256) k_gen_int32(uint64_t key, int32_t lo, int32_t hi, int64_t row0, int64_t rows,
+                                                   int32_t* out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < rows) out[i] = updec_gen::gen_int32(key, (uint64_t)(row0 + i), lo, hi);
+}
+
+struct ByteChoices {
+    u8 c[64];
+    int n;
+};
+__global__ void __launch_bounds__(256) k_gen_byte(uint64_t key, ByteChoices ch, int64_t row0, int64_t rows, u8* out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < rows) out[i] = updec_gen::gen_byte(key, (uint64_t)(row0 + i), ch.c, ch.n);
+}
+
+// largest magnitude bit length of a compact column (sign bit excluded)
+__global__ void __launch_bounds__(256) k_maxbits(const u8* col, int nbytes, int64_t rows, u32* out)
+{
+    u32 best = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x) {
+        const u8* r = col + i * nbytes;
+        u32 bl = 0;
+        for (int b = nbytes - 1; b >= 0; b--) {
+            u32 x = r[b];
+            if (b == nbytes - 1) x &= 0x7fu;
+            if (x) { bl = 8u * (u32)b + 32u - (u32)__builtin_clz(x); break; }
+        }
+        best = bl > best ? bl : best;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        u32 o = (u32)__shfl_xor((int)best, off, 64);
+        best = o > best ? o : best;
+    }
+    if ((threadIdx.x & 63u) == 0) atomicMax(out, best);
+}
+
+__device__ __forceinline__ bool filter_pass(i32 v, int op, i64 c)
+{
+    switch (op) {
+    case 0: return v <= c;
+    case 1: return v < c;
+    case 2: return v >= c;
+    case 3: return v > c;
+    case 4: return v == c;
+    default: return v != c;
+    }
+}
+
+// group discovery: presence bitmap of (key0 << 8 | key1) over filtered rows
+__global__ void __launch_bounds__(256) k_key_presence(const u8* key0, const u8* key1, int nkeys, const i32* fcol,
+                                                      int fop, i64 fval, int64_t rows, u32* bitmap)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x) {
+        if (fcol && !filter_pass(fcol[i], fop, fval)) continue;
+        u32 k = nkeys == 2 ? (((u32)key0[i] << 8) | key1[i]) : key0[i];
+        u32 bit = 1u << (k & 31u);
+        // skip the atomic when another lane already set the bit
+        if (!(__builtin_nontemporal_load(&bitmap[k >> 5]) & bit)) atomicOr(&bitmap[k >> 5], bit);
+    }
+}
+
+// AVG: Decimal(sum) /= Decimal((int64)count) per group (kernel_util.hxx:427-438)
+template <int FAITH>
+__global__ void __launch_bounds__(64) k_div_int64(const u8* sums, int64_t stride, const int64_t* counts, u8* out, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const u8* s = sums + i * stride;
+    Dec<32> a;
+    a.neg = s[1];
+    const u32 prec = s[2], frac = s[3], error = s[4];
+#pragma unroll
+    for (int w = 0; w < 32; w++)
+        a.m.w[w] = (u32)s[5 + 4 * w] | ((u32)s[6 + 4 * w] << 8) | ((u32)s[7 + 4 * w] << 16) | ((u32)s[8 + 4 * w] << 24);
+    const int64_t cnt = counts[i];
+    Dec<2> b;
+    const u64 uc = cnt < 0 ? (u64)(-(cnt + 1)) + 1u : (u64)cnt;
+    b.neg = cnt < 0 ? 1u : 0u;
+    b.m.w[0] = (u32)uc;
+    b.m.w[1] = (u32)(uc >> 32);
+    u32 err = error;
+    Dec<32> q = ddiv<32, 32, 2, 4, FAITH>(a, b, &err);
+    u8* o = out + i * 133;
+    o[0] = s[0];
+    o[1] = (u8)q.neg;
+    o[2] = is_zero(a.m) ? (u8)(frac + 4u + 1u) : (u8)prec;
+    o[3] = (u8)(frac + 4u);
+    o[4] = (u8)err;
+#pragma unroll
+    for (int w = 0; w < 32; w++) {
+        o[5 + 4 * w] = (u8)q.m.w[w];
+        o[6 + 4 * w] = (u8)(q.m.w[w] >> 8);
+        o[7 + 4 * w] = (u8)(q.m.w[w] >> 16);
+        o[8 + 4 * w] = (u8)(q.m.w[w] >> 24);
+    }
+}
+
+// ------------------------------------------------------------------ objects
+
+struct updec_expr_s {
+    Plan plan;
+    std::vector<ColumnType> types;
+    std::string name, src;
+    ModuleP mod;
+    hipFunction_t fn = nullptr;
+};
+
+struct updec_agg_s {
+    AggSpec spec;
+    std::vector<ColumnType> types;
+    int nkeys = 0, has_filter = 0;
+    std::string base_name, last_src;
+    std::map<int, std::pair<ModuleP, std::string>> by_ngroups;
+    u32* d_bitmap = nullptr;
+    u8* d_lut = nullptr;
+    u32* d_partials = nullptr;
+    size_t partial_words = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    ~updec_agg_s()
+    {
+        if (d_bitmap) (void)hipFree(d_bitmap);
+        if (d_lut) (void)hipFree(d_lut);
+        if (d_partials) (void)hipFree(d_partials);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
+};
+
+struct updec_module_s {
+    ModuleP mod;
+};
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+const char* updec_last_error(void) { return g_err.c_str(); }
+int updec_version(void) { return 1; }
+
+int updec_init(int device)
+{
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipFree(nullptr));
+    return UPDEC_OK;
+}
+
+int updec_compact_bytes(int precision) { return compact_bytes(precision); }
+
+int updec_column_maxbits(const uint8_t* d_col, int precision, int64_t rows, int32_t* bits_out, updec_stream_t stream)
+{
+    if (!bits_out || rows < 0) return fail(UPDEC_EINVAL, "bad argument");
+    int nb = compact_bytes(precision);
+    if (nb <= 0) return fail(UPDEC_EINVAL, "bad precision");
+    u32* d = nullptr;
+    HIPCHK(hipMallocAsync((void**)&d, sizeof(u32), S(stream)));
+    HIPCHK(hipMemsetAsync(d, 0, sizeof(u32), S(stream)));
+    if (rows > 0) {
+        int64_t blocks = std::min<int64_t>((rows + 255) / 256, (int64_t)device_cus() * 8);
+        hipLaunchKernelGGL(k_maxbits, dim3((unsigned)blocks), dim3(256), 0, S(stream), d_col, nb, rows, d);
+        HIPCHK(hipGetLastError());
+    }
+    u32 h = 0;
+    HIPCHK(hipMemcpyAsync(&h, d, sizeof(u32), hipMemcpyDeviceToHost, S(stream)));
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    HIPCHK(hipFreeAsync(d, S(stream)));
+    *bits_out = (int32_t)h;
+    return UPDEC_OK;
+}
+
+int updec_expr_compile(const char* text, const updec_column_desc* cols, int ncols, int mode, updec_expr_t* out)
+{
+    if (!text || !out || ncols < 0 || ncols > UPD_MAXCOLS || (ncols && !cols)) return fail(UPDEC_EINVAL, "bad argument");
+    std::vector<std::string> names;
+    auto types = to_types(cols, ncols, &names);
+    std::unique_ptr<updec_expr_s> e(new updec_expr_s());
+    std::string err;
+    if (!make_plan(text, names, types, mode, &e->plan, &err)) return fail(UPDEC_ECOMPILE, err);
+    e->types = types;
+    e->name = "calc_expr_" + std::to_string(g_kernel_id.fetch_add(1));
+    GenOptions o;
+    e->src = gen_calc_kernel(e->plan, types, "calc_expr", o);
+    *out = e.release();
+    return UPDEC_OK;
+}
+
+int updec_expr_info(updec_expr_t e, int32_t* P, int32_t* S_, int32_t* out_bytes, int32_t* faithful)
+{
+    if (!e) return fail(UPDEC_EINVAL, "null expression");
+    if (P) *P = e->plan.outP;
+    if (S_) *S_ = e->plan.outS;
+    if (out_bytes) *out_bytes = e->plan.outB;
+    if (faithful) *faithful = e->plan.mode == Mode::Faithful ? 1 : 0;
+    return UPDEC_OK;
+}
+
+int updec_expr_plan(updec_expr_t e, char* buf, size_t cap, size_t* needed)
+{
+    if (!e) return fail(UPDEC_EINVAL, "null expression");
+    char hdr[64];
+    snprintf(hdr, sizeof hdr, "%s %d %d %d ", e->plan.mode == Mode::Faithful ? "F" : "X", e->plan.outP, e->plan.outS,
+             e->plan.outB);
+    return copy_text(hdr + e->plan.text(), buf, cap, needed);
+}
+
+int updec_expr_source(updec_expr_t e, char* buf, size_t cap, size_t* needed)
+{
+    if (!e) return fail(UPDEC_EINVAL, "null expression");
+    return copy_text(e->src, buf, cap, needed);
+}
+
+int updec_expr_calc(updec_expr_t e, const uint8_t* const* d_cols, int64_t rows, uint8_t* d_out, updec_stream_t stream)
+{
+    if (!e || rows < 0 || (rows && (!d_cols || !d_out))) return fail(UPDEC_EINVAL, "bad argument");
+    if (reinterpret_cast<uintptr_t>(d_out) & 3u) return fail(UPDEC_EINVAL, "output must be 4-byte aligned");
+    if (rows == 0) return UPDEC_OK;
+    std::string err;
+    if (!e->mod) {
+        e->mod = ModuleCache::get().compile(e->src, &err);
+        if (!e->mod) return fail(UPDEC_ECOMPILE, err);
+        e->fn = e->mod->fn("calc_expr", &err);
+        if (!e->fn) return fail(UPDEC_ECOMPILE, err);
+    }
+    ColArgs a{};
+    for (size_t i = 0; i < e->types.size(); i++) {
+        a.col[i] = d_cols[i];
+        a.colbytes[i] = (usize)rows * (usize)compact_bytes(e->types[i].P);
+    }
+    a.rows = (usize)rows;
+    a.out = d_out;
+    int64_t tiles = (rows + 255) / 256;
+    unsigned grid = (unsigned)std::min<int64_t>(tiles, (int64_t)device_cus() * 16);
+    void* params[] = {&a};
+    HIPCHK(hipModuleLaunchKernel(e->fn, grid, 1, 1, 256, 1, 1, 0, S(stream), params, nullptr));
+    return UPDEC_OK;
+}
+
+void updec_expr_free(updec_expr_t e) { delete e; }
+
+int updec_agg_compile(const updec_agg_desc* d, updec_agg_t* out)
+{
+    if (!d || !out || d->ncols < 0 || d->ncols > UPD_MAXCOLS || d->nexprs <= 0 || d->nkeys < 0 || d->nkeys > 2)
+        return fail(UPDEC_EINVAL, "bad argument");
+    std::vector<std::string> names;
+    std::unique_ptr<updec_agg_s> a(new updec_agg_s());
+    a->types = to_types(d->cols, d->ncols, &names);
+    std::string err;
+    for (int k = 0; k < d->nexprs; k++) {
+        Plan p;
+        if (!make_plan(d->exprs[k], names, a->types, d->mode, &p, &err))
+            return fail(UPDEC_ECOMPILE, "expr " + std::to_string(k) + ": " + err);
+        a->spec.plans.push_back(std::move(p));
+    }
+    a->has_filter = d->has_filter;
+    a->spec.filter_col = d->has_filter ? 0 : -1;
+    a->spec.filter_op = d->filter_op;
+    a->spec.filter_val = d->filter_value;
+    a->nkeys = d->nkeys;
+    a->spec.nkeys = d->nkeys;
+    a->base_name = "agg";
+    *out = a.release();
+    return UPDEC_OK;
+}
+
+static int agg_module(updec_agg_t a, int ngroups, ModuleP* mod, std::string* err)
+{
+    auto it = a->by_ngroups.find(ngroups);
+    if (it != a->by_ngroups.end()) {
+        *mod = it->second.first;
+        a->last_src = it->second.second;
+        return UPDEC_OK;
+    }
+    AggSpec s = a->spec;
+    s.ngroups = ngroups;
+    GenOptions o;
+    int aw = 0;
+    std::string src = gen_agg_kernels(s, a->types, a->base_name, o, &aw);
+    ModuleP m = ModuleCache::get().compile(src, err);
+    if (!m) return UPDEC_ECOMPILE;
+    a->by_ngroups[ngroups] = {m, src};
+    a->last_src = src;
+    *mod = m;
+    return UPDEC_OK;
+}
+
+int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_filter, const uint8_t* d_key0,
+                  const uint8_t* d_key1, int64_t rows, uint8_t* d_images, int64_t* d_counts, int32_t max_groups,
+                  int32_t* ngroups_out, uint16_t* keys_out, updec_stream_t stream)
+{
+    if (!a || rows < 0 || !d_images || !d_counts || !ngroups_out) return fail(UPDEC_EINVAL, "bad argument");
+    if (a->has_filter && !d_filter) return fail(UPDEC_EINVAL, "filter column missing");
+    if (a->nkeys >= 1 && !d_key0) return fail(UPDEC_EINVAL, "key column missing");
+    if (a->nkeys == 2 && !d_key1) return fail(UPDEC_EINVAL, "second key column missing");
+    hipStream_t st = S(stream);
+    int ngroups = 1;
+    std::vector<uint16_t> keys;
+    if (a->nkeys > 0) {
+        if (!a->d_bitmap) HIPCHK(hipMalloc((void**)&a->d_bitmap, 65536 / 8));
+        if (!a->d_lut) HIPCHK(hipMalloc((void**)&a->d_lut, 65536));
+        HIPCHK(hipMemsetAsync(a->d_bitmap, 0, 65536 / 8, st));
+        if (rows > 0) {
+            int64_t blocks = std::min<int64_t>((rows + 255) / 256, (int64_t)device_cus() * 8);
+            hipLaunchKernelGGL(k_key_presence, dim3((unsigned)blocks), dim3(256), 0, st, d_key0, d_key1, a->nkeys,
+                               a->has_filter ? d_filter : nullptr, a->spec.filter_op, (i64)a->spec.filter_val, rows,
+                               a->d_bitmap);
+            HIPCHK(hipGetLastError());
+        }
+        std::vector<u32> bm(65536 / 32);
+        HIPCHK(hipMemcpyAsync(bm.data(), a->d_bitmap, 65536 / 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        std::vector<u8> lut(65536, 0xff);
+        for (u32 k = 0; k < 65536; k++)
+            if (bm[k >> 5] & (1u << (k & 31))) {
+                if ((int)keys.size() >= 255) return fail(UPDEC_EUNSUPPORTED, "more than 255 groups");
+                lut[k] = (u8)keys.size();
+                keys.push_back((uint16_t)(a->nkeys == 2 ? (((k >> 8) & 0xff) | ((k & 0xff) << 8)) : k));
+            }
+        ngroups = (int)keys.size();
+        if (ngroups > max_groups) return fail(UPDEC_EINVAL, "more groups than max_groups");
+        *ngroups_out = ngroups;
+        if (keys_out)
+            for (int g = 0; g < ngroups; g++) keys_out[g] = keys[g];
+        if (ngroups == 0) return UPDEC_OK;
+        HIPCHK(hipMemcpyAsync(a->d_lut, lut.data(), 65536, hipMemcpyHostToDevice, st));
+    } else {
+        if (max_groups < 1) return fail(UPDEC_EINVAL, "max_groups < 1");
+        *ngroups_out = 1;
+        if (keys_out) keys_out[0] = 0;
+    }
+    std::string err;
+    ModuleP mod;
+    if (agg_module(a, ngroups, &mod, &err) != UPDEC_OK) return fail(UPDEC_ECOMPILE, err);
+    hipFunction_t fmain = mod->fn(a->base_name + "_main", &err);
+    hipFunction_t ffin = fmain ? mod->fn(a->base_name + "_final", &err) : nullptr;
+    if (!fmain || !ffin) return fail(UPDEC_ECOMPILE, err);
+
+    const int T = 256;
+    const int NP = (int)a->spec.plans.size();
+    int aw = 0;
+    {
+        AggSpec s = a->spec;
+        s.ngroups = ngroups;
+        // recompute AW2 as the generator does
+        int vbits = 1;
+        for (auto& p : s.plans) vbits = std::max(vbits, std::min(p.n[p.root].bits, 8 * p.outB - 1));
+        aw = std::max(1, (vbits + 40 + 1 + 31) / 32);
+    }
+    const size_t PW = (size_t)NP * ngroups * aw + 2 * (size_t)ngroups;
+    int64_t tiles = std::max<int64_t>(1, (rows + T - 1) / T);
+    int blocks_per_cu = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, fmain, T, 0) != hipSuccess || blocks_per_cu < 1)
+        blocks_per_cu = 2;
+    int64_t grid = std::min<int64_t>(tiles, (int64_t)device_cus() * blocks_per_cu);
+    int64_t min_grid = (rows + (int64_t)T * 65536 - 1) / ((int64_t)T * 65536);  // <= 2^16 rows per lane
+    grid = std::max(grid, std::max<int64_t>(min_grid, 1));
+    if ((size_t)grid * PW > a->partial_words) {
+        if (a->d_partials) HIPCHK(hipFree(a->d_partials));
+        a->partial_words = (size_t)grid * PW;
+        HIPCHK(hipMalloc((void**)&a->d_partials, a->partial_words * sizeof(u32)));
+    }
+    AggArgs args{};
+    for (size_t i = 0; i < a->types.size(); i++) {
+        args.col[i] = d_cols ? d_cols[i] : nullptr;
+        args.colbytes[i] = (usize)rows * (usize)compact_bytes(a->types[i].P);
+    }
+    args.rows = (usize)rows;
+    args.fcol = d_filter;
+    args.key0 = d_key0;
+    args.key1 = d_key1;
+    args.lut = a->d_lut;
+    args.partials = a->d_partials;
+    if (!a->ev0) {
+        HIPCHK(hipEventCreate(&a->ev0));
+        HIPCHK(hipEventCreate(&a->ev1));
+    }
+    HIPCHK(hipEventRecord(a->ev0, st));
+    void* pm[] = {&args};
+    HIPCHK(hipModuleLaunchKernel(fmain, (unsigned)grid, 1, 1, T, 1, 1, 0, st, pm, nullptr));
+    HIPCHK(hipEventRecord(a->ev1, st));
+    a->timed = true;
+    const u32* cp = a->d_partials;
+    u32 nb = (u32)grid;
+    uint8_t* img = d_images;
+    uint64_t* cnt = reinterpret_cast<uint64_t*>(d_counts);
+    void* pf[] = {&cp, &nb, &img, &cnt};
+    HIPCHK(hipModuleLaunchKernel(ffin, (unsigned)(NP * ngroups + ngroups), 1, 1, 256, 1, 1, 0, st, pf, nullptr));
+    return UPDEC_OK;
+}
+
+int updec_agg_last_kernel_ms(updec_agg_t a, float* ms)
+{
+    if (!a || !ms || !a->timed) return fail(UPDEC_EINVAL, "no timed run");
+    HIPCHK(hipEventSynchronize(a->ev1));
+    HIPCHK(hipEventElapsedTime(ms, a->ev0, a->ev1));
+    return UPDEC_OK;
+}
+
+int updec_agg_source(updec_agg_t a, int32_t ngroups, char* buf, size_t cap, size_t* needed)
+{
+    if (!a || ngroups < 1 || ngroups > 255) return fail(UPDEC_EINVAL, "bad argument");
+    AggSpec s = a->spec;
+    s.ngroups = ngroups;
+    GenOptions o;
+    int aw = 0;
+    return copy_text(gen_agg_kernels(s, a->types, a->base_name, o, &aw), buf, cap, needed);
+}
+
+void updec_agg_free(updec_agg_t a) { delete a; }
+
+int updec_div_int64(const uint8_t* d_sums, int64_t stride_bytes, const int64_t* d_counts, uint8_t* d_out, int64_t n,
+                    int mode, updec_stream_t stream)
+{
+    if (n < 0 || (n && (!d_sums || !d_counts || !d_out)) || stride_bytes < 133) return fail(UPDEC_EINVAL, "bad argument");
+    if (n == 0) return UPDEC_OK;
+    unsigned blocks = (unsigned)((n + 63) / 64);
+    if (mode == UPDEC_MODE_EXTENDED)
+        hipLaunchKernelGGL(k_div_int64<0>, dim3(blocks), dim3(64), 0, S(stream), d_sums, stride_bytes, d_counts, d_out, n);
+    else
+        hipLaunchKernelGGL(k_div_int64<1>, dim3(blocks), dim3(64), 0, S(stream), d_sums, stride_bytes, d_counts, d_out, n);
+    HIPCHK(hipGetLastError());
+    return UPDEC_OK;
+}
+
+// segmented SUM: chunks of at most 2^16 rows, one block per chunk, then one
+// block per segment over its chunk partials
+static std::string gen_segsum(int P, int S_, int mode)
+{
+    const int B = compact_bytes(P);
+    const int NW = (B + 3) / 4;
+    const int AW = std::max(1, (8 * B - 1 + 40 + 1 + 31) / 32);
+    std::string t =
+        "#include \"dec_device.h\"\nusing namespace updec;\n"
+        "// segmented SUM of DECIMAL(@P,@S), mode @M\n"
+        "struct SegArgs { const u8* col; usize colbytes; const i32* assoc; const i64* chunk_lo; const i64* chunk_hi;\n"
+        "                 const i32* seg_chunk0; i32 nseg; u32* partials; u8* out; };\n"
+        "extern \"C\" __global__ void __launch_bounds__(256) seg_chunk(SegArgs a) {\n"
+        "  const i64 lo = a.chunk_lo[blockIdx.x], hi = a.chunk_hi[blockIdx.x];\n"
+        "  Mag<@AW> acc = mzero<@AW>();\n"
+        "  for (i64 i = lo + threadIdx.x; i < hi; i += 256) {\n"
+        "    const usize row = a.assoc ? (usize)a.assoc[i] : (usize)i;\n"
+        "    const Dec<@NW> v = load_compact<@B>(a.col, row, a.colbytes);\n"
+        "    acc_add<@AW, @NW>(acc, v);\n  }\n"
+        "  wave_reduce<@AW>(acc);\n"
+        "  __shared__ u32 red[4 * @AW];\n"
+        "  if ((threadIdx.x & 63u) == 0) for (int w = 0; w < @AW; w++) red[(threadIdx.x >> 6) * @AW + w] = acc.w[w];\n"
+        "  __syncthreads();\n"
+        "  if (threadIdx.x == 0) {\n    Mag<@AW> t = mzero<@AW>();\n"
+        "    for (int wv = 0; wv < 4; wv++) { Mag<@AW> o; for (int w = 0; w < @AW; w++) o.w[w] = red[wv * @AW + w]; add_to<@AW>(t, t, o); }\n"
+        "    for (int w = 0; w < @AW; w++) a.partials[(usize)blockIdx.x * @AW + w] = t.w[w];\n  }\n}\n"
+        "extern \"C\" __global__ void __launch_bounds__(64) seg_final(SegArgs a) {\n"
+        "  const i32 g = blockIdx.x * 64 + threadIdx.x;\n  if (g >= a.nseg) return;\n"
+        "  Mag<@AW> t = mzero<@AW>();\n"
+        "  for (i32 c = a.seg_chunk0[g]; c < a.seg_chunk0[g + 1]; c++) { Mag<@AW> o; for (int w = 0; w < @AW; w++) o.w[w] = a.partials[(usize)c * @AW + w]; add_to<@AW>(t, t, o); }\n"
+        "  write_sum_image<@AW>(a.out + (usize)g * 133u, t, @PBu, @SBu, 0u);\n}\n";
+    auto sub = [&](const std::string& key, const std::string& val) {
+        for (size_t pos = t.find(key); pos != std::string::npos; pos = t.find(key, pos + val.size())) t.replace(pos, key.size(), val);
+    };
+    sub("@PB", std::to_string(P & 0xff));
+    sub("@SB", std::to_string(S_ & 0xff));
+    sub("@AW", std::to_string(AW));
+    sub("@NW", std::to_string(NW));
+    sub("@B", std::to_string(B));
+    sub("@P", std::to_string(P));
+    sub("@S", std::to_string(S_));
+    sub("@M", std::to_string(mode));
+    return t;
+}
+
+int updec_segsum_compact(const uint8_t* d_col, int precision, int scale, int64_t rows, const int32_t* d_associated,
+                         const int32_t* d_groups, int32_t ngroups, uint8_t* d_out, int mode, updec_stream_t stream)
+{
+    if (rows < 0 || ngroups < 0 || (ngroups && (!d_groups || !d_out)) || (rows && !d_col) || precision <= 0)
+        return fail(UPDEC_EINVAL, "bad argument");
+    if (ngroups == 0) return UPDEC_OK;
+    hipStream_t st = S(stream);
+    std::vector<int32_t> g(ngroups);
+    HIPCHK(hipMemcpyAsync(g.data(), d_groups, sizeof(int32_t) * ngroups, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<int64_t> lo, hi;
+    std::vector<int32_t> c0(ngroups + 1);
+    const int64_t CH = 65536;
+    for (int s = 0; s < ngroups; s++) {
+        int64_t a = g[s], b = s + 1 < ngroups ? g[s + 1] : rows;
+        if (a < 0 || b > rows || a > b) return fail(UPDEC_EINVAL, "segment starts out of order or out of range");
+        c0[s] = (int32_t)lo.size();
+        for (int64_t x = a; x < b; x += CH) { lo.push_back(x); hi.push_back(std::min(b, x + CH)); }
+    }
+    c0[ngroups] = (int32_t)lo.size();
+    std::string err;
+    ModuleP m = ModuleCache::get().compile(gen_segsum(precision, scale, mode), &err);
+    if (!m) return fail(UPDEC_ECOMPILE, err);
+    hipFunction_t fc = m->fn("seg_chunk", &err), ff = fc ? m->fn("seg_final", &err) : nullptr;
+    if (!fc || !ff) return fail(UPDEC_ECOMPILE, err);
+    const int B = compact_bytes(precision);
+    const int AW = std::max(1, (8 * B - 1 + 40 + 1 + 31) / 32);
+    const size_t nch = lo.size();
+    // one device block for the chunk table and partials
+    size_t bytes = nch * 16 + (ngroups + 1) * 4 + std::max<size_t>(nch, 1) * AW * 4 + 64;
+    uint8_t* dmem = nullptr;
+    HIPCHK(hipMallocAsync((void**)&dmem, bytes, st));
+    i64* dlo = reinterpret_cast<i64*>(dmem);
+    i64* dhi = dlo + nch;
+    i32* dc0 = reinterpret_cast<i32*>(dhi + nch);
+    u32* dpart = reinterpret_cast<u32*>(dmem + ((nch * 16 + (ngroups + 1) * 4 + 15) & ~size_t(15)));
+    if (nch) {
+        HIPCHK(hipMemcpyAsync(dlo, lo.data(), nch * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dhi, hi.data(), nch * 8, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(dc0, c0.data(), (ngroups + 1) * 4, hipMemcpyHostToDevice, st));
+    struct {
+        const u8* col;
+        usize colbytes;
+        const i32* assoc;
+        const i64* lo;
+        const i64* hi;
+        const i32* c0;
+        i32 nseg;
+        u32* partials;
+        u8* out;
+    } args = {d_col, (usize)rows * (usize)B, d_associated, dlo, dhi, dc0, ngroups, dpart, d_out};
+    void* pm[] = {&args};
+    if (nch) HIPCHK(hipModuleLaunchKernel(fc, (unsigned)nch, 1, 1, 256, 1, 1, 0, st, pm, nullptr));
+    HIPCHK(hipModuleLaunchKernel(ff, (unsigned)((ngroups + 63) / 64), 1, 1, 64, 1, 1, 0, st, pm, nullptr));
+    HIPCHK(hipFreeAsync(dmem, st));
+    return UPDEC_OK;
+}
+
+int updec_compile(const char* src, size_t len, updec_module_t* out)
+{
+    if (!src || !out) return fail(UPDEC_EINVAL, "bad argument");
+    std::string err;
+    ModuleP m = ModuleCache::get().compile(std::string(src, len), &err);
+    if (!m) return fail(UPDEC_ECOMPILE, err);
+    *out = new updec_module_s{m};
+    return UPDEC_OK;
+}
+
+int updec_launch(updec_module_t m, const char* fn, uint32_t grid, uint32_t block, const void* args, size_t args_size,
+                 updec_stream_t stream)
+{
+    if (!m || !fn || grid == 0 || block == 0 || block > 1024) return fail(UPDEC_EINVAL, "bad argument");
+    std::string err;
+    hipFunction_t f = m->mod->fn(fn, &err);
+    if (!f) return fail(UPDEC_ECOMPILE, err);
+    size_t sz = args_size;
+    std::vector<uint8_t> buf((const uint8_t*)args, (const uint8_t*)args + args_size);
+    void* extra[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, buf.data(), HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END};
+    HIPCHK(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, S(stream), nullptr, extra));
+    return UPDEC_OK;
+}
+
+void updec_module_free(updec_module_t m) { delete m; }
+
+int updec_check_compile(const char* src, size_t len)
+{
+    if (!src) return fail(UPDEC_EINVAL, "bad argument");
+    std::vector<char> code;
+    std::string err;
+    if (!ModuleCache::rtc(std::string(src, len), &code, &err)) return fail(UPDEC_ECOMPILE, err);
+    return UPDEC_OK;
+}
+
+int updec_gen_compact(const updec_gen_desc* g, int64_t row0, int64_t rows, uint8_t* out, int on_device,
+                      updec_stream_t stream)
+{
+    if (!g || !out || rows < 0) return fail(UPDEC_EINVAL, "bad argument");
+    int nb = compact_bytes(g->precision);
+    if (nb <= 0 || nb > 128) return fail(UPDEC_EINVAL, "precision out of generator range");
+    if (g->kind == UPDEC_GEN_DIGITS && (g->digits < 1 || g->digits > 460)) return fail(UPDEC_EINVAL, "digits out of range");
+    updec_gen::Desc d{g->kind, g->precision, g->digits, g->neg_permille, g->lo, g->hi, g->step, g->seed, g->column_id};
+    uint64_t key = updec_gen::column_key(g->seed, g->column_id);
+    if (on_device) {
+        if (rows == 0) return UPDEC_OK;
+        hipLaunchKernelGGL(k_gen_compact, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, S(stream), d, key, row0, rows,
+                           out, nb);
+        HIPCHK(hipGetLastError());
+        return UPDEC_OK;
+    }
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (rows < 65536) nt = 1;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++) {
+        int64_t a = rows * t / nt, b = rows * (t + 1) / nt;
+        th.emplace_back([=]() {
+            for (int64_t i = a; i < b; i++) updec_gen::gen_row(d, key, (uint64_t)(row0 + i), out + i * nb, nb);
+        });
+    }
+    for (auto& x : th) x.join();
+    return UPDEC_OK;
+}
+
+int updec_gen_int32(uint64_t seed, uint32_t column_id, int32_t lo, int32_t hi, int64_t row0, int64_t rows, int32_t* out,
+                    int on_device, updec_stream_t stream)
+{
+    if (!out || rows < 0 || hi < lo) return fail(UPDEC_EINVAL, "bad argument");
+    uint64_t key = updec_gen::column_key(seed, column_id);
+    if (on_device) {
+        if (rows == 0) return UPDEC_OK;
+        hipLaunchKernelGGL(k_gen_int32, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, S(stream), key, lo, hi, row0, rows,
+                           out);
+        HIPCHK(hipGetLastError());
+        return UPDEC_OK;
+    }
+    for (int64_t i = 0; i < rows; i++) out[i] = updec_gen::gen_int32(key, (uint64_t)(row0 + i), lo, hi);
+    return UPDEC_OK;
+}
+
+int updec_gen_byte(uint64_t seed, uint32_t column_id, const uint8_t* choices, int32_t nchoices, int64_t row0,
+                   int64_t rows, uint8_t* out, int on_device, updec_stream_t stream)
+{
+    if (!out || !choices || nchoices <= 0 || nchoices > 64 || rows < 0) return fail(UPDEC_EINVAL, "bad argument");
+    uint64_t key = updec_gen::column_key(seed, column_id);
+    if (on_device) {
+        if (rows == 0) return UPDEC_OK;
+        ByteChoices ch{};
+        memcpy(ch.c, choices, nchoices);
+        ch.n = nchoices;
+        hipLaunchKernelGGL(k_gen_byte, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, S(stream), key, ch, row0, rows, out);
+        HIPCHK(hipGetLastError());
+        return UPDEC_OK;
+    }
+    for (int64_t i = 0; i < rows; i++) out[i] = updec_gen::gen_byte(key, (uint64_t)(row0 + i), choices, nchoices);
+    return UPDEC_OK;
+}
+
+}  // extern "C"
