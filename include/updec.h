@@ -50,6 +50,27 @@ int updec_version(void);
 /* Select the device (hipSetDevice) and warm the runtime. */
 int updec_init(int device);
 
+/* Device-memory and stream plumbing for hosts without their own HIP
+ * runtime binding (the Python layer uses these; C++ hosts can use HIP
+ * directly).  No counterpart in the reference, whose buffers are CUDA
+ * managed memory (src/CudaAcc/AriesEngineDef.h:72-122). */
+int updec_malloc(void** d_ptr, size_t bytes);
+int updec_free(void* d_ptr);
+int updec_memcpy_htod(void* d_dst, const void* h_src, size_t bytes, updec_stream_t stream);
+int updec_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes, updec_stream_t stream);
+int updec_memset(void* d_ptr, int value, size_t bytes, updec_stream_t stream);
+int updec_stream_create(updec_stream_t* stream);
+int updec_stream_destroy(updec_stream_t stream);
+int updec_stream_sync(updec_stream_t stream);
+int updec_device_sync(void);
+/* device properties: compute units, clock kHz, HBM bytes */
+int updec_device_info(int32_t* cus, int32_t* clock_khz, int64_t* mem_bytes, char* name, size_t name_cap);
+/* event timing helpers: record on a stream, elapsed ms between two events */
+int updec_event_create(void** ev);
+int updec_event_record(void* ev, updec_stream_t stream);
+int updec_event_elapsed(void* ev0, void* ev1, float* ms);
+int updec_event_destroy(void* ev);
+
 /* Compact bytes per row of DECIMAL(P, *): GetDecimalRealBytes
  * (src/datatypes/decimal.cu:464-467, table at :92). */
 int updec_compact_bytes(int precision);

@@ -1,0 +1,218 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the C oracle (updec_oracle.c).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this module, and only as the checker / the timed CPU baseline.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+FAITHFUL, EXTENDED = 0, 1
+OD_MAXW = 192
+
+
+class OdDec(ctypes.Structure):
+    _fields_ = [("sign", ctypes.c_int32), ("prec", ctypes.c_int32), ("frac", ctypes.c_int32),
+                ("error", ctypes.c_int32), ("v", ctypes.c_uint32 * OD_MAXW)]
+
+    @property
+    def magnitude(self):
+        m = 0
+        for i in reversed(range(OD_MAXW)):
+            m = (m << 32) | self.v[i]
+        return m
+
+    def set_magnitude(self, m):
+        for i in range(OD_MAXW):
+            self.v[i] = m & 0xffffffff
+            m >>= 32
+
+
+def build():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    D = ctypes.POINTER(OdDec)
+    P, I64 = ctypes.c_void_p, ctypes.c_int64
+    sig = {
+        "od_compact_bytes": (ctypes.c_int, [ctypes.c_int]),
+        "od_zero": (None, [D]),
+        "od_from_compact": (None, [D, P, ctypes.c_int, ctypes.c_int]),
+        "od_to_compact": (None, [D, P, ctypes.c_int]),
+        "od_from_string": (ctypes.c_int, [D, ctypes.c_char_p, ctypes.c_int]),
+        "od_cast": (None, [D, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "od_from_int32_scale": (None, [D, ctypes.c_int32, ctypes.c_int32, ctypes.c_int]),
+        "od_from_int64": (None, [D, I64]),
+        "od_rescale_up": (None, [D, ctypes.c_int, ctypes.c_int]),
+        "od_shl10": (None, [D, ctypes.c_int, ctypes.c_int]),
+        "od_shr10": (None, [D, ctypes.c_int, ctypes.c_int]),
+        "od_neg": (None, [D]),
+        "od_add": (None, [D, D, ctypes.c_int]),
+        "od_sub": (None, [D, D, ctypes.c_int]),
+        "od_mul": (None, [D, D, ctypes.c_int]),
+        "od_div": (None, [D, D, ctypes.c_int]),
+        "od_cmp": (ctypes.c_int, [D, D, ctypes.c_int]),
+        "od_is_zero": (ctypes.c_int, [D]),
+        "od_bitlen": (ctypes.c_int, [D]),
+        "od_to_nullable_image": (None, [D, P]),
+        "od_to_string": (ctypes.c_int, [D, ctypes.c_char_p, ctypes.c_int]),
+        "od_overflow_flag": (ctypes.c_int, []),
+        "od_clear_overflow": (None, []),
+        "od_plan_parse": (P, [ctypes.c_char_p]),
+        "od_plan_free": (None, [P]),
+        "od_plan_result": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "od_plan_eval": (ctypes.c_int, [P, P, P, P, I64, P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "od_plan_sum": (I64, [P, P, P, P, I64, P, P, ctypes.c_int, ctypes.c_int]),
+        "od_plan_group_sum": (ctypes.c_int, [P, P, P, P, I64, P, ctypes.c_int, P, P, ctypes.c_int, ctypes.c_int]),
+        "od_segsum_compact": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I64, P, P, ctypes.c_int, P, ctypes.c_int,
+                                             ctypes.c_int]),
+        "od_avg_from_sum": (None, [P, I64, P, ctypes.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def dec(s, mode=FAITHFUL):
+    d = OdDec()
+    lib.od_from_string(ctypes.byref(d), s.encode(), mode)
+    return d
+
+
+def dec_cast(s, P, S, mode=FAITHFUL):
+    """Decimal(P, S, "s") (decimal.cu:154-165): parse then cast to scale S."""
+    d = dec(s, mode)
+    lib.od_cast(ctypes.byref(d), P, S, mode)
+    return d
+
+
+def to_str(d):
+    buf = ctypes.create_string_buffer(4096)
+    lib.od_to_string(ctypes.byref(d), buf, 4096)
+    return buf.value.decode()
+
+
+def copy(d):
+    e = OdDec()
+    ctypes.memmove(ctypes.byref(e), ctypes.byref(d), ctypes.sizeof(OdDec))
+    return e
+
+
+def binop(op, a, b, mode=FAITHFUL):
+    r = copy(a)
+    getattr(lib, "od_" + op)(ctypes.byref(r), ctypes.byref(b), mode)
+    return r
+
+
+def from_compact(buf, P, S):
+    d = OdDec()
+    b = bytes(buf)
+    lib.od_from_compact(ctypes.byref(d), b, P, S)
+    return d
+
+
+def to_compact(d, nbytes):
+    out = ctypes.create_string_buffer(nbytes)
+    lib.od_to_compact(ctypes.byref(d), out, nbytes)
+    return out.raw
+
+
+def image(d):
+    out = ctypes.create_string_buffer(133)
+    lib.od_to_nullable_image(ctypes.byref(d), out)
+    return out.raw
+
+
+class Plan:
+    def __init__(self, text):
+        # accept the product's "F|X P S B <plan>" header form too
+        parts = text.split(" ", 4)
+        if parts[0] in ("F", "X") and len(parts) == 5:
+            text = parts[4]
+        self._h = lib.od_plan_parse(text.encode())
+        if not self._h:
+            raise ValueError("unparsable plan: " + text)
+        P, S = ctypes.c_int(), ctypes.c_int()
+        lib.od_plan_result(self._h, ctypes.byref(P), ctypes.byref(S))
+        self.P, self.S = P.value, S.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib.od_plan_free(self._h)
+            self._h = None
+
+
+def _colargs(cols, precisions, scales):
+    import numpy as np
+    n = len(cols)
+    arrs = [np.ascontiguousarray(c, dtype=np.uint8) for c in cols]
+    ptrs = (ctypes.c_void_p * max(1, n))(*[a.ctypes.data for a in arrs])
+    P = (ctypes.c_int * max(1, n))(*precisions)
+    S = (ctypes.c_int * max(1, n))(*scales)
+    return arrs, ptrs, P, S
+
+
+def plan_eval(plan, cols, precisions, scales, rows, out_bytes, mode=FAITHFUL, threads=8):
+    import numpy as np
+    arrs, ptrs, P, S = _colargs(cols, precisions, scales)
+    out = np.zeros(rows * out_bytes, dtype=np.uint8)
+    lib.od_plan_eval(plan._h, ptrs, P, S, rows, out.ctypes.data, out_bytes, mode, threads)
+    return out
+
+
+def plan_sum(plan, cols, precisions, scales, rows, mask=None, mode=FAITHFUL, threads=8):
+    import numpy as np
+    arrs, ptrs, P, S = _colargs(cols, precisions, scales)
+    out = ctypes.create_string_buffer(133)
+    m = None
+    if mask is not None:
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+    n = lib.od_plan_sum(plan._h, ptrs, P, S, rows, None if m is None else m.ctypes.data, out, mode, threads)
+    return out.raw, n
+
+
+def plan_group_sum(plan, cols, precisions, scales, rows, gid, ngroups, mode=FAITHFUL, threads=8):
+    import numpy as np
+    arrs, ptrs, P, S = _colargs(cols, precisions, scales)
+    g = np.ascontiguousarray(gid, dtype=np.int32)
+    out = np.zeros(ngroups * 133, dtype=np.uint8)
+    cnt = np.zeros(ngroups, dtype=np.int64)
+    lib.od_plan_group_sum(plan._h, ptrs, P, S, rows, g.ctypes.data, ngroups, out.ctypes.data, cnt.ctypes.data, mode,
+                          threads)
+    return out.reshape(ngroups, 133), cnt
+
+
+def segsum_compact(col, P, S, rows, associated, groups, mode=FAITHFUL, threads=8):
+    import numpy as np
+    c = np.ascontiguousarray(col, dtype=np.uint8)
+    a = None if associated is None else np.ascontiguousarray(associated, dtype=np.int32)
+    g = np.ascontiguousarray(groups, dtype=np.int32)
+    out = np.zeros(len(g) * 133, dtype=np.uint8)
+    lib.od_segsum_compact(c.ctypes.data, P, S, rows, None if a is None else a.ctypes.data, g.ctypes.data, len(g),
+                          out.ctypes.data, mode, threads)
+    return out.reshape(len(g), 133)
+
+
+def avg_from_sum(img133, count, mode=FAITHFUL):
+    out = ctypes.create_string_buffer(133)
+    lib.od_avg_from_sum(bytes(img133), count, out, mode)
+    return out.raw
+
+
+def image_value(img):
+    """(sign, prec, frac, error, magnitude) of a 133-byte image."""
+    img = bytes(img)
+    m = int.from_bytes(img[5:133], "little")
+    return img[1], img[2], img[3], img[4], m

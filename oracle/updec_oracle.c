@@ -619,6 +619,19 @@ static void* run_eval(void* arg)
     return NULL;
 }
 
+/* SUM sees the materialized result column: the value is stored as compact
+ * (P_root, S_root) bytes and decoded again before it is added
+ * (AriesExprCalcNode.cpp:1623-1700 materializes the child expression). */
+static void eval_stored(const evctx* c, int64_t row, od_dec* r)
+{
+    const pnode* root = &c->p->n[c->p->root];
+    eval_node(c, c->p->root, row, r);
+    uint8_t buf[OD_MAXW * 4];
+    int len = od_compact_bytes(root->P);
+    od_to_compact(r, buf, len);
+    od_from_compact(r, buf, root->P, root->S);
+}
+
 static void* run_sum(void* arg)
 {
     job* j = (job*)arg;
@@ -626,7 +639,7 @@ static void* run_sum(void* arg)
     j->n = 0;
     for (int64_t i = j->lo; i < j->hi; i++) {
         if (j->mask && !j->mask[i]) continue;
-        eval_node(&j->c, j->c.p->root, i, &r);
+        eval_stored(&j->c, i, &r);
         od_add(j->acc, &r, j->c.mode);
         j->n++;
     }
@@ -640,7 +653,7 @@ static void* run_gsum(void* arg)
     for (int64_t i = j->lo; i < j->hi; i++) {
         int g = j->gid[i];
         if (g < 0) continue;
-        eval_node(&j->c, j->c.p->root, i, &r);
+        eval_stored(&j->c, i, &r);
         od_add(&j->acc[g], &r, j->c.mode);
         j->counts[g]++;
     }

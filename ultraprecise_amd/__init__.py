@@ -3,7 +3,8 @@
 Python host layer over the C ABI of ``libupdec.so`` (include/updec.h).
 The compute path is HIP only: when the library is missing, importing this
 package raises instead of falling back to anything else.  PyTorch is used
-for device memory and streams.
+for nothing on the GPU side: device memory and streams come from the
+library itself, so one HIP runtime serves the whole process.
 """
 import ctypes
 import os
@@ -48,6 +49,21 @@ def _load():
         "updec_version": (ctypes.c_int, []),
         "updec_init": (ctypes.c_int, [ctypes.c_int]),
         "updec_compact_bytes": (ctypes.c_int, [ctypes.c_int]),
+        "updec_malloc": (ctypes.c_int, [ctypes.POINTER(P), SZ]),
+        "updec_free": (ctypes.c_int, [P]),
+        "updec_memcpy_htod": (ctypes.c_int, [P, P, SZ, P]),
+        "updec_memcpy_dtoh": (ctypes.c_int, [P, P, SZ, P]),
+        "updec_memset": (ctypes.c_int, [P, ctypes.c_int, SZ, P]),
+        "updec_stream_create": (ctypes.c_int, [ctypes.POINTER(P)]),
+        "updec_stream_destroy": (ctypes.c_int, [P]),
+        "updec_stream_sync": (ctypes.c_int, [P]),
+        "updec_device_sync": (ctypes.c_int, []),
+        "updec_device_info": (ctypes.c_int, [ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(I64),
+                                             ctypes.c_char_p, SZ]),
+        "updec_event_create": (ctypes.c_int, [ctypes.POINTER(P)]),
+        "updec_event_record": (ctypes.c_int, [P, P]),
+        "updec_event_elapsed": (ctypes.c_int, [P, P, ctypes.POINTER(ctypes.c_float)]),
+        "updec_event_destroy": (ctypes.c_int, [P]),
         "updec_column_maxbits": (ctypes.c_int, [P, ctypes.c_int, I64, ctypes.POINTER(I32), P]),
         "updec_expr_compile": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ColumnDesc), ctypes.c_int, ctypes.c_int,
                                               ctypes.POINTER(P)]),
@@ -84,6 +100,10 @@ _lib = _load()
 
 # C ABI symbols declared in include/updec.h (checked by the CPU tests)
 EXPORTED = ["updec_last_error", "updec_version", "updec_init", "updec_compact_bytes", "updec_column_maxbits",
+            "updec_malloc", "updec_free", "updec_memcpy_htod", "updec_memcpy_dtoh", "updec_memset",
+            "updec_stream_create", "updec_stream_destroy", "updec_stream_sync", "updec_device_sync",
+            "updec_device_info", "updec_event_create", "updec_event_record", "updec_event_elapsed",
+            "updec_event_destroy",
             "updec_expr_compile", "updec_expr_info", "updec_expr_plan", "updec_expr_source", "updec_expr_calc",
             "updec_expr_free", "updec_agg_compile", "updec_agg_run", "updec_agg_last_kernel_ms", "updec_agg_source",
             "updec_agg_free", "updec_div_int64", "updec_segsum_compact", "updec_compile", "updec_launch",
@@ -112,14 +132,96 @@ def compact_bytes(precision):
 
 
 def _ptr(t):
-    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+    return ctypes.c_void_p(0 if t is None else t.ptr)
 
 
 def _stream(stream):
-    if stream is None:
-        import torch
-        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(0 if stream is None else stream.handle)
+
+
+class Stream:
+    """A HIP stream created by the library (default priority)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(_lib.updec_stream_create(ctypes.byref(h)))
+        self.handle = h.value
+
+    def sync(self):
+        _check(_lib.updec_stream_sync(ctypes.c_void_p(self.handle)))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            _lib.updec_stream_destroy(ctypes.c_void_p(self.handle))
+            self.handle = None
+
+
+class DeviceBuffer:
+    """Device memory owned by this object (hipMalloc through the library)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(_lib.updec_malloc(ctypes.byref(p), max(4, self.nbytes)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, arr, stream=None):
+        import numpy as np
+        a = np.ascontiguousarray(arr)
+        b = cls(a.nbytes)
+        _check(_lib.updec_memcpy_htod(ctypes.c_void_p(b.ptr), a.ctypes.data_as(ctypes.c_void_p), a.nbytes,
+                                      _stream(stream)))
+        return b
+
+    def numpy(self, dtype="uint8", count=None, stream=None):
+        import numpy as np
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else int(count)
+        out = np.empty(n, dtype=dt)
+        _check(_lib.updec_memcpy_dtoh(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr), out.nbytes,
+                                      _stream(stream)))
+        return out
+
+    def zero(self, stream=None):
+        _check(_lib.updec_memset(ctypes.c_void_p(self.ptr), 0, self.nbytes, _stream(stream)))
+        return self
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            _lib.updec_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+
+class Event:
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(_lib.updec_event_create(ctypes.byref(h)))
+        self.handle = h.value
+
+    def record(self, stream=None):
+        _check(_lib.updec_event_record(ctypes.c_void_p(self.handle), _stream(stream)))
+
+    def elapsed_ms(self, end):
+        ms = ctypes.c_float(0)
+        _check(_lib.updec_event_elapsed(ctypes.c_void_p(self.handle), ctypes.c_void_p(end.handle), ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            _lib.updec_event_destroy(ctypes.c_void_p(self.handle))
+            self.handle = None
+
+
+def device_sync():
+    _check(_lib.updec_device_sync())
+
+
+def device_info():
+    cus, clk, mem = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+    name = ctypes.create_string_buffer(256)
+    _check(_lib.updec_device_info(ctypes.byref(cus), ctypes.byref(clk), ctypes.byref(mem), name, 256))
+    return {"cus": cus.value, "clock_khz": clk.value, "mem_bytes": mem.value, "name": name.value.decode()}
 
 
 def _col_array(cols):
@@ -162,9 +264,9 @@ class Expr:
         return _text(_lib.updec_expr_source, self._h)
 
     def calc(self, columns, rows, out, stream=None):
-        """columns: device uint8 tensors in descriptor order; out: device
-        uint8 tensor of rows * out_bytes bytes."""
-        ptrs = (ctypes.c_void_p * max(1, len(columns)))(*[c.data_ptr() for c in columns])
+        """columns: DeviceBuffers in descriptor order; out: DeviceBuffer of
+        at least rows * out_bytes bytes, 4-byte aligned."""
+        ptrs = (ctypes.c_void_p * max(1, len(columns)))(*[c.ptr for c in columns])
         _check(_lib.updec_expr_calc(self._h, ptrs, rows, _ptr(out), _stream(stream)))
 
     def __del__(self):
@@ -197,9 +299,9 @@ class Agg:
         return _text(_lib.updec_agg_source, self._h, ctypes.c_int32(ngroups))
 
     def run(self, columns, rows, images, counts, filter_col=None, key0=None, key1=None, max_groups=255, stream=None):
-        """Returns (ngroups, keys).  images: device uint8 tensor of at least
-        max_groups * nexprs * 133 bytes; counts: device int64 tensor."""
-        ptrs = (ctypes.c_void_p * max(1, len(columns)))(*[c.data_ptr() for c in columns])
+        """Returns (ngroups, keys).  images: DeviceBuffer of at least
+        max_groups * nexprs * 133 bytes; counts: DeviceBuffer of max_groups int64."""
+        ptrs = (ctypes.c_void_p * max(1, len(columns)))(*[c.ptr for c in columns])
         ng = ctypes.c_int32(0)
         keys = (ctypes.c_uint16 * max(1, max_groups))()
         _check(_lib.updec_agg_run(self._h, ptrs, _ptr(filter_col), _ptr(key0), _ptr(key1), rows, _ptr(images),
@@ -251,9 +353,8 @@ def gen_compact_host(desc, rows, row0=0):
 
 
 def gen_compact_device(desc, rows, row0=0, stream=None):
-    import torch
     nb = compact_bytes(desc.precision)
-    out = torch.empty(max(1, rows * nb), dtype=torch.uint8, device="cuda")
+    out = DeviceBuffer(max(4, rows * nb + 16))
     _check(_lib.updec_gen_compact(ctypes.byref(desc), row0, rows, _ptr(out), 1, _stream(stream)))
     return out
 
@@ -266,8 +367,7 @@ def gen_int32_host(seed, column_id, lo, hi, rows, row0=0):
 
 
 def gen_int32_device(seed, column_id, lo, hi, rows, row0=0, stream=None):
-    import torch
-    out = torch.empty(max(1, rows), dtype=torch.int32, device="cuda")
+    out = DeviceBuffer(max(4, rows * 4))
     _check(_lib.updec_gen_int32(seed, column_id, lo, hi, row0, rows, _ptr(out), 1, _stream(stream)))
     return out
 
@@ -282,8 +382,7 @@ def gen_byte_host(seed, column_id, choices, rows, row0=0):
 
 
 def gen_byte_device(seed, column_id, choices, rows, row0=0, stream=None):
-    import torch
     ch = (ctypes.c_uint8 * len(choices))(*choices)
-    out = torch.empty(max(1, rows), dtype=torch.uint8, device="cuda")
+    out = DeviceBuffer(max(4, rows))
     _check(_lib.updec_gen_byte(seed, column_id, ch, len(choices), row0, rows, _ptr(out), 1, _stream(stream)))
     return out

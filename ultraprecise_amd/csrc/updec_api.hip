@@ -392,6 +392,94 @@ int updec_init(int device)
     return UPDEC_OK;
 }
 
+int updec_malloc(void** d_ptr, size_t bytes)
+{
+    if (!d_ptr) return fail(UPDEC_EINVAL, "bad argument");
+    HIPCHK(hipMalloc(d_ptr, bytes ? bytes : 4));
+    return UPDEC_OK;
+}
+int updec_free(void* d_ptr)
+{
+    if (d_ptr) HIPCHK(hipFree(d_ptr));
+    return UPDEC_OK;
+}
+int updec_memcpy_htod(void* d_dst, const void* h_src, size_t bytes, updec_stream_t stream)
+{
+    if (bytes) HIPCHK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, S(stream)));
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    return UPDEC_OK;
+}
+int updec_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes, updec_stream_t stream)
+{
+    if (bytes) HIPCHK(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, S(stream)));
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    return UPDEC_OK;
+}
+int updec_memset(void* d_ptr, int value, size_t bytes, updec_stream_t stream)
+{
+    if (bytes) HIPCHK(hipMemsetAsync(d_ptr, value, bytes, S(stream)));
+    return UPDEC_OK;
+}
+int updec_stream_create(updec_stream_t* stream)
+{
+    hipStream_t s = nullptr;
+    HIPCHK(hipStreamCreate(&s));
+    *stream = s;
+    return UPDEC_OK;
+}
+int updec_stream_destroy(updec_stream_t stream)
+{
+    if (stream) HIPCHK(hipStreamDestroy(S(stream)));
+    return UPDEC_OK;
+}
+int updec_stream_sync(updec_stream_t stream)
+{
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    return UPDEC_OK;
+}
+int updec_device_sync(void)
+{
+    HIPCHK(hipDeviceSynchronize());
+    return UPDEC_OK;
+}
+int updec_device_info(int32_t* cus, int32_t* clock_khz, int64_t* mem_bytes, char* name, size_t name_cap)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, dev));
+    if (cus) *cus = p.multiProcessorCount;
+    if (clock_khz) *clock_khz = p.clockRate;
+    if (mem_bytes) *mem_bytes = (int64_t)p.totalGlobalMem;
+    if (name && name_cap) {
+        snprintf(name, name_cap, "%s %s", p.name, p.gcnArchName);
+    }
+    return UPDEC_OK;
+}
+int updec_event_create(void** ev)
+{
+    hipEvent_t e = nullptr;
+    HIPCHK(hipEventCreate(&e));
+    *ev = e;
+    return UPDEC_OK;
+}
+int updec_event_record(void* ev, updec_stream_t stream)
+{
+    HIPCHK(hipEventRecord(reinterpret_cast<hipEvent_t>(ev), S(stream)));
+    return UPDEC_OK;
+}
+int updec_event_elapsed(void* ev0, void* ev1, float* ms)
+{
+    HIPCHK(hipEventSynchronize(reinterpret_cast<hipEvent_t>(ev1)));
+    HIPCHK(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(ev0), reinterpret_cast<hipEvent_t>(ev1)));
+    return UPDEC_OK;
+}
+int updec_event_destroy(void* ev)
+{
+    if (ev) HIPCHK(hipEventDestroy(reinterpret_cast<hipEvent_t>(ev)));
+    return UPDEC_OK;
+}
+
 int updec_compact_bytes(int precision) { return compact_bytes(precision); }
 
 int updec_column_maxbits(const uint8_t* d_col, int precision, int64_t rows, int32_t* bits_out, updec_stream_t stream)
