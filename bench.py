@@ -1,0 +1,264 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X decimal expression + SUM path on one GPU.
+
+Headline workload: grouped aggregation of the TPC-H Q1 shape, 60,000,000
+rows (seed 2): filter ship <= 2435, group by (flag1, flag2), SUM(qty),
+SUM(price), SUM(price*(1-disc)), SUM(price*(1-disc)*(1+tax)), AVG(qty),
+AVG(price), AVG(disc), COUNT(*).  One step = one full pass (group
+discovery, fused filter/expression/SUM kernel, final combine, the three
+AVG divisions) over the device-resident columns.
+
+Also reported: the plain-method workload (1,000,000 rows, SUM(p*(1-d)) and
+SUM(q*p)) and the precision sweep ((a+b)*c - a/d over 10,000,000 rows at
+P = 18, 38, 76, 150, 300), each with its kernel time and bytes.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "Mrows/s decimal expr+SUM (fraction of HBM roofline); batched vs per-query p99"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def headline(U, W, steps, warmup, cpu_rows, threads):
+    C = W.CONFIG3
+    rows = C["rows"]
+    seed = C["seed"]
+    dev = [U.gen_compact_device(W.gen_desc(U, seed, i, c), rows) for i, c in enumerate(C["cols"])]
+    f1 = U.gen_byte_device(seed, W.COL_FLAG1, C["flag1"], rows)
+    f2 = U.gen_byte_device(seed, W.COL_FLAG2, C["flag2"], rows)
+    ship = U.gen_int32_device(seed, W.COL_SHIP, C["ship"][0], C["ship"][1], rows)
+    U.device_sync()
+    # column statistics are computed once at load time, outside the timed region
+    bits = [U.column_maxbits(t, c[1], rows) for t, c in zip(dev, C["cols"])]
+    cols = [(c[0], c[1], c[2], b) for c, b in zip(C["cols"], bits)]
+    t0 = time.time()
+    agg = U.Agg(C["exprs"], cols, filter=C["filter"], nkeys=2)
+    ne = len(C["exprs"])
+    maxg = 16
+    images = U.DeviceBuffer(maxg * ne * 133).zero()
+    counts = U.DeviceBuffer(maxg * 8).zero()
+    avgs = U.DeviceBuffer(3 * maxg * 133).zero()
+
+    def step():
+        ng, keys = agg.run(dev, rows, images, counts, filter_col=ship, key0=f1, key1=f2, max_groups=maxg)
+        for j, k in enumerate((0, 1, 4)):  # AVG(qty), AVG(price), AVG(disc)
+            U.div_int64(images, ne * 133, counts, avgs, ng, sums_offset=k * 133, out_offset=j * maxg * 133)
+        return ng
+
+    ng = step()
+    U.device_sync()
+    compile_s = time.time() - t0
+    for _ in range(warmup):
+        step()
+    U.device_sync()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+        kms.append(agg.last_kernel_ms())
+    U.device_sync()
+    total = time.perf_counter() - t0
+    ms_per_step = 1000.0 * total / steps
+    kernel_ms = sum(kms) / len(kms)
+    alg_bytes = 34 * rows  # flag1 + flag2 + ship(4) + 4 x DECIMAL(15,2) (7 B)
+    achieved = alg_bytes / (kernel_ms / 1000.0) / 1e9
+    res = {
+        "rows": rows, "groups": ng, "ms_per_step": ms_per_step, "kernel_ms": kernel_ms,
+        "alg_bytes": alg_bytes, "achieved_gbps": achieved, "compile_s": compile_s,
+        "column_bits": bits,
+    }
+    # CPU baseline: the oracle's restatement of the reference's host Decimal
+    # path on a bounded sample of the same rows
+    if cpu_rows > 0:
+        from oracle import oracle as O
+        import numpy as np
+        n = min(cpu_rows, rows)
+        host = [t.numpy(count=n * U.compact_bytes(c[1])) for t, c in zip(dev, C["cols"])]
+        hf1, hf2 = f1.numpy(count=n), f2.numpy(count=n)
+        hship = ship.numpy("int32", count=n)
+        P = [c[1] for c in cols]
+        S = [c[2] for c in cols]
+        plans = [O.Plan(U.Expr(t, cols).plan) for t in C["exprs"]]
+        t0 = time.perf_counter()
+        mask = hship <= C["filter"][1]
+        key = (hf1.astype(np.int32) << 8) | hf2
+        keys = np.unique(key[mask])
+        lut = np.full(65536, -1, dtype=np.int32)
+        lut[keys] = np.arange(len(keys), dtype=np.int32)
+        gid = np.where(mask, lut[key], -1).astype(np.int32)
+        sums = []
+        for p in plans:
+            img, cnt = O.plan_group_sum(p, host, P, S, n, gid, len(keys), threads=threads)
+            sums.append(img)
+        for k in (0, 1, 4):
+            for g in range(len(keys)):
+                O.avg_from_sum(bytes(sums[k][g]), int(cnt[g]))
+        cpu_s = time.perf_counter() - t0
+        res["cpu"] = {"rows": n, "seconds": cpu_s, "mrows_per_s": n / cpu_s / 1e6, "threads": threads}
+    return res
+
+
+def plain_method(U, W, steps, warmup):
+    C = W.CONFIG1
+    rows = C["rows"]
+    dev = [U.gen_compact_device(W.gen_desc(U, C["seed"], i, c), rows) for i, c in enumerate(C["cols"])]
+    U.device_sync()
+    cols = [(c[0], c[1], c[2]) for c in C["cols"]]
+    agg = U.Agg(C["exprs"], cols)
+    images = U.DeviceBuffer(len(C["exprs"]) * 133).zero()
+    counts = U.DeviceBuffer(8).zero()
+    for _ in range(warmup + 1):
+        agg.run(dev, rows, images, counts)
+    U.device_sync()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        agg.run(dev, rows, images, counts)
+        kms.append(agg.last_kernel_ms())
+    U.device_sync()
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    kernel_ms = sum(kms) / len(kms)
+    return {"rows": rows, "ms_per_step": ms, "kernel_ms": kernel_ms, "mrows_per_s": rows / ms / 1e3,
+            "kernel_mrows_per_s": rows / kernel_ms / 1e3, "alg_bytes": 21 * rows,
+            "achieved_gbps": 21 * rows / (kernel_ms / 1000.0) / 1e9}
+
+
+def precision_sweep(U, W, steps, rows_override=None):
+    C = W.CONFIG2
+    rows = rows_override or C["rows"]
+    out = {}
+    for P, S in C["precisions"]:
+        cols = [(n, P, S) for n in "abcd"]
+        dev = [U.gen_compact_device(W.digits_desc(U, C["seed"], i, P), rows) for i in range(4)]
+        e = U.Expr(C["expr"], cols)
+        obuf = U.DeviceBuffer(rows * e.out_bytes + 16)
+        t0 = time.time()
+        e.calc(dev, rows, obuf)
+        U.device_sync()
+        first_s = time.time() - t0
+        ev0, ev1 = U.Event(), U.Event()
+        n = steps if P <= 76 else max(1, steps // 3)
+        ev0.record()
+        for _ in range(n):
+            e.calc(dev, rows, obuf)
+        ev1.record()
+        ms = ev0.elapsed_ms(ev1) / n
+        alg = (4 * U.compact_bytes(P) + e.out_bytes) * rows
+        out[f"P{P}"] = {"rows": rows, "kernel_ms": ms, "mrows_per_s": rows / ms / 1e3, "alg_bytes": alg,
+                        "achieved_gbps": alg / (ms / 1000.0) / 1e9, "out_type": [e.precision, e.scale],
+                        "faithful": e.faithful, "first_call_s": first_s}
+        del dev, obuf
+    return out
+
+
+def copy_bandwidth(U, nbytes=1 << 30, reps=5):
+    a = U.DeviceBuffer(nbytes)
+    b = U.DeviceBuffer(nbytes)
+    U.copy_kernel(b, a, nbytes)
+    U.device_sync()
+    ev0, ev1 = U.Event(), U.Event()
+    ev0.record()
+    for _ in range(reps):
+        U.copy_kernel(b, a, nbytes)
+    ev1.record()
+    ms = ev0.elapsed_ms(ev1) / reps
+    return {"bytes_moved": 2 * nbytes, "ms": ms, "gbps": 2 * nbytes / (ms / 1000.0) / 1e9}
+
+
+def load_traffic():
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel
+    calibration), when present."""
+    path = os.path.join(ROOT, "profiles", "agg_main_q1_60M_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-rows", type=int, default=2_000_000, help="rows of the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-extra", action="store_true", help="headline only")
+    ap.add_argument("--sweep-rows", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(the sharded form is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import ultraprecise_amd as U
+    from ultraprecise_amd import workloads as W
+    U.init(0)
+    info = U.device_info()
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    log("device:", info)
+    h = headline(U, W, args.steps, args.warmup, args.cpu_rows, threads)
+    log("headline:", h)
+    extra = {}
+    if not args.no_extra:
+        extra["copy"] = copy_bandwidth(U)
+        log("copy:", extra["copy"])
+        extra["plain_1M"] = plain_method(U, W, args.steps, args.warmup)
+        log("plain:", extra["plain_1M"])
+        extra["sweep_10M"] = precision_sweep(U, W, max(3, args.steps // 4), args.sweep_rows or None)
+        log("sweep:", extra["sweep_10M"])
+    value = h["rows"] / h["ms_per_step"] / 1e3  # Mrows/s
+    achieved = h["achieved_gbps"]
+    cpu = h.get("cpu")
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mrows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(h["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded counter-based generator, device resident)",
+        "config": {
+            "workload": "grouped aggregation, TPC-H Q1 shape: 60,000,000 rows, seed 2, filter ship<=2435, "
+                        "group by (flag1,flag2), 4 SUM + 3 AVG + COUNT over DECIMAL(15,2)",
+            "rows": h["rows"], "groups": h["groups"], "column_bits": h["column_bits"],
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": load_traffic(),
+            "kernel": "agg_main", "kernel_ms": round(h["kernel_ms"], 4), "alg_bytes_per_launch": h["alg_bytes"],
+        },
+        "cpu_baseline": None if cpu is None else {
+            "value": round(cpu["mrows_per_s"], 4), "unit": "Mrows/s", "cores": cpu["threads"], "kind": "port",
+            "sample": f"first {cpu['rows']:,} of {h['rows']:,} rows, same query (4 SUM + 3 AVG + COUNT), "
+                      f"oracle restatement of the reference's host Decimal, {cpu['threads']} threads",
+        },
+        "device": info["name"],
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

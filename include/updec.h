@@ -71,6 +71,11 @@ int updec_event_record(void* ev, updec_stream_t stream);
 int updec_event_elapsed(void* ev0, void* ev1, float* ms);
 int updec_event_destroy(void* ev);
 
+/* Plain streaming copy kernel (16-byte vector loads/stores), used to
+ * calibrate HBM counters and bandwidth; bytes must be a multiple of 16 and
+ * both pointers 16-byte aligned.  No counterpart in the reference. */
+int updec_copy_kernel(void* d_dst, const void* d_src, size_t bytes, updec_stream_t stream);
+
 /* Compact bytes per row of DECIMAL(P, *): GetDecimalRealBytes
  * (src/datatypes/decimal.cu:464-467, table at :92). */
 int updec_compact_bytes(int precision);

@@ -1,0 +1,127 @@
+"""GPU parity for the grouped aggregation (filter, two 1-byte keys, fused
+SUMs, COUNT, AVG) and the segmented SUM / AVG entry points."""
+import numpy as np
+import pytest
+
+from ultraprecise_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def config3_columns(U, rows, seed):
+    C = W.CONFIG3
+    dev, host = [], []
+    for i, c in enumerate(C["cols"]):
+        d = W.gen_desc(U, seed, i, c)
+        t = U.gen_compact_device(d, rows)
+        dev.append(t)
+        host.append(t.numpy(count=rows * U.compact_bytes(c[1])))
+    f1 = U.gen_byte_device(seed, W.COL_FLAG1, C["flag1"], rows)
+    f2 = U.gen_byte_device(seed, W.COL_FLAG2, C["flag2"], rows)
+    ship = U.gen_int32_device(seed, W.COL_SHIP, C["ship"][0], C["ship"][1], rows)
+    hf1, hf2, hship = f1.numpy(count=rows), f2.numpy(count=rows), ship.numpy("int32", count=rows)
+    return dev, host, (f1, f2, ship), (hf1, hf2, hship)
+
+
+def host_groups(hf1, hf2, hship, limit):
+    mask = hship <= limit
+    key = (hf1.astype(np.int32) << 8) | hf2
+    keys = np.unique(key[mask])
+    gid = np.full(len(key), -1, dtype=np.int32)
+    for g, k in enumerate(keys):
+        gid[(key == k) & mask] = g
+    return keys, gid
+
+
+@pytest.mark.parametrize("rows", [1000, 1_000_000])
+def test_config3_grouped_bit_exact(U, O, gpu, rows):
+    C = W.CONFIG3
+    dev, host, (f1, f2, ship), (hf1, hf2, hship) = config3_columns(U, rows, C["seed"])
+    bits = [U.column_maxbits(t, c[1], rows) for t, c in zip(dev, C["cols"])]
+    # statistics agree with the host view of the data
+    for b, h, c in zip(bits, host, C["cols"]):
+        nb = U.compact_bytes(c[1])
+        mags = [int.from_bytes(bytes(h[i * nb:(i + 1) * nb]), "little") & ((1 << (8 * nb - 1)) - 1)
+                for i in range(0, rows, max(1, rows // 2000))]
+        assert max(m.bit_length() for m in mags) <= b
+    cols = [(c[0], c[1], c[2], b) for c, b in zip(C["cols"], bits)]
+    agg = U.Agg(C["exprs"], cols, filter=C["filter"], nkeys=2)
+    maxg = 16
+    images = U.DeviceBuffer(maxg * len(C["exprs"]) * 133).zero()
+    counts = U.DeviceBuffer(maxg * 8).zero()
+    ng, keys = agg.run(dev, rows, images, counts, filter_col=ship, key0=f1, key1=f2, max_groups=maxg)
+    hkeys, gid = host_groups(hf1, hf2, hship, C["filter"][1])
+    assert ng == len(hkeys)
+    assert keys == [int((k >> 8) | ((k & 0xff) << 8)) for k in hkeys]
+    got = images.numpy().reshape(maxg, len(C["exprs"]), 133)
+    gotc = counts.numpy("int64")[:ng]
+    P = [c[1] for c in cols]
+    S = [c[2] for c in cols]
+    for k, text in enumerate(C["exprs"]):
+        e = U.Expr(text, cols)
+        want, wc = O.plan_group_sum(O.Plan(e.plan), host, P, S, rows, gid, ng)
+        assert np.array_equal(gotc, wc)
+        for g in range(ng):
+            assert bytes(got[g, k]) == bytes(want[g]), f"SUM({text}) group {g}"
+    # AVG(qty), AVG(price), AVG(disc) through the division kernel
+    avg2 = U.DeviceBuffer(maxg * 133).zero()
+    for k in (0, 1, 4):
+        # contiguous copy of column k's sums
+        sums = U.DeviceBuffer.from_numpy(np.ascontiguousarray(got[:ng, k]).reshape(-1))
+        U.div_int64(sums, 133, counts, avg2, ng)
+        ga2 = avg2.numpy().reshape(maxg, 133)
+        for g in range(ng):
+            want = O.avg_from_sum(bytes(got[g, k]), int(gotc[g]))
+            assert bytes(ga2[g]) == want, f"AVG expr {k} group {g}"
+
+
+def test_div_int64_edges(U, O, gpu):
+    """AVG division on crafted sums: zero, negative, path (i) wrap region, wide sums, zero count"""
+    rng = np.random.default_rng(5)
+    imgs, cnts = [], []
+    for i in range(400):
+        d = O.OdDec()
+        k = i % 5
+        if k == 0:
+            m = 0
+        elif k == 1:
+            m = int(rng.integers(1, 1 << 40))
+        elif k == 2:
+            m = int(rng.integers(1 << 48, 1 << 51))
+        elif k == 3:
+            m = int.from_bytes(rng.bytes(40), "little")
+        else:
+            m = int(rng.integers(1 << 60, 1 << 63))
+        d.set_magnitude(m)
+        d.sign = int(rng.integers(0, 2))
+        d.frac = int(rng.integers(0, 8))
+        d.prec = 30
+        imgs.append(O.image(d))
+        cnts.append(0 if i % 97 == 0 else int(rng.integers(1, 1 << 40)))
+    n = len(imgs)
+    src = U.DeviceBuffer.from_numpy(np.frombuffer(b"".join(imgs), dtype=np.uint8))
+    cnt = U.DeviceBuffer.from_numpy(np.array(cnts, dtype=np.int64))
+    out = U.DeviceBuffer(n * 133).zero()
+    U.div_int64(src, 133, cnt, out, n, mode=U.MODE_FAITHFUL)
+    got = out.numpy().reshape(n, 133)
+    for i in range(n):
+        assert bytes(got[i]) == O.avg_from_sum(imgs[i], cnts[i], mode=O.FAITHFUL), i
+
+
+@pytest.mark.parametrize("P,S", [(15, 2), (38, 10), (76, 20)])
+def test_segsum_compact_bit_exact(U, O, gpu, P, S):
+    rng = np.random.default_rng(P)
+    rows = 300_000
+    host = U.gen_compact_host(U.gen_desc(U.GEN_DIGITS, P, 9, P, digits=P - 3, neg_permille=500), rows)
+    assoc = rng.permutation(rows).astype(np.int32)
+    cuts = np.sort(rng.choice(np.arange(1, rows), size=40, replace=False))
+    groups = np.concatenate([[0], cuts, [rows - 5, rows - 5]]).astype(np.int32)  # includes empty segments
+    groups = np.sort(groups)
+    col = U.DeviceBuffer.from_numpy(host)
+    da = U.DeviceBuffer.from_numpy(assoc)
+    dg = U.DeviceBuffer.from_numpy(groups)
+    out = U.DeviceBuffer(len(groups) * 133).zero()
+    U.segsum_compact(col, P, S, rows, da, dg, len(groups), out)
+    got = out.numpy().reshape(-1, 133)
+    want = O.segsum_compact(host, P, S, rows, assoc, groups)
+    assert np.array_equal(got, want)

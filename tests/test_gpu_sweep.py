@@ -1,0 +1,110 @@
+"""GPU parity for the precision sweep expression (a+b)*c - a/d and for the
+division paths of DivOrMod, bit-exact on every row."""
+import numpy as np
+import pytest
+
+from ultraprecise_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def encode(vals, P):
+    """signed Python ints -> compact bytes of DECIMAL(P, *)"""
+    import ultraprecise_amd as U
+    nb = U.compact_bytes(P)
+    out = bytearray()
+    for v in vals:
+        m = abs(v) % (1 << (8 * nb))
+        b = bytearray(m.to_bytes(nb, "little"))
+        if v < 0:
+            b[-1] |= 0x80
+        out += b
+    return np.frombuffer(bytes(out), dtype=np.uint8).copy()
+
+
+def run_calc(U, O, text, cols, host_cols, rows, mode=0):
+    e = U.Expr(text, cols, mode=mode)
+    dev = [U.DeviceBuffer.from_numpy(h) for h in host_cols]
+    out = U.DeviceBuffer(rows * e.out_bytes + 4).zero()
+    e.calc(dev, rows, out)
+    full = out.numpy()
+    got = full[: rows * e.out_bytes]
+    want = O.plan_eval(O.Plan(e.plan), host_cols, [c[1] for c in cols], [c[2] for c in cols], rows, e.out_bytes,
+                       mode=O.FAITHFUL if e.faithful else O.EXTENDED)
+    return e, got, want, full
+
+
+@pytest.mark.parametrize("P,S", W.CONFIG2["precisions"])
+def test_sweep_rows_bit_exact(U, O, gpu, P, S):
+    rows = 20_000 if P <= 76 else 4_000
+    cols = [(n, P, S) for n in "abcd"]
+    host = [U.gen_compact_host(W.digits_desc(U, W.CONFIG2["seed"], i, P), rows) for i in range(4)]
+    e, got, want, full = run_calc(U, O, W.CONFIG2["expr"], cols, host, rows)
+    assert e.faithful == (P <= 76)
+    bad = np.nonzero((got != want).reshape(rows, -1).any(axis=1))[0]
+    assert bad.size == 0, f"P={P}: {bad.size} rows differ, first {bad[:5]}"
+    assert not full[rows * e.out_bytes:].any()
+
+
+def _classes(rng, n):
+    """magnitudes chosen to hit every DivOrMod path at S=(*,0..4)"""
+    vals = []
+    for _ in range(n):
+        k = rng.integers(0, 8)
+        if k == 0:
+            vals.append(0)
+        elif k == 1:
+            vals.append(int(rng.integers(1, 1 << 20)))
+        elif k == 2:  # around the uint64 path boundary
+            vals.append(int(rng.integers(1 << 49, 1 << 51)))
+        elif k == 3:  # single-word divisors with large remainders
+            vals.append(int(rng.integers(1 << 31, 1 << 32)))
+        elif k == 4:
+            vals.append(int(rng.integers(1 << 62, 1 << 63)) * 2 + int(rng.integers(0, 2)))
+        elif k == 5:
+            vals.append(int.from_bytes(rng.bytes(12), "little") >> int(rng.integers(0, 8)))
+        elif k == 6:
+            vals.append(int.from_bytes(rng.bytes(15), "little") >> 1)
+        else:
+            vals.append(int(rng.integers(1, 1 << 62)))
+    return vals
+
+
+@pytest.mark.parametrize("Sa,Sb", [(0, 0), (2, 0), (4, 4), (0, 6), (8, 2)])
+def test_division_paths_bit_exact(U, O, gpu, Sa, Sb):
+    rng = np.random.default_rng(100 + 10 * Sa + Sb)
+    rows = 50_000
+    P = 38
+    a = [v * (1 if rng.integers(0, 2) else -1) for v in _classes(rng, rows)]
+    b = [v * (1 if rng.integers(0, 2) else -1) for v in _classes(rng, rows)]
+    cols = [("a", P, Sa), ("b", P, Sb)]
+    e, got, want, _ = run_calc(U, O, "a/b", cols, [encode(a, P), encode(b, P)], rows, mode=U.MODE_FAITHFUL)
+    bad = np.nonzero((got != want).reshape(rows, -1).any(axis=1))[0]
+    assert bad.size == 0, f"{bad.size} rows differ; first a={a[bad[0]]} b={b[bad[0]]}"
+
+
+def test_division_extended_exact(U, O, gpu):
+    rng = np.random.default_rng(7)
+    rows = 20_000
+    P = 38
+    a = [v * (1 if rng.integers(0, 2) else -1) for v in _classes(rng, rows)]
+    b = [v * (1 if rng.integers(0, 2) else -1) for v in _classes(rng, rows)]
+    cols = [("a", P, 3), ("b", P, 1)]
+    e, got, want, _ = run_calc(U, O, "a/b", cols, [encode(a, P), encode(b, P)], rows, mode=U.MODE_EXTENDED)
+    assert not e.faithful
+    assert np.array_equal(got, want)
+
+
+def test_sign_and_zero_rules(U, O, gpu):
+    """-0 from multiplication, +0 on cancellation, 0/x = +0, x/0 keeps |x|"""
+    P, S = 15, 2
+    a = [0, 5, -5, 7, 0, 123, -123, 0, 100]
+    b = [-3, -5, 5, 0, -0, 7, -7, 9, 100]
+    rows = len(a)
+    ea = encode(a, P)
+    eb = encode(b, P)
+    eb[(4 + 1) * U.compact_bytes(P) - 1] |= 0x80  # a negative zero in b
+    cols = [("a", P, S), ("b", P, S)]
+    for text in ["a*b", "a+b", "a-b", "a/b", "-a", "(a+b)*a", "a*b+a"]:
+        e, got, want, _ = run_calc(U, O, text, cols, [ea, eb], rows)
+        assert np.array_equal(got, want), text

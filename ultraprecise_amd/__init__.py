@@ -49,6 +49,7 @@ def _load():
         "updec_version": (ctypes.c_int, []),
         "updec_init": (ctypes.c_int, [ctypes.c_int]),
         "updec_compact_bytes": (ctypes.c_int, [ctypes.c_int]),
+        "updec_copy_kernel": (ctypes.c_int, [P, P, SZ, P]),
         "updec_malloc": (ctypes.c_int, [ctypes.POINTER(P), SZ]),
         "updec_free": (ctypes.c_int, [P]),
         "updec_memcpy_htod": (ctypes.c_int, [P, P, SZ, P]),
@@ -103,7 +104,7 @@ EXPORTED = ["updec_last_error", "updec_version", "updec_init", "updec_compact_by
             "updec_malloc", "updec_free", "updec_memcpy_htod", "updec_memcpy_dtoh", "updec_memset",
             "updec_stream_create", "updec_stream_destroy", "updec_stream_sync", "updec_device_sync",
             "updec_device_info", "updec_event_create", "updec_event_record", "updec_event_elapsed",
-            "updec_event_destroy",
+            "updec_event_destroy", "updec_copy_kernel",
             "updec_expr_compile", "updec_expr_info", "updec_expr_plan", "updec_expr_source", "updec_expr_calc",
             "updec_expr_free", "updec_agg_compile", "updec_agg_run", "updec_agg_last_kernel_ms", "updec_agg_source",
             "updec_agg_free", "updec_div_int64", "updec_segsum_compact", "updec_compile", "updec_launch",
@@ -213,6 +214,10 @@ class Event:
             self.handle = None
 
 
+def copy_kernel(dst, src, nbytes, stream=None):
+    _check(_lib.updec_copy_kernel(_ptr(dst), _ptr(src), nbytes, _stream(stream)))
+
+
 def device_sync():
     _check(_lib.updec_device_sync())
 
@@ -319,8 +324,12 @@ class Agg:
             self._h = None
 
 
-def div_int64(sums, stride, counts, out, n, mode=MODE_AUTO, stream=None):
-    _check(_lib.updec_div_int64(_ptr(sums), stride, _ptr(counts), _ptr(out), n, mode, _stream(stream)))
+def div_int64(sums, stride, counts, out, n, mode=MODE_AUTO, stream=None, sums_offset=0, out_offset=0):
+    """AVG images out[i] = sums[i] / counts[i]; offsets are byte offsets
+    into the two buffers (e.g. one expression column of a [group][expr]
+    image array)."""
+    _check(_lib.updec_div_int64(ctypes.c_void_p(sums.ptr + sums_offset), stride, _ptr(counts),
+                                ctypes.c_void_p(out.ptr + out_offset), n, mode, _stream(stream)))
 
 
 def segsum_compact(col, precision, scale, rows, associated, groups, ngroups, out, mode=MODE_AUTO, stream=None):

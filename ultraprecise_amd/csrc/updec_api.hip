@@ -259,6 +259,13 @@ __global__ void __launch_bounds__(256) k_gen_byte(uint64_t key, ByteChoices ch, 
     if (i < rows) out[i] = updec_gen::gen_byte(key, (uint64_t)(row0 + i), ch.c, ch.n);
 }
 
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_copy16(v4u* __restrict__ dst, const v4u* __restrict__ src, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
 // largest magnitude bit length of a compact column (sign bit excluded)
 __global__ void __launch_bounds__(256) k_maxbits(const u8* col, int nbytes, int64_t rows, u32* out)
 {
@@ -477,6 +484,19 @@ int updec_event_elapsed(void* ev0, void* ev1, float* ms)
 int updec_event_destroy(void* ev)
 {
     if (ev) HIPCHK(hipEventDestroy(reinterpret_cast<hipEvent_t>(ev)));
+    return UPDEC_OK;
+}
+
+int updec_copy_kernel(void* d_dst, const void* d_src, size_t bytes, updec_stream_t stream)
+{
+    if ((bytes & 15u) || (reinterpret_cast<uintptr_t>(d_dst) & 15u) || (reinterpret_cast<uintptr_t>(d_src) & 15u))
+        return fail(UPDEC_EINVAL, "copy needs 16-byte aligned pointers and size");
+    size_t n = bytes / 16;
+    if (!n) return UPDEC_OK;
+    unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)device_cus() * 16);
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, S(stream), reinterpret_cast<v4u*>(d_dst),
+                       reinterpret_cast<const v4u*>(d_src), n);
+    HIPCHK(hipGetLastError());
     return UPDEC_OK;
 }
 
