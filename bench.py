@@ -201,6 +201,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=2_000_000, help="rows of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="headline only")
+    ap.add_argument("--extra", default="copy,plain,sweep", help="comma list of extra workloads")
     ap.add_argument("--sweep-rows", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -216,11 +217,14 @@ def main():
     h = headline(U, W, args.steps, args.warmup, args.cpu_rows, threads)
     log("headline:", h)
     extra = {}
-    if not args.no_extra:
+    want = set() if args.no_extra else set(args.extra.split(","))
+    if "copy" in want:
         extra["copy"] = copy_bandwidth(U)
         log("copy:", extra["copy"])
+    if "plain" in want:
         extra["plain_1M"] = plain_method(U, W, args.steps, args.warmup)
         log("plain:", extra["plain_1M"])
+    if "sweep" in want:
         extra["sweep_10M"] = precision_sweep(U, W, max(3, args.steps // 4), args.sweep_rows or None)
         log("sweep:", extra["sweep_10M"])
     value = h["rows"] / h["ms_per_step"] / 1e3  # Mrows/s

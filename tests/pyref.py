@@ -1,0 +1,123 @@
+"""Independent Python-integer restatement of the reference's host Decimal
+rules, written for cross-checking the C oracle (tests only).
+
+A value is (sign, magnitude, scale).  Rules (decimal.cu):
+  add/sub (1284-1323, 1575-1613): align to the larger scale; same sign ->
+      magnitudes add, sign kept; different signs -> larger minus smaller with
+      the sign of the larger, +0 on a tie.  a - b == a + (-b).
+  mul (1325-1351): sign xor, scale sum, magnitude product.
+  div (2201-2333): scale Sa + 4; 0/x -> +0; x/0 -> |x| kept (error flag);
+      q = round-half-up(|a| * 10^(Sb+4) / |b|); faithful mode keeps the
+      uint64, uint32 and single-bit-short path behaviours.
+Faithful mode reduces magnitudes mod 2^1024 after every operation.
+"""
+MOD1024 = 1 << 1024
+
+
+def wrap(m, faithful):
+    return m % MOD1024 if faithful else m
+
+
+def parse(text):
+    """StringToDecimal (decimal.cu:2980-3025)"""
+    sign = 0
+    if text.startswith("-"):
+        sign, text = 1, text[1:]
+    if "." in text:
+        i, f = text.split(".")
+    else:
+        i, f = text, ""
+    return sign, int(i + f or "0"), len(f)
+
+
+def cast(v, S):
+    """cast to scale S: multiply up, truncate down (decimal.cu:471-533)"""
+    sign, m, s = v
+    if S >= s:
+        return sign, m * 10 ** (S - s), S
+    return sign, m // 10 ** (s - S), S
+
+
+def decimal(P, S, text):
+    return cast(parse(text), S)
+
+
+def neg(v):
+    return (v[0] ^ 1, v[1], v[2])
+
+
+def add(a, b, faithful=True):
+    sa, ma, xa = a
+    sb, mb, xb = b
+    S = max(xa, xb)
+    ma = wrap(ma * 10 ** (S - xa), faithful)
+    mb = wrap(mb * 10 ** (S - xb), faithful)
+    if sa == sb:
+        return sa, wrap(ma + mb, faithful), S
+    if ma > mb:
+        return sa, ma - mb, S
+    if ma < mb:
+        return sb, mb - ma, S
+    return 0, 0, S
+
+
+def sub(a, b, faithful=True):
+    return add(a, neg(b), faithful)
+
+
+def mul(a, b, faithful=True):
+    return a[0] ^ b[0], wrap(a[1] * b[1], faithful), a[2] + b[2]
+
+
+def _est_shift_bits(s):
+    return (s // 3) * 10 + {0: 0, 1: 4, 2: 7}[s % 3]
+
+
+def div(a, b, faithful=True):
+    """returns (sign, magnitude, scale, error)"""
+    sa, ma, xa = a
+    sb, mb, xb = b
+    S = xa + 4
+    if ma == 0:
+        return 0, 0, S, 0
+    if mb == 0:
+        return sa ^ sb, ma, S, 2
+    s = xb + 4
+    sign = sa ^ sb
+    if faithful and ma.bit_length() + _est_shift_bits(s) - 1 <= 64 and mb.bit_length() <= 64:
+        M64 = (1 << 64) - 1
+        dvs = (ma & M64) * 10 ** s & M64
+        q, r = divmod(dvs, mb)
+        q = (q + (1 if ((r << 1) & M64) >= mb else 0)) & M64
+        return sign, q, S, 0
+    X = ma * 10 ** s
+    q, r = divmod(X, mb)
+    if faithful and mb < (1 << 32):
+        up = ((2 * r) & 0xFFFFFFFF) >= mb
+    elif faithful and X.bit_length() == mb.bit_length() - 1:
+        q, up = 1, False
+    else:
+        up = 2 * r >= mb
+    return sign, wrap(q + (1 if up else 0), faithful), S, 0
+
+
+def to_text(v):
+    sign, m, s = v[0], v[1], v[2]
+    t = str(m).rjust(s + 1, "0")
+    if s:
+        t = t[:-s] + "." + t[-s:]
+    return ("-" if sign else "") + t
+
+
+def compact_encode(v, nbytes):
+    sign, m = v[0], v[1]
+    b = bytearray((m % (1 << (8 * nbytes))).to_bytes(nbytes, "little"))
+    b[-1] |= sign << 7
+    return bytes(b)
+
+
+def compact_decode(buf, S):
+    b = bytearray(buf)
+    sign = b[-1] >> 7
+    b[-1] &= 0x7F
+    return sign, int.from_bytes(bytes(b), "little"), S

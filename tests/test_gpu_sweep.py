@@ -83,6 +83,28 @@ def test_division_paths_bit_exact(U, O, gpu, Sa, Sb):
     assert bad.size == 0, f"{bad.size} rows differ; first a={a[bad[0]]} b={b[bad[0]]}"
 
 
+QUIRK_CASES = [
+    # (a, Sa, b, Sb): path (i) uint64 wrap, path (i) plain, path (ii) uint32
+    # round-compare wrap, path (iii) single-bit-short -> 1, path (iii) plain
+    (2_200_000_000_000_000, 0, 3, 0),
+    (12345, 2, -7, 0),
+    ((1 << 70) + 5, 0, 3_000_000_001, 0),
+    (1 << 86, 0, (1 << 101) - (1 << 97), 0),
+    (-((1 << 100) + 12345), 3, (1 << 70) + 999, 1),
+]
+
+
+@pytest.mark.parametrize("a,Sa,b,Sb", QUIRK_CASES)
+def test_division_quirk_cases(U, O, gpu, a, Sa, b, Sb):
+    import pyref
+    P = 38
+    cols = [("a", P, Sa), ("b", P, Sb)]
+    e, got, want, _ = run_calc(U, O, "a/b", cols, [encode([a], P), encode([b], P)], 1, mode=U.MODE_FAITHFUL)
+    assert np.array_equal(got, want)
+    r = pyref.div((1 if a < 0 else 0, abs(a), Sa), (1 if b < 0 else 0, abs(b), Sb), True)
+    assert bytes(got) == pyref.compact_encode(r[:3], e.out_bytes)
+
+
 def test_division_extended_exact(U, O, gpu):
     rng = np.random.default_rng(7)
     rows = 20_000

@@ -1,0 +1,126 @@
+"""CPU: the rewrite (AlignOptimize), result typing, classification, code
+generation and the C ABI surface -- no device needed."""
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+C15 = [("q", 15, 2), ("p", 15, 2), ("d", 15, 2)]
+
+
+def plan(U, text, cols, mode=0):
+    return U.Expr(text, cols, mode=mode).plan
+
+
+# Known answers derived by hand from the reference's rewrite rules
+# (AriesEngineShell.cpp:90-685) and type rules (decimal.cu:1353-2000).
+def test_plain_method_plans(U):
+    # 1 - d: the integer folds into Decimal(1, scale 2) = 1.00 (:420-436), then
+    # SUB node (16,2) (ListToBinTree :104-116); MUL (31,4) (:117-123); B(31) = 14
+    assert plan(U, "p*(1-d)", C15) == "F 31 4 14 (MUL 31 4 c1:15:2 (SUB 16 2 k+64:2 c2:15:2))"
+    assert plan(U, "q*p", C15) == "F 30 4 13 (MUL 30 4 c0:15:2 c1:15:2)"
+
+
+def test_constant_folding_and_alignment_examples(U):
+    # query.md "4*(col1+col2)*0.25": the constants fold to 1.00 and are dropped
+    assert plan(U, "4*(q+p)*0.25", C15) == "F 16 2 7 (ADD 16 2 c0:15:2 c1:15:2)"
+    # "1+col1+2-3": the integers cancel and the column replaces the node (:402-406)
+    assert plan(U, "1+q+2-3", C15) == "F 15 2 7 c0:15:2"
+    # "1+col1+2+11" with col1 DECIMAL(17,10): 14 folds into Decimal(14, 10)
+    # placed first; ADD type max(0-10, 17-10)+1+10 = 18
+    k = format(14 * 10 ** 10, "x")
+    assert plan(U, "1+c+2+11", [("c", 17, 10)]) == f"F 18 10 8 (ADD 18 10 k+{k}:10 c0:17:10)"
+    # alignment scheduling (query.md "col1+col2+col1"): operands sorted by
+    # scale, so the two DECIMAL(2,1) operands are added before the (17,11) one
+    assert plan(U, "a+b+a", [("a", 2, 1), ("b", 17, 11)]) == \
+        "F 18 11 8 (ADD 18 11 (ADD 3 1 c0:2:1 c0:2:1) c1:17:11)"
+    # decimal constants fold with the integer into one constant: 1.5 + 2 = 3.5
+    # (scale 1), rescaled to 3.50 and inserted before the first operand of
+    # larger scale (:437-470)
+    assert plan(U, "1.5+q+2", C15) == "F 16 2 7 (ADD 16 2 k+15e:2 c0:15:2)"
+    # a multiplication by constants that fold to 1.00 keeps only the column
+    assert plan(U, "q*0.5*2", C15) == "F 15 2 7 c0:15:2"
+
+
+def test_sweep_plans_and_modes(U):
+    cols = [(n, 18, 4) for n in "abcd"]
+    assert plan(U, "(a+b)*c - a/d", cols) == \
+        "F 38 8 16 (ADD 38 8 (MUL 37 8 (ADD 19 4 c0:18:4 c1:18:4) c2:18:4) (DIV 9 8 (NEG c0:18:4) c3:18:4))"
+    # at (38,10) the division (scale 14) sorts before the product (scale 20)
+    cols = [(n, 38, 10) for n in "abcd"]
+    assert plan(U, "(a+b)*c - a/d", cols).startswith("F 78 20 33 (ADD 78 20 (DIV 15 14 (NEG c0:38:10)")
+    for P, S, faithful in [(76, 20, True), (150, 50, False), (300, 100, False)]:
+        e = U.Expr("(a+b)*c - a/d", [(n, P, S) for n in "abcd"])
+        assert e.faithful is faithful
+    # faithful mode caps the root precision at 288 (AriesEngineShell.cpp:638-640)
+    e = U.Expr("a*b", [("a", 200, 0), ("b", 200, 0)], mode=U.MODE_AUTO)
+    assert e.precision == (288 if e.faithful else 400)
+    with pytest.raises(U.UpdecError):
+        U.Expr("(a+b)*c - a/d", [(n, 150, 50) for n in "abcd"], mode=U.MODE_FAITHFUL)
+
+
+def test_parse_errors(U):
+    with pytest.raises(U.UpdecError, match="unknown column"):
+        U.Expr("x+1", C15)
+    with pytest.raises(U.UpdecError):
+        U.Expr("q+", C15)
+    with pytest.raises(U.UpdecError, match="MOD"):
+        U.Expr("q%p", C15)
+
+
+def test_rewritten_plan_parses_in_oracle(U, O):
+    for text in ["p*(1-d)", "q*p", "(q+p)*(d-q)/p", "-q*p+d", "q-(p-d)"]:
+        pl = O.Plan(U.Expr(text, C15).plan)
+        assert pl.P > 0
+
+
+@pytest.mark.parametrize("P,S", [(18, 4), (38, 10), (76, 20), (150, 50), (300, 100)])
+def test_sweep_kernel_compiles(U, P, S):
+    e = U.Expr("(a+b)*c - a/d", [(n, P, S) for n in "abcd"])
+    U.check_compile(e.source)
+
+
+def test_agg_kernels_compile(U):
+    from ultraprecise_amd import workloads as W
+    cols = [(c[0], c[1], c[2], b) for c, b in zip(W.CONFIG3["cols"], [13, 24, 4, 4])]
+    a = U.Agg(W.CONFIG3["exprs"], cols, filter=W.CONFIG3["filter"], nkeys=2)
+    src = a.source(6)
+    assert "LDS 64-bit atomic slots" in src
+    U.check_compile(src)
+    # without statistics the accumulators are wider and use lane-private slots
+    cols = [(c[0], c[1], c[2]) for c in W.CONFIG3["cols"]]
+    src = U.Agg(W.CONFIG3["exprs"][:2], cols, nkeys=1).source(3)
+    U.check_compile(src)
+    src = U.Agg(W.CONFIG1["exprs"], C15).source(1)
+    assert "registers" in src
+    U.check_compile(src)
+
+
+def test_c_abi_exports_every_declared_symbol(U):
+    hdr = open(os.path.join(ROOT, "include", "updec.h")).read()
+    declared = set(re.findall(r"\b(updec_[a-z0-9_]+)\s*\(", hdr))
+    lib = U.lib()
+    missing = [n for n in sorted(declared) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(U.EXPORTED) <= declared
+
+
+def test_generator_is_counter_based(U):
+    from ultraprecise_amd import workloads as W
+    d = W.gen_desc(U, 0, 1, W.CONFIG1["cols"][1])
+    full = U.gen_compact_host(d, 1000)
+    part = U.gen_compact_host(d, 100, row0=400)
+    assert (full[400 * 7:500 * 7] == part).all()
+    # price column within its declared range
+    import numpy as np
+    m = full.reshape(-1, 7).astype(np.uint64)
+    vals = sum(m[:, i] << np.uint64(8 * i) for i in range(7))
+    assert vals.min() >= 90000 and vals.max() <= 10495000
+    dg = W.digits_desc(U, 1, 0, 38)
+    x = U.gen_compact_host(dg, 50)
+    for i in range(50):
+        b = bytearray(x[i * 16:(i + 1) * 16])
+        b[-1] &= 0x7F
+        assert len(str(int.from_bytes(bytes(b), "little"))) == 38

@@ -1043,6 +1043,13 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
     return E.os.str();
 }
 
+int agg_acc_words(const AggSpec& s)
+{
+    int vbits = 1;
+    for (const Plan& p : s.plans) vbits = std::max(vbits, std::min(p.n[p.root].bits, 8 * p.outB - 1));
+    return words_for(vbits + 40 + 1);
+}
+
 std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& cols, const std::string& name,
                             const GenOptions& o, int* acc_words)
 {
@@ -1052,25 +1059,29 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
     const int NG = s.ngroups;
     // value bound after the compact round trip of the materialized result
     int vbits = 1;
-    std::vector<int> pb(NP);
     for (int i = 0; i < NP; i++) {
         const Plan& p = s.plans[i];
-        pb[i] = std::min(p.n[p.root].bits, 8 * p.outB - 1);
-        vbits = std::max(vbits, pb[i]);
+        vbits = std::max(vbits, std::min(p.n[p.root].bits, 8 * p.outB - 1));
     }
-    // per-lane accumulators hold at most 2^16 rows (the launch guarantees
-    // it); block and grid partials hold up to 2^40 rows
-    const int AW1 = words_for(vbits + 16 + 1);
+    // The launch keeps every lane at <= 2^14 rows (kAggRowsPerLaneLog2), so a
+    // lane accumulator needs vbits + 14 + 1 bits and a slot shared by 64/SPLIT
+    // lanes needs vbits + 20 + 1; block and grid partials hold up to 2^40 rows.
+    const int SPLIT = std::max(1, std::min(64, o.slot_split));
+    const int lane_log = kAggRowsPerLaneLog2;
+    const int share_log = lane_log + (SPLIT >= 64 ? 0 : 6 - __builtin_ctz((unsigned)SPLIT));
+    const int AW1 = words_for(vbits + lane_log + 1);
     const int AW2 = words_for(vbits + 40 + 1);
     *acc_words = AW2;
-    const bool lds_slots = NG > 1;
+    enum { REG, ATOMIC, PRIVATE } strat = NG == 1 ? REG : (vbits + share_log + 1 <= 64 ? ATOMIC : PRIVATE);
     const int PW = NP * NG * AW2 + 2 * NG;  // partial words per block
     const char* fops[] = {"<=", "<", ">=", ">", "==", "!="};
 
     E.os << "#include \"dec_device.h\"\nusing namespace updec;\n" << kArgsStruct;
     for (int i = 0; i < NP; i++) E.os << "// plan " << i << ": " << s.plans[i].text() << "\n";
+    E.os << "// accumulation: " << (strat == REG ? "registers" : strat == ATOMIC ? "LDS 64-bit atomic slots" : "LDS lane-private slots")
+         << "\n";
     E.os << "#define NP " << NP << "\n#define NG " << NG << "\n#define AW1 " << AW1 << "\n#define AW2 " << AW2 << "\n#define T "
-         << T << "\n#define PW " << PW << "\n";
+         << T << "\n#define PW " << PW << "\n#define SPLIT " << SPLIT << "\n#define NWAVE (T / 64)\n";
     // round trip through the materialized compact column (the reference sums
     // the stored compact result, AriesExprCalcNode.cpp:1623-1700)
     E.os << "template <int B, int W> UPD_DEV Dec<W> stored(const Dec<W>& v) {\n"
@@ -1087,13 +1098,26 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
             "  }\n"
             "  return r;\n"
             "}\n";
+    // signed 64-bit view of a value known to fit (used by the atomic slots)
+    E.os << "template <int W> UPD_DEV i64 as_i64(const Dec<W>& v) {\n"
+            "  u64 m = v.m.w[0];\n"
+            "  if constexpr (W > 1) m |= (u64)v.m.w[1] << 32;\n"
+            "  return v.neg ? -(i64)m : (i64)m;\n"
+            "}\n";
     E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name << "_main(AggArgs a)\n{\n";
-    E.os << "  const u32 tid = threadIdx.x;\n";
-    if (lds_slots) {
+    E.os << "  const u32 tid = threadIdx.x;\n  const u32 wave = tid >> 6, lane = tid & 63u;\n";
+    if (strat == PRIVATE) {
         E.os << "  __shared__ u32 slot[NG * NP * AW1 * T];\n  __shared__ u32 cslot[NG * T];\n";
         E.os << "  for (u32 i = tid; i < NG * NP * AW1 * T; i += T) slot[i] = 0u;\n";
         E.os << "  for (u32 i = tid; i < NG * T; i += T) cslot[i] = 0u;\n";
         E.os << "  __syncthreads();\n";
+    } else if (strat == ATOMIC) {
+        E.os << "  // one slot set per (wave, lane group): [wave][copy][group][plan] + counts\n";
+        E.os << "  __shared__ u64 aslot[NWAVE * SPLIT * NG * NP];\n  __shared__ u32 acnt[NWAVE * SPLIT * NG];\n";
+        E.os << "  for (u32 i = tid; i < NWAVE * SPLIT * NG * NP; i += T) aslot[i] = 0ull;\n";
+        E.os << "  for (u32 i = tid; i < NWAVE * SPLIT * NG; i += T) acnt[i] = 0u;\n";
+        E.os << "  __syncthreads();\n";
+        E.os << "  const u32 copy = wave * SPLIT + lane / (64 / SPLIT);\n";
     } else {
         for (int i = 0; i < NP; i++) E.os << "  Mag<AW1> acc" << i << " = mzero<AW1>();\n";
         E.os << "  u32 cnt = 0;\n";
@@ -1103,59 +1127,77 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
     if (s.nkeys == 2) E.os << "    const u32 g = a.lut[((u32)a.key0[row] << 8) | a.key1[row]];\n";
     else if (s.nkeys == 1) E.os << "    const u32 g = a.lut[a.key0[row]];\n";
     else E.os << "    const u32 g = 0;\n";
+    if (strat == ATOMIC) E.os << "    const u32 sbase = (copy * NG + g) * NP;\n";
     std::vector<std::string> loaded(cols.size());
     for (int i = 0; i < NP; i++) {
         const Plan& p = s.plans[i];
+        const int W = p.n[p.root].W;
         std::string rv = emit_plan(E, p, cols, loaded, "    ", "p" + std::to_string(i) + "_");
-        E.os << "    {\n      const Dec<" << p.n[p.root].W << "> v = stored<" << p.outB << ", " << p.n[p.root].W << ">(" << rv
-             << ");\n";
-        if (lds_slots) {
+        E.os << "    {\n      const Dec<" << W << "> v = stored<" << p.outB << ", " << W << ">(" << rv << ");\n";
+        if (strat == PRIVATE) {
             E.os << "      u32* sp = slot + ((g * NP + " << i << ") * AW1) * T + tid;\n";
             E.os << "      Mag<AW1> s;\n#pragma unroll\n      for (int w = 0; w < AW1; w++) s.w[w] = sp[w * T];\n";
-            E.os << "      acc_add<AW1, " << p.n[p.root].W << ">(s, v);\n";
+            E.os << "      acc_add<AW1, " << W << ">(s, v);\n";
             E.os << "#pragma unroll\n      for (int w = 0; w < AW1; w++) sp[w * T] = s.w[w];\n";
+        } else if (strat == ATOMIC) {
+            E.os << "      atomicAdd(&aslot[sbase + " << i << "], (u64)as_i64<" << W << ">(v));\n";
         } else {
-            E.os << "      acc_add<AW1, " << p.n[p.root].W << ">(acc" << i << ", v);\n";
+            E.os << "      acc_add<AW1, " << W << ">(acc" << i << ", v);\n";
         }
         E.os << "    }\n";
     }
-    if (lds_slots) E.os << "    cslot[g * T + tid] += 1u;\n";
+    if (strat == PRIVATE) E.os << "    cslot[g * T + tid] += 1u;\n";
+    else if (strat == ATOMIC) E.os << "    atomicAdd(&acnt[copy * NG + g], 1u);\n";
     else E.os << "    cnt += 1u;\n";
     E.os << "  }\n";
-    // block reduction: each wave reduces, wave leaders combine through LDS
-    E.os << "  __shared__ u32 red[(T / 64) * (NP * NG * AW2 + 2 * NG)];\n";
-    E.os << "  const u32 wave = tid >> 6, lane = tid & 63u;\n";
-    if (lds_slots) E.os << "  __syncthreads();\n";
-    E.os << "#pragma unroll\n  for (int gg = 0; gg < NG; gg++) {\n";
-    E.os << "#pragma unroll\n    for (int pp = 0; pp < NP; pp++) {\n";
-    E.os << "      Mag<AW1> s1;\n";
-    if (lds_slots)
-        E.os << "#pragma unroll\n      for (int w = 0; w < AW1; w++) s1.w[w] = slot[((gg * NP + pp) * AW1 + w) * T + tid];\n";
-    else {
-        E.os << "      s1 = mzero<AW1>();\n";
-        for (int i = 0; i < NP; i++) E.os << "      if (pp == " << i << ") s1 = acc" << i << ";\n";
+    E.os << "  __shared__ u32 red[NWAVE * PW];\n";
+    if (strat == ATOMIC) {
+        // combine the slot copies of the block: one thread per (group, plan)
+        E.os << "  __syncthreads();\n";
+        E.os << "  for (u32 k = tid; k < NP * NG + NG; k += T) {\n";
+        E.os << "    u32* dst = a.partials + (usize)blockIdx.x * PW;\n";
+        E.os << "    if (k < NP * NG) {\n      const u32 gg = k / NP, pp = k % NP;\n      Mag<AW2> s = mzero<AW2>();\n";
+        E.os << "      for (u32 c = 0; c < NWAVE * SPLIT; c++) {\n        const u64 x = aslot[(c * NG + gg) * NP + pp];\n";
+        E.os << "        Mag<AW2> o;\n        const u32 ext = (x >> 63) ? 0xffffffffu : 0u;\n";
+        E.os << "#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = w == 0 ? (u32)x : (w == 1 ? (u32)(x >> 32) : ext);\n";
+        E.os << "        add_to<AW2>(s, s, o);\n      }\n";
+        E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) dst[k * AW2 + w] = s.w[w];\n";
+        E.os << "    } else {\n      const u32 gg = k - NP * NG;\n      u64 c = 0;\n";
+        E.os << "      for (u32 cc = 0; cc < NWAVE * SPLIT; cc++) c += acnt[cc * NG + gg];\n";
+        E.os << "      dst[NP * NG * AW2 + 2 * gg] = (u32)c; dst[NP * NG * AW2 + 2 * gg + 1] = (u32)(c >> 32);\n    }\n  }\n}\n";
+    } else {
+        // block reduction: each wave reduces, wave leaders combine through LDS
+        if (strat == PRIVATE) E.os << "  __syncthreads();\n";
+        E.os << "#pragma unroll\n  for (int gg = 0; gg < NG; gg++) {\n";
+        E.os << "#pragma unroll\n    for (int pp = 0; pp < NP; pp++) {\n";
+        E.os << "      Mag<AW1> s1;\n";
+        if (strat == PRIVATE)
+            E.os << "#pragma unroll\n      for (int w = 0; w < AW1; w++) s1.w[w] = slot[((gg * NP + pp) * AW1 + w) * T + tid];\n";
+        else {
+            E.os << "      s1 = mzero<AW1>();\n";
+            for (int i = 0; i < NP; i++) E.os << "      if (pp == " << i << ") s1 = acc" << i << ";\n";
+        }
+        E.os << "      Mag<AW2> s2;\n      const u32 ext = (s1.w[AW1 - 1] >> 31) ? 0xffffffffu : 0u;\n";
+        E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) s2.w[w] = w < AW1 ? s1.w[w] : ext;\n";
+        E.os << "      wave_reduce<AW2>(s2);\n";
+        E.os << "      if (lane == 0) {\n#pragma unroll\n        for (int w = 0; w < AW2; w++) red[wave * PW + (gg * NP + pp) * AW2 + w] = s2.w[w];\n      }\n";
+        E.os << "    }\n";
+        if (strat == PRIVATE) E.os << "    u64 c = cslot[gg * T + tid];\n";
+        else E.os << "    u64 c = cnt;\n";
+        E.os << "#pragma unroll\n    for (int off = 32; off >= 1; off >>= 1) c += (u64)__shfl_xor((long long)c, off, 64);\n";
+        E.os << "    if (lane == 0) { red[wave * PW + NP * NG * AW2 + 2 * gg] = (u32)c; red[wave * PW + NP * NG * AW2 + 2 * gg + 1] = (u32)(c >> 32); }\n";
+        E.os << "  }\n  __syncthreads();\n";
+        E.os << "  for (u32 k = tid; k < NP * NG + NG; k += T) {\n";
+        E.os << "    u32* dst = a.partials + (usize)blockIdx.x * PW;\n";
+        E.os << "    if (k < NP * NG) {\n      Mag<AW2> s = mzero<AW2>();\n";
+        E.os << "      for (int wv = 0; wv < NWAVE; wv++) {\n        Mag<AW2> o;\n#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = red[wv * PW + k * AW2 + w];\n        add_to<AW2>(s, s, o);\n      }\n";
+        E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) dst[k * AW2 + w] = s.w[w];\n";
+        E.os << "    } else {\n      const u32 gg = k - NP * NG;\n      u64 c = 0;\n";
+        E.os << "      for (int wv = 0; wv < NWAVE; wv++) c += (u64)red[wv * PW + NP * NG * AW2 + 2 * gg] | ((u64)red[wv * PW + NP * NG * AW2 + 2 * gg + 1] << 32);\n";
+        E.os << "      dst[NP * NG * AW2 + 2 * gg] = (u32)c; dst[NP * NG * AW2 + 2 * gg + 1] = (u32)(c >> 32);\n    }\n  }\n}\n";
     }
-    E.os << "      Mag<AW2> s2;\n      const u32 ext = (s1.w[AW1 - 1] >> 31) ? 0xffffffffu : 0u;\n";
-    E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) s2.w[w] = w < AW1 ? s1.w[w] : ext;\n";
-    E.os << "      wave_reduce<AW2>(s2);\n";
-    E.os << "      if (lane == 0) {\n#pragma unroll\n        for (int w = 0; w < AW2; w++) red[wave * PW + (gg * NP + pp) * AW2 + w] = s2.w[w];\n      }\n";
-    E.os << "    }\n";
-    if (lds_slots) E.os << "    u64 c = cslot[gg * T + tid];\n";
-    else E.os << "    u64 c = cnt;\n";
-    E.os << "#pragma unroll\n    for (int off = 32; off >= 1; off >>= 1) c += (u64)__shfl_xor((long long)c, off, 64);\n";
-    E.os << "    if (lane == 0) { red[wave * PW + NP * NG * AW2 + 2 * gg] = (u32)c; red[wave * PW + NP * NG * AW2 + 2 * gg + 1] = (u32)(c >> 32); }\n";
-    E.os << "  }\n  __syncthreads();\n";
-    E.os << "  // combine waves: one thread per partial value\n";
-    E.os << "  for (u32 k = tid; k < NP * NG + NG; k += T) {\n";
-    E.os << "    u32* dst = a.partials + (usize)blockIdx.x * PW;\n";
-    E.os << "    if (k < NP * NG) {\n      Mag<AW2> s = mzero<AW2>();\n";
-    E.os << "      for (int wv = 0; wv < T / 64; wv++) {\n        Mag<AW2> o;\n#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = red[wv * PW + k * AW2 + w];\n        add_to<AW2>(s, s, o);\n      }\n";
-    E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) dst[k * AW2 + w] = s.w[w];\n";
-    E.os << "    } else {\n      const u32 gg = k - NP * NG;\n      u64 c = 0;\n";
-    E.os << "      for (int wv = 0; wv < T / 64; wv++) c += (u64)red[wv * PW + NP * NG * AW2 + 2 * gg] | ((u64)red[wv * PW + NP * NG * AW2 + 2 * gg + 1] << 32);\n";
-    E.os << "      dst[NP * NG * AW2 + 2 * gg] = (u32)c; dst[NP * NG * AW2 + 2 * gg + 1] = (u32)(c >> 32);\n    }\n  }\n}\n";
 
-    // final: one block per (plan, group) sum and one per group count
+    // final: one block per (group, plan) sum and one per group count
     E.os << "extern \"C\" __global__ void __launch_bounds__(256) " << name
          << "_final(const u32* __restrict__ partials, u32 nblocks, u8* __restrict__ images, u64* __restrict__ counts)\n{\n";
     E.os << "  const u32 k = blockIdx.x, tid = threadIdx.x;\n";

@@ -109,7 +109,14 @@ bool plan_is_faithful_envelope(const Plan& p, const std::vector<ColumnType>& col
 // ---- code generation ----
 struct GenOptions {
     int block = 256;
+    // grouped aggregation: slot copies per wave for the LDS atomic
+    // accumulators (1 = one set shared by the 64 lanes, 64 = per lane)
+    int slot_split = 4;
 };
+
+// The fused aggregation launch keeps every lane at <= 2^kAggRowsPerLaneLog2
+// rows; the generated accumulator widths rely on it.
+constexpr int kAggRowsPerLaneLog2 = 14;
 
 // calc kernel: one compact output row per input row (the reference's
 // calc_expr_<id>, AriesExprCalcNode.cpp:1285-1318)
@@ -130,5 +137,7 @@ struct AggSpec {
 };
 std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& cols, const std::string& name,
                             const GenOptions& o, int* acc_words);
+// limbs of one block partial of the fused aggregation
+int agg_acc_words(const AggSpec& s);
 
 }  // namespace updec

@@ -300,17 +300,24 @@ __device__ __forceinline__ bool filter_pass(i32 v, int op, i64 c)
     }
 }
 
-// group discovery: presence bitmap of (key0 << 8 | key1) over filtered rows
+// group discovery: presence bitmap of (key0 << 8 | key1) over filtered rows.
+// Each block builds its bitmap in LDS and merges the nonzero words into the
+// global bitmap once, so global atomics are per block, not per row.
 __global__ void __launch_bounds__(256) k_key_presence(const u8* key0, const u8* key1, int nkeys, const i32* fcol,
                                                       int fop, i64 fval, int64_t rows, u32* bitmap)
 {
+    __shared__ u32 bm[2048];
+    for (int i = threadIdx.x; i < 2048; i += 256) bm[i] = 0u;
+    __syncthreads();
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x) {
         if (fcol && !filter_pass(fcol[i], fop, fval)) continue;
         u32 k = nkeys == 2 ? (((u32)key0[i] << 8) | key1[i]) : key0[i];
         u32 bit = 1u << (k & 31u);
-        // skip the atomic when another lane already set the bit
-        if (!(__builtin_nontemporal_load(&bitmap[k >> 5]) & bit)) atomicOr(&bitmap[k >> 5], bit);
+        if (!(bm[k >> 5] & bit)) atomicOr(&bm[k >> 5], bit);
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2048; i += 256)
+        if (bm[i]) atomicOr(&bitmap[i], bm[i]);
 }
 
 // AVG: Decimal(sum) /= Decimal((int64)count) per group (kernel_util.hxx:427-438)
@@ -692,21 +699,15 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
     const int T = 256;
     const int NP = (int)a->spec.plans.size();
     int aw = 0;
-    {
-        AggSpec s = a->spec;
-        s.ngroups = ngroups;
-        // recompute AW2 as the generator does
-        int vbits = 1;
-        for (auto& p : s.plans) vbits = std::max(vbits, std::min(p.n[p.root].bits, 8 * p.outB - 1));
-        aw = std::max(1, (vbits + 40 + 1 + 31) / 32);
-    }
+    aw = agg_acc_words(a->spec);
     const size_t PW = (size_t)NP * ngroups * aw + 2 * (size_t)ngroups;
     int64_t tiles = std::max<int64_t>(1, (rows + T - 1) / T);
     int blocks_per_cu = 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, fmain, T, 0) != hipSuccess || blocks_per_cu < 1)
         blocks_per_cu = 2;
     int64_t grid = std::min<int64_t>(tiles, (int64_t)device_cus() * blocks_per_cu);
-    int64_t min_grid = (rows + (int64_t)T * 65536 - 1) / ((int64_t)T * 65536);  // <= 2^16 rows per lane
+    const int64_t per_block = (int64_t)T << kAggRowsPerLaneLog2;
+    int64_t min_grid = (rows + per_block - 1) / per_block;  // every lane <= 2^14 rows
     grid = std::max(grid, std::max<int64_t>(min_grid, 1));
     if ((size_t)grid * PW > a->partial_words) {
         if (a->d_partials) HIPCHK(hipFree(a->d_partials));
