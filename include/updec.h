@@ -138,7 +138,8 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
 /* device time of the last run's main aggregation kernel, in milliseconds
  * (HIP events on the run's stream; synchronizes) */
 int updec_agg_last_kernel_ms(updec_agg_t a, float* ms);
-/* generated source for a given group count (no device needed) */
+/* generated source for a given group count of the two-pass form, or of the
+ * one-pass form with ngroups = 0 (no device needed) */
 int updec_agg_source(updec_agg_t a, int32_t ngroups, char* buf, size_t cap, size_t* needed);
 void updec_agg_free(updec_agg_t a);
 

@@ -75,6 +75,43 @@ def test_config3_grouped_bit_exact(U, O, gpu, rows):
             assert bytes(ga2[g]) == want, f"AVG expr {k} group {g}"
 
 
+@pytest.mark.parametrize("nf1,nf2,nkeys", [(40, 3, 2), (3, 2, 2), (7, 1, 1), (60, 1, 1)])
+def test_grouping_many_keys_and_fallback(U, O, gpu, nf1, nf2, nkeys):
+    """more distinct keys than a block's one-pass table holds (16) forces
+    the two-pass form; results must not depend on which form ran"""
+    rows = 200_000
+    seed = 11
+    C = W.CONFIG3
+    spec = C["cols"][:2]
+    dev = [U.gen_compact_device(W.gen_desc(U, seed, i, c), rows) for i, c in enumerate(spec)]
+    host = [t.numpy(count=rows * U.compact_bytes(c[1])) for t, c in zip(dev, spec)]
+    letters1 = bytes(range(1, 1 + nf1))
+    letters2 = bytes(range(60, 60 + nf2))
+    f1 = U.gen_byte_device(seed, 1, letters1, rows)
+    f2 = U.gen_byte_device(seed, 2, letters2, rows)
+    hf1, hf2 = f1.numpy(count=rows), f2.numpy(count=rows)
+    cols = [(c[0], c[1], c[2]) for c in spec]
+    exprs = ["qty", "price*qty"]
+    agg = U.Agg(exprs, cols, nkeys=nkeys)
+    maxg = 255
+    images = U.DeviceBuffer(maxg * len(exprs) * 133).zero()
+    counts = U.DeviceBuffer(maxg * 8).zero()
+    ng, keys = agg.run(dev, rows, images, counts, key0=f1, key1=f2 if nkeys == 2 else None, max_groups=maxg)
+    key = (hf1.astype(np.int32) << 8) | hf2 if nkeys == 2 else hf1.astype(np.int32)
+    hkeys = np.unique(key)
+    assert ng == len(hkeys)
+    lut = np.full(65536, -1, dtype=np.int32)
+    lut[hkeys] = np.arange(len(hkeys), dtype=np.int32)
+    gid = lut[key].astype(np.int32)
+    got = images.numpy().reshape(maxg, len(exprs), 133)
+    gotc = counts.numpy("int64")[:ng]
+    for k, text in enumerate(exprs):
+        e = U.Expr(text, cols)
+        want, wc = O.plan_group_sum(O.Plan(e.plan), host, [c[1] for c in cols], [c[2] for c in cols], rows, gid, ng)
+        assert np.array_equal(gotc, wc)
+        assert np.array_equal(got[:ng, k], want), text
+
+
 def test_div_int64_edges(U, O, gpu):
     """AVG division on crafted sums: zero, negative, path (i) wrap region, wide sums, zero count"""
     rng = np.random.default_rng(5)

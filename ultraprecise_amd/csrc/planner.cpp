@@ -1043,45 +1043,65 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
     return E.os.str();
 }
 
-int agg_acc_words(const AggSpec& s)
+static int agg_value_bits(const AggSpec& s)
 {
     int vbits = 1;
     for (const Plan& p : s.plans) vbits = std::max(vbits, std::min(p.n[p.root].bits, 8 * p.outB - 1));
-    return words_for(vbits + 40 + 1);
+    return vbits;
 }
 
-std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& cols, const std::string& name,
-                            const GenOptions& o, int* acc_words)
-{
-    Emit E;
-    const int T = o.block;
-    const int NP = (int)s.plans.size();
-    const int NG = s.ngroups;
-    // value bound after the compact round trip of the materialized result
-    int vbits = 1;
-    for (int i = 0; i < NP; i++) {
-        const Plan& p = s.plans[i];
-        vbits = std::max(vbits, std::min(p.n[p.root].bits, 8 * p.outB - 1));
-    }
-    // The launch keeps every lane at <= 2^14 rows (kAggRowsPerLaneLog2), so a
-    // lane accumulator needs vbits + 14 + 1 bits and a slot shared by 64/SPLIT
-    // lanes needs vbits + 20 + 1; block and grid partials hold up to 2^40 rows.
-    const int SPLIT = std::max(1, std::min(64, o.slot_split));
-    const int lane_log = kAggRowsPerLaneLog2;
-    const int share_log = lane_log + (SPLIT >= 64 ? 0 : 6 - __builtin_ctz((unsigned)SPLIT));
-    const int AW1 = words_for(vbits + lane_log + 1);
-    const int AW2 = words_for(vbits + 40 + 1);
-    *acc_words = AW2;
-    enum { REG, ATOMIC, PRIVATE } strat = NG == 1 ? REG : (vbits + share_log + 1 <= 64 ? ATOMIC : PRIVATE);
-    const int PW = NP * NG * AW2 + 2 * NG;  // partial words per block
-    const char* fops[] = {"<=", "<", ">=", ">", "==", "!="};
+int agg_acc_words(const AggSpec& s) { return words_for(agg_value_bits(s) + 40 + 1); }
 
+// rows that can reach one slot copy: 2^14 per lane times the lanes sharing it
+static int share_log_for(int split) { return kAggRowsPerLaneLog2 + (split >= 64 ? 0 : 6 - __builtin_ctz((unsigned)split)); }
+
+// Accumulator layout of the grouped kernels: u64 slots [copy][slot][plan][AWR]
+// where copy = (wave, lane group).  AWR = 1 holds the signed 64-bit sum
+// directly; otherwise AWR = AW2 slots hold the 32-bit limbs of the two's
+// complement value separately ("redundant" limbs: each slot collects its
+// limb's column sum, carries are resolved once at the end), so every update
+// is a plain ds_add_u64 with no carry chain.
+struct AccCfg {
+    int T = 256, SPLIT = 4, AWR = 1;
+};
+
+static bool choose_acc(const AggSpec& s, int nslots, const GenOptions& o, AccCfg* c)
+{
+    const int vbits = agg_value_bits(s), AW2 = agg_acc_words(s), NP = (int)s.plans.size();
+    const int cands[][2] = {{o.block, o.slot_split}, {o.block, 2}, {o.block, 1}, {128, 1}, {64, 1}};
+    for (auto& cd : cands) {
+        const int T = cd[0], SP = std::max(1, std::min(64, cd[1]));
+        const int AWR = (vbits + share_log_for(SP) + 1 <= 64) ? 1 : AW2;
+        const long bytes = (long)(T / 64) * SP * nslots * ((long)NP * AWR * 8 + 4);
+        if (bytes <= 60 * 1024) {
+            c->T = T;
+            c->SPLIT = SP;
+            c->AWR = AWR;
+            return true;
+        }
+    }
+    return false;
+}
+
+bool agg_onepass_ok(const AggSpec& s, const GenOptions& o)
+{
+    AccCfg c;
+    return s.nkeys > 0 && choose_acc(s, s.maxslots, o, &c);
+}
+
+int agg_onepass_record_words(const AggSpec& s)
+{
+    return s.maxslots * (1 + (int)s.plans.size() * agg_acc_words(s) + 2);
+}
+
+static void emit_agg_prelude(Emit& E, const AggSpec& s, const AccCfg& c, int nslots)
+{
+    const int NP = (int)s.plans.size();
     E.os << "#include \"dec_device.h\"\nusing namespace updec;\n" << kArgsStruct;
     for (int i = 0; i < NP; i++) E.os << "// plan " << i << ": " << s.plans[i].text() << "\n";
-    E.os << "// accumulation: " << (strat == REG ? "registers" : strat == ATOMIC ? "LDS 64-bit atomic slots" : "LDS lane-private slots")
-         << "\n";
-    E.os << "#define NP " << NP << "\n#define NG " << NG << "\n#define AW1 " << AW1 << "\n#define AW2 " << AW2 << "\n#define T "
-         << T << "\n#define PW " << PW << "\n#define SPLIT " << SPLIT << "\n#define NWAVE (T / 64)\n";
+    E.os << "#define NP " << NP << "\n#define AW2 " << agg_acc_words(s) << "\n#define T " << c.T << "\n#define SPLIT "
+         << c.SPLIT << "\n#define NWAVE (T / 64)\n#define NCOPY (NWAVE * SPLIT)\n#define NS " << nslots
+         << "\n#define AWR " << c.AWR << "\n";
     // round trip through the materialized compact column (the reference sums
     // the stored compact result, AriesExprCalcNode.cpp:1623-1700)
     E.os << "template <int B, int W> UPD_DEV Dec<W> stored(const Dec<W>& v) {\n"
@@ -1098,24 +1118,232 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
             "  }\n"
             "  return r;\n"
             "}\n";
-    // signed 64-bit view of a value known to fit (used by the atomic slots)
-    E.os << "template <int W> UPD_DEV i64 as_i64(const Dec<W>& v) {\n"
-            "  u64 m = v.m.w[0];\n"
-            "  if constexpr (W > 1) m |= (u64)v.m.w[1] << 32;\n"
-            "  return v.neg ? -(i64)m : (i64)m;\n"
+    // add one signed value into a slot of AWR u64 words
+    E.os << "template <int W> UPD_DEV void slot_add(u64* sl, const Dec<W>& v) {\n"
+            "  if constexpr (AWR == 1) {\n"
+            "    u64 m = v.m.w[0];\n"
+            "    if constexpr (W > 1) m |= (u64)v.m.w[1] << 32;\n"
+            "    atomicAdd(sl, v.neg ? (u64)(-(i64)m) : m);\n"
+            "  } else {\n"
+            "    const u32 x = v.neg ? 0xffffffffu : 0u;\n"
+            "#pragma unroll\n"
+            "    for (int i = 0; i < AWR; i++) {\n"
+            "      const u64 add = (u64)((i < W ? v.m.w[i] : 0u) ^ x) + (i == 0 ? (u64)(v.neg & 1u) : 0ull);\n"
+            "      if (add) atomicAdd(sl + i, add);\n"
+            "    }\n"
+            "  }\n"
             "}\n";
-    E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name << "_main(AggArgs a)\n{\n";
+    // slot -> AW2-limb two's complement value
+    E.os << "UPD_DEV Mag<AW2> slot_value(const u64* sl) {\n"
+            "  Mag<AW2> r;\n"
+            "  if constexpr (AWR == 1) {\n"
+            "    const u64 x = sl[0];\n"
+            "    const u32 ext = (x >> 63) ? 0xffffffffu : 0u;\n"
+            "#pragma unroll\n"
+            "    for (int w = 0; w < AW2; w++) r.w[w] = w == 0 ? (u32)x : (w == 1 ? (u32)(x >> 32) : ext);\n"
+            "  } else {\n"
+            "    u64 carry = 0;\n"
+            "#pragma unroll\n"
+            "    for (int w = 0; w < AW2; w++) {\n"
+            "      const u64 lo = (sl[w] & 0xffffffffull) + (carry & 0xffffffffull);\n"
+            "      r.w[w] = (u32)lo;\n"
+            "      carry = (sl[w] >> 32) + (carry >> 32) + (lo >> 32);\n"
+            "    }\n"
+            "  }\n"
+            "  return r;\n"
+            "}\n";
+}
+
+// per-row body shared by both forms: evaluate every plan and add it to the
+// slots at `sbase` (u64 index), then count the row
+static void emit_agg_row(Emit& E, const AggSpec& s, const std::vector<ColumnType>& cols, const std::string& ind)
+{
+    std::vector<std::string> loaded(cols.size());
+    for (int i = 0; i < (int)s.plans.size(); i++) {
+        const Plan& p = s.plans[i];
+        const int W = p.n[p.root].W;
+        std::string rv = emit_plan(E, p, cols, loaded, ind, "p" + std::to_string(i) + "_");
+        E.os << ind << "slot_add<" << W << ">(&aslot[sbase + " << i << " * AWR], stored<" << p.outB << ", " << W << ">(" << rv
+             << "));\n";
+    }
+}
+
+static void emit_agg_final_image(Emit& E, const AggSpec& s, const std::string& target)
+{
+    E.os << "      u32 prec = 0, frac = 0;\n";
+    for (int i = 0; i < (int)s.plans.size(); i++)
+        E.os << "      if (pp == " << i << ") { prec = " << (s.plans[i].outP & 0xff) << "u; frac = " << (s.plans[i].outS & 0xff)
+             << "u; }\n";
+    E.os << "      write_sum_image<AW2>(" << target << ", t, prec, frac, 0u);\n";
+}
+
+static const char* kBlockReduceAW2 =
+    "    wave_reduce<AW2>(sm);\n"
+    "    if ((tid & 63u) == 0) {\n#pragma unroll\n      for (int w = 0; w < AW2; w++) red[(tid >> 6) * AW2 + w] = sm.w[w];\n    }\n"
+    "    __syncthreads();\n"
+    "    if (tid == 0) {\n      Mag<AW2> t = mzero<AW2>();\n      for (int wv = 0; wv < 4; wv++) {\n        Mag<AW2> o;\n"
+    "#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = red[wv * AW2 + w];\n        add_to<AW2>(t, t, o);\n      }\n";
+
+static const char* kBlockReduceCount =
+    "#pragma unroll\n    for (int off = 32; off >= 1; off >>= 1) c += (u64)__shfl_xor((long long)c, off, 64);\n"
+    "    if ((tid & 63u) == 0) { red[2 * (tid >> 6)] = (u32)c; red[2 * (tid >> 6) + 1] = (u32)(c >> 32); }\n"
+    "    __syncthreads();\n";
+
+// One-pass grouped aggregation.  agg_main: each block keeps an LDS hash
+// table from the 16-bit key to a dense slot (assigned on first sight) and
+// 64-bit atomic accumulators per (lane group, slot, plan); at the end it
+// writes one record per slot {key+1, sums, count}.  agg_final: every block
+// rebuilds the ascending key order from all records (bitmap + prefix
+// popcounts), then block (g, p) sums the records of group g.
+static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnType>& cols, const std::string& name,
+                                   const GenOptions& o, int* acc_words, int* block_out)
+{
+    Emit E;
+    AccCfg c;
+    if (!choose_acc(s, s.maxslots, o, &c)) return std::string();
+    const int NP = (int)s.plans.size();
+    const int AW2 = agg_acc_words(s);
+    const int MAXS = s.maxslots;
+    const int EW = 1 + NP * AW2 + 2;
+    *acc_words = AW2;
+    *block_out = c.T;
+    const char* fops[] = {"<=", "<", ">=", ">", "==", "!="};
+    emit_agg_prelude(E, s, c, MAXS);
+    E.os << "// accumulation: one pass, LDS key hash + 64-bit atomic slots\n";
+    E.os << "#define MAXS " << MAXS << "\n#define HT 64\n#define EW " << EW << "\n";
+    // key -> slot: open addressing over HT entries {key+1 << 8 | slot};
+    // slot 0xff marks an entry whose slot number is being assigned.  All
+    // table accesses are volatile so a waiting lane re-reads it.
+    E.os << "UPD_DEV u32 slot_of(u32 key, volatile u32* htab, u32* nslot, u32* skey) {\n"
+            "  u32 h = (key * 2654435761u) >> 26;\n"
+            "  for (int probe = 0; probe < 4 * HT; probe++) {\n"
+            "    u32 e = htab[h];\n"
+            "    if (e == 0u) {\n"
+            "      const u32 old = atomicCAS((u32*)&htab[h], 0u, ((key + 1u) << 8) | 0xffu);\n"
+            "      if (old == 0u) {\n"
+            "        u32 sl = atomicAdd(nslot, 1u);\n"
+            "        if (sl >= MAXS) sl = 0xfeu;\n"
+            "        else skey[sl] = key + 1u;\n"
+            "        htab[h] = ((key + 1u) << 8) | sl;\n"
+            "        return sl;\n"
+            "      }\n"
+            "      e = old;\n"
+            "    }\n"
+            "    if ((e >> 8) == key + 1u) {\n"
+            "      if ((e & 0xffu) != 0xffu) return e & 0xffu;\n"
+            "      continue;\n"
+            "    }\n"
+            "    h = (h + 1u) & (HT - 1);\n"
+            "  }\n"
+            "  return 0xfeu;\n"
+            "}\n";
+    E.os << "extern \"C\" __global__ void __launch_bounds__(" << c.T << ") " << name
+         << "_main(AggArgs a, u32* __restrict__ overflow)\n{\n";
     E.os << "  const u32 tid = threadIdx.x;\n  const u32 wave = tid >> 6, lane = tid & 63u;\n";
-    if (strat == PRIVATE) {
-        E.os << "  __shared__ u32 slot[NG * NP * AW1 * T];\n  __shared__ u32 cslot[NG * T];\n";
-        E.os << "  for (u32 i = tid; i < NG * NP * AW1 * T; i += T) slot[i] = 0u;\n";
-        E.os << "  for (u32 i = tid; i < NG * T; i += T) cslot[i] = 0u;\n";
-        E.os << "  __syncthreads();\n";
-    } else if (strat == ATOMIC) {
-        E.os << "  // one slot set per (wave, lane group): [wave][copy][group][plan] + counts\n";
-        E.os << "  __shared__ u64 aslot[NWAVE * SPLIT * NG * NP];\n  __shared__ u32 acnt[NWAVE * SPLIT * NG];\n";
-        E.os << "  for (u32 i = tid; i < NWAVE * SPLIT * NG * NP; i += T) aslot[i] = 0ull;\n";
-        E.os << "  for (u32 i = tid; i < NWAVE * SPLIT * NG; i += T) acnt[i] = 0u;\n";
+    E.os << "  __shared__ u32 htab[HT];\n  __shared__ u32 nslot;\n  __shared__ u32 skey[MAXS];\n";
+    E.os << "  __shared__ u64 aslot[NCOPY * NS * NP * AWR];\n  __shared__ u32 acnt[NCOPY * NS];\n";
+    E.os << "  for (u32 i = tid; i < HT; i += T) htab[i] = 0u;\n";
+    E.os << "  for (u32 i = tid; i < MAXS; i += T) skey[i] = 0u;\n";
+    E.os << "  if (tid == 0) nslot = 0u;\n";
+    E.os << "  for (u32 i = tid; i < NCOPY * NS * NP * AWR; i += T) aslot[i] = 0ull;\n";
+    E.os << "  for (u32 i = tid; i < NCOPY * NS; i += T) acnt[i] = 0u;\n";
+    E.os << "  __syncthreads();\n";
+    E.os << "  const u32 copy = wave * SPLIT + lane / (64 / SPLIT);\n";
+    E.os << "  u32 ovf = 0;\n";
+    E.os << "  for (usize row = (usize)blockIdx.x * T + tid; row < a.rows; row += (usize)gridDim.x * T) {\n";
+    if (s.filter_col >= 0) E.os << "    if (!(a.fcol[row] " << fops[s.filter_op] << " (i64)" << s.filter_val << "LL)) continue;\n";
+    if (s.nkeys == 2) E.os << "    const u32 key = ((u32)a.key0[row] << 8) | a.key1[row];\n";
+    else E.os << "    const u32 key = a.key0[row];\n";
+    E.os << "    const u32 g = slot_of(key, htab, &nslot, skey);\n";
+    E.os << "    if (g >= MAXS) { ovf = 1u; continue; }\n";
+    E.os << "    const u32 sbase = (copy * NS + g) * NP * AWR;\n";
+    emit_agg_row(E, s, cols, "    ");
+    E.os << "    atomicAdd(&acnt[copy * NS + g], 1u);\n";
+    E.os << "  }\n";
+    E.os << "  if (ovf) atomicOr(overflow, 1u);\n";
+    E.os << "  __syncthreads();\n";
+    E.os << "  u32* rec = a.partials + (usize)blockIdx.x * (MAXS * EW);\n";
+    E.os << "  const u32 ns = nslot < MAXS ? nslot : MAXS;\n";
+    E.os << "  for (u32 k = tid; k < MAXS * (NP + 1); k += T) {\n";
+    E.os << "    const u32 sl = k / (NP + 1), pp = k % (NP + 1);\n";
+    E.os << "    u32* e = rec + sl * EW;\n";
+    E.os << "    if (pp == 0) e[0] = sl < ns ? skey[sl] : 0u;\n";
+    E.os << "    if (pp < NP) {\n      Mag<AW2> sm = mzero<AW2>();\n";
+    E.os << "      if (sl < ns) for (u32 cc = 0; cc < NCOPY; cc++) add_to<AW2>(sm, sm, slot_value(&aslot[((cc * NS + sl) * NP + pp) * AWR]));\n";
+    E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) e[1 + pp * AW2 + w] = sm.w[w];\n";
+    E.os << "    } else {\n      u64 cnt = 0;\n";
+    E.os << "      if (sl < ns) for (u32 cc = 0; cc < NCOPY; cc++) cnt += acnt[cc * NS + sl];\n";
+    E.os << "      e[1 + NP * AW2] = (u32)cnt; e[2 + NP * AW2] = (u32)(cnt >> 32);\n    }\n  }\n}\n";
+
+    // final: meta[0] = ngroups, meta[1] = 1 when ngroups > maxg, meta[2..] = keys
+    E.os << "extern \"C\" __global__ void __launch_bounds__(256) " << name
+         << "_final(const u32* __restrict__ rec, u32 nblocks, u32 maxg, u8* __restrict__ images, u64* __restrict__ counts, u32* __restrict__ meta)\n{\n";
+    E.os << "  const u32 tid = threadIdx.x;\n";
+    E.os << "  __shared__ u32 bm[2048];\n  __shared__ u32 pre[2049];\n  __shared__ u32 part[256];\n";
+    E.os << "  __shared__ u32 red[4 * AW2 + 8];\n  __shared__ u32 gkey;\n";
+    E.os << "  for (u32 i = tid; i < 2048; i += 256) bm[i] = 0u;\n  __syncthreads();\n";
+    E.os << "  const u32 nrec = nblocks * MAXS;\n";
+    E.os << "  for (u32 i = tid; i < nrec; i += 256) { const u32 k1 = rec[(usize)i * EW]; if (k1) atomicOr(&bm[(k1 - 1u) >> 5], 1u << ((k1 - 1u) & 31u)); }\n";
+    E.os << "  __syncthreads();\n";
+    E.os << "  // prefix popcounts: thread t owns words [8t, 8t+8)\n";
+    E.os << "  u32 loc = 0;\n#pragma unroll\n  for (int j = 0; j < 8; j++) loc += __popc(bm[tid * 8 + j]);\n";
+    E.os << "  part[tid] = loc;\n  __syncthreads();\n";
+    E.os << "  if (tid == 0) { u32 acc = 0; for (int i = 0; i < 256; i++) { const u32 x = part[i]; part[i] = acc; acc += x; } pre[2048] = acc; }\n";
+    E.os << "  __syncthreads();\n";
+    E.os << "  { u32 acc = part[tid];\n#pragma unroll\n    for (int j = 0; j < 8; j++) { pre[tid * 8 + j] = acc; acc += __popc(bm[tid * 8 + j]); } }\n";
+    E.os << "  __syncthreads();\n";
+    E.os << "  const u32 ngroups = pre[2048];\n";
+    E.os << "  if (blockIdx.x == 0) {\n";
+    E.os << "    if (tid == 0) { meta[0] = ngroups; meta[1] = ngroups > maxg ? 1u : 0u; }\n";
+    E.os << "    for (u32 w = tid; w < 2048; w += 256) { u32 x = bm[w]; u32 r = pre[w];\n";
+    E.os << "      while (x) { const u32 b = __ffs(x) - 1u; x &= x - 1u; if (r < maxg) meta[2 + r] = w * 32u + b; r++; } }\n";
+    E.os << "  }\n";
+    E.os << "  const u32 g = blockIdx.x / (NP + 1), pp = blockIdx.x % (NP + 1);\n";
+    E.os << "  if (g >= ngroups || g >= maxg) return;\n";
+    E.os << "  for (u32 w = tid; w < 2048; w += 256) { u32 x = bm[w]; u32 r = pre[w];\n";
+    E.os << "    while (x) { const u32 b = __ffs(x) - 1u; x &= x - 1u; if (r == g) gkey = w * 32u + b + 1u; r++; } }\n";
+    E.os << "  __syncthreads();\n  const u32 want = gkey;\n";
+    E.os << "  if (pp < NP) {\n    Mag<AW2> sm = mzero<AW2>();\n";
+    E.os << "    for (u32 i = tid; i < nrec; i += 256) {\n      const u32* e = rec + (usize)i * EW;\n      if (e[0] != want) continue;\n";
+    E.os << "      Mag<AW2> o;\n#pragma unroll\n      for (int w = 0; w < AW2; w++) o.w[w] = e[1 + pp * AW2 + w];\n      add_to<AW2>(sm, sm, o);\n    }\n";
+    E.os << kBlockReduceAW2;
+    emit_agg_final_image(E, s, "images + ((usize)g * NP + pp) * 133u");
+    E.os << "    }\n  } else {\n    u64 c = 0;\n";
+    E.os << "    for (u32 i = tid; i < nrec; i += 256) { const u32* e = rec + (usize)i * EW; if (e[0] == want) c += (u64)e[1 + NP * AW2] | ((u64)e[2 + NP * AW2] << 32); }\n";
+    E.os << kBlockReduceCount;
+    E.os << "    if (tid == 0) { u64 t = 0; for (int wv = 0; wv < 4; wv++) t += (u64)red[2 * wv] | ((u64)red[2 * wv + 1] << 32); counts[g] = t; }\n  }\n}\n";
+    return E.os.str();
+}
+
+std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& cols, const std::string& name,
+                            const GenOptions& o, int* acc_words, int* block_out)
+{
+    if (s.onepass) return gen_agg_onepass(s, cols, name, o, acc_words, block_out);
+    Emit E;
+    const int NP = (int)s.plans.size();
+    const int NG = s.ngroups;
+    const int AW2 = agg_acc_words(s);
+    *acc_words = AW2;
+    const bool reg = NG == 1;
+    AccCfg c;
+    if (!reg && !choose_acc(s, NG, o, &c)) return std::string();
+    if (reg) c.T = o.block;
+    *block_out = c.T;
+    const int PW = NP * NG * AW2 + 2 * NG;  // partial words per block
+    const char* fops[] = {"<=", "<", ">=", ">", "==", "!="};
+    emit_agg_prelude(E, s, c, NG);
+    E.os << "// accumulation: " << (reg ? "registers" : "LDS 64-bit atomic slots") << "\n";
+    E.os << "#define NG " << NG << "\n#define PW " << PW << "\n";
+    // a lane accumulates at most 2^14 rows (kAggRowsPerLaneLog2)
+    const int vbits = agg_value_bits(s);
+    const int AW1 = words_for(vbits + kAggRowsPerLaneLog2 + 1);
+    E.os << "#define AW1 " << AW1 << "\n";
+    E.os << "extern \"C\" __global__ void __launch_bounds__(" << c.T << ") " << name << "_main(AggArgs a)\n{\n";
+    E.os << "  const u32 tid = threadIdx.x;\n  const u32 wave = tid >> 6, lane = tid & 63u;\n";
+    if (!reg) {
+        E.os << "  __shared__ u64 aslot[NCOPY * NS * NP * AWR];\n  __shared__ u32 acnt[NCOPY * NS];\n";
+        E.os << "  for (u32 i = tid; i < NCOPY * NS * NP * AWR; i += T) aslot[i] = 0ull;\n";
+        E.os << "  for (u32 i = tid; i < NCOPY * NS; i += T) acnt[i] = 0u;\n";
         E.os << "  __syncthreads();\n";
         E.os << "  const u32 copy = wave * SPLIT + lane / (64 / SPLIT);\n";
     } else {
@@ -1124,77 +1352,56 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
     }
     E.os << "  for (usize row = (usize)blockIdx.x * T + tid; row < a.rows; row += (usize)gridDim.x * T) {\n";
     if (s.filter_col >= 0) E.os << "    if (!(a.fcol[row] " << fops[s.filter_op] << " (i64)" << s.filter_val << "LL)) continue;\n";
-    if (s.nkeys == 2) E.os << "    const u32 g = a.lut[((u32)a.key0[row] << 8) | a.key1[row]];\n";
-    else if (s.nkeys == 1) E.os << "    const u32 g = a.lut[a.key0[row]];\n";
-    else E.os << "    const u32 g = 0;\n";
-    if (strat == ATOMIC) E.os << "    const u32 sbase = (copy * NG + g) * NP;\n";
-    std::vector<std::string> loaded(cols.size());
-    for (int i = 0; i < NP; i++) {
-        const Plan& p = s.plans[i];
-        const int W = p.n[p.root].W;
-        std::string rv = emit_plan(E, p, cols, loaded, "    ", "p" + std::to_string(i) + "_");
-        E.os << "    {\n      const Dec<" << W << "> v = stored<" << p.outB << ", " << W << ">(" << rv << ");\n";
-        if (strat == PRIVATE) {
-            E.os << "      u32* sp = slot + ((g * NP + " << i << ") * AW1) * T + tid;\n";
-            E.os << "      Mag<AW1> s;\n#pragma unroll\n      for (int w = 0; w < AW1; w++) s.w[w] = sp[w * T];\n";
-            E.os << "      acc_add<AW1, " << W << ">(s, v);\n";
-            E.os << "#pragma unroll\n      for (int w = 0; w < AW1; w++) sp[w * T] = s.w[w];\n";
-        } else if (strat == ATOMIC) {
-            E.os << "      atomicAdd(&aslot[sbase + " << i << "], (u64)as_i64<" << W << ">(v));\n";
-        } else {
-            E.os << "      acc_add<AW1, " << W << ">(acc" << i << ", v);\n";
+    if (reg) {
+        std::vector<std::string> loaded(cols.size());
+        for (int i = 0; i < NP; i++) {
+            const Plan& p = s.plans[i];
+            const int W = p.n[p.root].W;
+            std::string rv = emit_plan(E, p, cols, loaded, "    ", "p" + std::to_string(i) + "_");
+            E.os << "    acc_add<AW1, " << W << ">(acc" << i << ", stored<" << p.outB << ", " << W << ">(" << rv << "));\n";
         }
-        E.os << "    }\n";
+        E.os << "    cnt += 1u;\n  }\n";
+    } else {
+        if (s.nkeys == 2) E.os << "    const u32 g = a.lut[((u32)a.key0[row] << 8) | a.key1[row]];\n";
+        else E.os << "    const u32 g = a.lut[a.key0[row]];\n";
+        E.os << "    const u32 sbase = (copy * NS + g) * NP * AWR;\n";
+        emit_agg_row(E, s, cols, "    ");
+        E.os << "    atomicAdd(&acnt[copy * NS + g], 1u);\n  }\n";
     }
-    if (strat == PRIVATE) E.os << "    cslot[g * T + tid] += 1u;\n";
-    else if (strat == ATOMIC) E.os << "    atomicAdd(&acnt[copy * NG + g], 1u);\n";
-    else E.os << "    cnt += 1u;\n";
-    E.os << "  }\n";
     E.os << "  __shared__ u32 red[NWAVE * PW];\n";
-    if (strat == ATOMIC) {
+    if (!reg) {
         // combine the slot copies of the block: one thread per (group, plan)
         E.os << "  __syncthreads();\n";
         E.os << "  for (u32 k = tid; k < NP * NG + NG; k += T) {\n";
         E.os << "    u32* dst = a.partials + (usize)blockIdx.x * PW;\n";
-        E.os << "    if (k < NP * NG) {\n      const u32 gg = k / NP, pp = k % NP;\n      Mag<AW2> s = mzero<AW2>();\n";
-        E.os << "      for (u32 c = 0; c < NWAVE * SPLIT; c++) {\n        const u64 x = aslot[(c * NG + gg) * NP + pp];\n";
-        E.os << "        Mag<AW2> o;\n        const u32 ext = (x >> 63) ? 0xffffffffu : 0u;\n";
-        E.os << "#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = w == 0 ? (u32)x : (w == 1 ? (u32)(x >> 32) : ext);\n";
-        E.os << "        add_to<AW2>(s, s, o);\n      }\n";
-        E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) dst[k * AW2 + w] = s.w[w];\n";
-        E.os << "    } else {\n      const u32 gg = k - NP * NG;\n      u64 c = 0;\n";
-        E.os << "      for (u32 cc = 0; cc < NWAVE * SPLIT; cc++) c += acnt[cc * NG + gg];\n";
-        E.os << "      dst[NP * NG * AW2 + 2 * gg] = (u32)c; dst[NP * NG * AW2 + 2 * gg + 1] = (u32)(c >> 32);\n    }\n  }\n}\n";
+        E.os << "    if (k < NP * NG) {\n      const u32 gg = k / NP, pp = k % NP;\n      Mag<AW2> sm = mzero<AW2>();\n";
+        E.os << "      for (u32 cc = 0; cc < NCOPY; cc++) add_to<AW2>(sm, sm, slot_value(&aslot[((cc * NS + gg) * NP + pp) * AWR]));\n";
+        E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) dst[k * AW2 + w] = sm.w[w];\n";
+        E.os << "    } else {\n      const u32 gg = k - NP * NG;\n      u64 cn = 0;\n";
+        E.os << "      for (u32 cc = 0; cc < NCOPY; cc++) cn += acnt[cc * NS + gg];\n";
+        E.os << "      dst[NP * NG * AW2 + 2 * gg] = (u32)cn; dst[NP * NG * AW2 + 2 * gg + 1] = (u32)(cn >> 32);\n    }\n  }\n}\n";
     } else {
         // block reduction: each wave reduces, wave leaders combine through LDS
-        if (strat == PRIVATE) E.os << "  __syncthreads();\n";
-        E.os << "#pragma unroll\n  for (int gg = 0; gg < NG; gg++) {\n";
-        E.os << "#pragma unroll\n    for (int pp = 0; pp < NP; pp++) {\n";
-        E.os << "      Mag<AW1> s1;\n";
-        if (strat == PRIVATE)
-            E.os << "#pragma unroll\n      for (int w = 0; w < AW1; w++) s1.w[w] = slot[((gg * NP + pp) * AW1 + w) * T + tid];\n";
-        else {
-            E.os << "      s1 = mzero<AW1>();\n";
-            for (int i = 0; i < NP; i++) E.os << "      if (pp == " << i << ") s1 = acc" << i << ";\n";
-        }
-        E.os << "      Mag<AW2> s2;\n      const u32 ext = (s1.w[AW1 - 1] >> 31) ? 0xffffffffu : 0u;\n";
-        E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) s2.w[w] = w < AW1 ? s1.w[w] : ext;\n";
-        E.os << "      wave_reduce<AW2>(s2);\n";
-        E.os << "      if (lane == 0) {\n#pragma unroll\n        for (int w = 0; w < AW2; w++) red[wave * PW + (gg * NP + pp) * AW2 + w] = s2.w[w];\n      }\n";
-        E.os << "    }\n";
-        if (strat == PRIVATE) E.os << "    u64 c = cslot[gg * T + tid];\n";
-        else E.os << "    u64 c = cnt;\n";
+        E.os << "#pragma unroll\n  for (int pp = 0; pp < NP; pp++) {\n";
+        E.os << "    Mag<AW1> s1 = mzero<AW1>();\n";
+        for (int i = 0; i < NP; i++) E.os << "    if (pp == " << i << ") s1 = acc" << i << ";\n";
+        E.os << "    Mag<AW2> s2;\n    const u32 ext = (s1.w[AW1 - 1] >> 31) ? 0xffffffffu : 0u;\n";
+        E.os << "#pragma unroll\n    for (int w = 0; w < AW2; w++) s2.w[w] = w < AW1 ? s1.w[w] : ext;\n";
+        E.os << "    wave_reduce<AW2>(s2);\n";
+        E.os << "    if (lane == 0) {\n#pragma unroll\n      for (int w = 0; w < AW2; w++) red[wave * PW + pp * AW2 + w] = s2.w[w];\n    }\n";
+        E.os << "  }\n";
+        E.os << "  {\n    u64 c = cnt;\n";
         E.os << "#pragma unroll\n    for (int off = 32; off >= 1; off >>= 1) c += (u64)__shfl_xor((long long)c, off, 64);\n";
-        E.os << "    if (lane == 0) { red[wave * PW + NP * NG * AW2 + 2 * gg] = (u32)c; red[wave * PW + NP * NG * AW2 + 2 * gg + 1] = (u32)(c >> 32); }\n";
+        E.os << "    if (lane == 0) { red[wave * PW + NP * AW2] = (u32)c; red[wave * PW + NP * AW2 + 1] = (u32)(c >> 32); }\n";
         E.os << "  }\n  __syncthreads();\n";
-        E.os << "  for (u32 k = tid; k < NP * NG + NG; k += T) {\n";
+        E.os << "  for (u32 k = tid; k < NP + 1; k += T) {\n";
         E.os << "    u32* dst = a.partials + (usize)blockIdx.x * PW;\n";
-        E.os << "    if (k < NP * NG) {\n      Mag<AW2> s = mzero<AW2>();\n";
-        E.os << "      for (int wv = 0; wv < NWAVE; wv++) {\n        Mag<AW2> o;\n#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = red[wv * PW + k * AW2 + w];\n        add_to<AW2>(s, s, o);\n      }\n";
-        E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) dst[k * AW2 + w] = s.w[w];\n";
-        E.os << "    } else {\n      const u32 gg = k - NP * NG;\n      u64 c = 0;\n";
-        E.os << "      for (int wv = 0; wv < NWAVE; wv++) c += (u64)red[wv * PW + NP * NG * AW2 + 2 * gg] | ((u64)red[wv * PW + NP * NG * AW2 + 2 * gg + 1] << 32);\n";
-        E.os << "      dst[NP * NG * AW2 + 2 * gg] = (u32)c; dst[NP * NG * AW2 + 2 * gg + 1] = (u32)(c >> 32);\n    }\n  }\n}\n";
+        E.os << "    if (k < NP) {\n      Mag<AW2> sm = mzero<AW2>();\n";
+        E.os << "      for (int wv = 0; wv < NWAVE; wv++) {\n        Mag<AW2> o;\n#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = red[wv * PW + k * AW2 + w];\n        add_to<AW2>(sm, sm, o);\n      }\n";
+        E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) dst[k * AW2 + w] = sm.w[w];\n";
+        E.os << "    } else {\n      u64 cn = 0;\n";
+        E.os << "      for (int wv = 0; wv < NWAVE; wv++) cn += (u64)red[wv * PW + NP * AW2] | ((u64)red[wv * PW + NP * AW2 + 1] << 32);\n";
+        E.os << "      dst[NP * AW2] = (u32)cn; dst[NP * AW2 + 1] = (u32)(cn >> 32);\n    }\n  }\n}\n";
     }
 
     // final: one block per (group, plan) sum and one per group count
@@ -1202,21 +1409,14 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
          << "_final(const u32* __restrict__ partials, u32 nblocks, u8* __restrict__ images, u64* __restrict__ counts)\n{\n";
     E.os << "  const u32 k = blockIdx.x, tid = threadIdx.x;\n";
     E.os << "  __shared__ u32 red[4 * AW2 + 8];\n";
-    E.os << "  if (k < NP * NG) {\n    Mag<AW2> s = mzero<AW2>();\n";
-    E.os << "    for (u32 b = tid; b < nblocks; b += 256) {\n      Mag<AW2> o;\n#pragma unroll\n      for (int w = 0; w < AW2; w++) o.w[w] = partials[(usize)b * PW + k * AW2 + w];\n      add_to<AW2>(s, s, o);\n    }\n";
-    E.os << "    wave_reduce<AW2>(s);\n";
-    E.os << "    if ((tid & 63u) == 0) {\n#pragma unroll\n      for (int w = 0; w < AW2; w++) red[(tid >> 6) * AW2 + w] = s.w[w];\n    }\n";
-    E.os << "    __syncthreads();\n    if (tid == 0) {\n      Mag<AW2> t = mzero<AW2>();\n      for (int wv = 0; wv < 4; wv++) {\n        Mag<AW2> o;\n#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = red[wv * AW2 + w];\n        add_to<AW2>(t, t, o);\n      }\n";
-    E.os << "      const u32 pp = k % NP;\n      u32 prec = 0, frac = 0;\n";
-    for (int i = 0; i < NP; i++)
-        E.os << "      if (pp == " << i << ") { prec = " << (s.plans[i].outP & 0xff) << "u; frac = " << (s.plans[i].outS & 0xff)
-             << "u; }\n";
-    E.os << "      write_sum_image<AW2>(images + (usize)k * 133u, t, prec, frac, 0u);\n    }\n";
-    E.os << "  } else {\n    const u32 gg = k - NP * NG;\n    u64 c = 0;\n";
+    E.os << "  if (k < NP * NG) {\n    const u32 pp = k % NP;\n    Mag<AW2> sm = mzero<AW2>();\n";
+    E.os << "    for (u32 b = tid; b < nblocks; b += 256) {\n      Mag<AW2> o;\n#pragma unroll\n      for (int w = 0; w < AW2; w++) o.w[w] = partials[(usize)b * PW + k * AW2 + w];\n      add_to<AW2>(sm, sm, o);\n    }\n";
+    E.os << kBlockReduceAW2;
+    emit_agg_final_image(E, s, "images + (usize)k * 133u");
+    E.os << "    }\n  } else {\n    const u32 gg = k - NP * NG;\n    u64 c = 0;\n";
     E.os << "    for (u32 b = tid; b < nblocks; b += 256) c += (u64)partials[(usize)b * PW + NP * NG * AW2 + 2 * gg] | ((u64)partials[(usize)b * PW + NP * NG * AW2 + 2 * gg + 1] << 32);\n";
-    E.os << "#pragma unroll\n    for (int off = 32; off >= 1; off >>= 1) c += (u64)__shfl_xor((long long)c, off, 64);\n";
-    E.os << "    if ((tid & 63u) == 0) { red[2 * (tid >> 6)] = (u32)c; red[2 * (tid >> 6) + 1] = (u32)(c >> 32); }\n";
-    E.os << "    __syncthreads();\n    if (tid == 0) { u64 t = 0; for (int wv = 0; wv < 4; wv++) t += (u64)red[2 * wv] | ((u64)red[2 * wv + 1] << 32); counts[gg] = t; }\n  }\n}\n";
+    E.os << kBlockReduceCount;
+    E.os << "    if (tid == 0) { u64 t = 0; for (int wv = 0; wv < 4; wv++) t += (u64)red[2 * wv] | ((u64)red[2 * wv + 1] << 32); counts[gg] = t; }\n  }\n}\n";
     return E.os.str();
 }
 

@@ -134,9 +134,22 @@ struct AggSpec {
     int64_t filter_val = 0;
     int nkeys = 0;            // 0, 1 or 2 key byte columns
     int ngroups = 1;          // groups addressed by the LUT (max)
+    // one-pass form: every block numbers the keys it meets in an LDS hash
+    // table of at most `maxslots` entries; the final kernel merges the
+    // per-block tables by key.  Needs nkeys > 0 and 64-bit accumulators.
+    bool onepass = false;
+    int maxslots = 16;
 };
+
+// true when the one-pass grouped form applies to this aggregation
+bool agg_onepass_ok(const AggSpec& s, const GenOptions& o);
+// words per block record of the one-pass form: MAXS entries of
+// {key+1, NP x AW2 sums, 2 count words}
+int agg_onepass_record_words(const AggSpec& s);
+// Returns an empty string when the accumulators cannot fit in LDS;
+// *block_out receives the block size the kernels were generated for.
 std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& cols, const std::string& name,
-                            const GenOptions& o, int* acc_words);
+                            const GenOptions& o, int* acc_words, int* block_out);
 // limbs of one block partial of the fused aggregation
 int agg_acc_words(const AggSpec& s);
 

@@ -371,11 +371,23 @@ struct updec_agg_s {
     std::vector<ColumnType> types;
     int nkeys = 0, has_filter = 0;
     std::string base_name, last_src;
-    std::map<int, std::pair<ModuleP, std::string>> by_ngroups;
+    struct GenMod {
+        ModuleP mod;
+        std::string src;
+        int block = 256;
+    };
+    std::map<int, GenMod> by_ngroups;
     u32* d_bitmap = nullptr;
     u8* d_lut = nullptr;
     u32* d_partials = nullptr;
     size_t partial_words = 0;
+    // one-pass grouped form
+    bool onepass = false;
+    ModuleP op_mod;
+    std::string op_src;
+    int op_block = 256;
+    u32* d_meta = nullptr;  // [0] overflow flag, [1] ngroups, [2] too many, [3..] keys
+    u32* h_meta = nullptr;  // pinned host copy
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     ~updec_agg_s()
@@ -383,6 +395,8 @@ struct updec_agg_s {
         if (d_bitmap) (void)hipFree(d_bitmap);
         if (d_lut) (void)hipFree(d_lut);
         if (d_partials) (void)hipFree(d_partials);
+        if (d_meta) (void)hipFree(d_meta);
+        if (h_meta) (void)hipHostFree(h_meta);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
     }
@@ -620,28 +634,138 @@ int updec_agg_compile(const updec_agg_desc* d, updec_agg_t* out)
     a->nkeys = d->nkeys;
     a->spec.nkeys = d->nkeys;
     a->base_name = "agg";
+    a->onepass = agg_onepass_ok(a->spec, GenOptions());
     *out = a.release();
     return UPDEC_OK;
 }
 
-static int agg_module(updec_agg_t a, int ngroups, ModuleP* mod, std::string* err)
+static int agg_module(updec_agg_t a, int ngroups, ModuleP* mod, int* block, std::string* err)
 {
     auto it = a->by_ngroups.find(ngroups);
     if (it != a->by_ngroups.end()) {
-        *mod = it->second.first;
-        a->last_src = it->second.second;
+        *mod = it->second.mod;
+        *block = it->second.block;
+        a->last_src = it->second.src;
         return UPDEC_OK;
     }
     AggSpec s = a->spec;
     s.ngroups = ngroups;
     GenOptions o;
-    int aw = 0;
-    std::string src = gen_agg_kernels(s, a->types, a->base_name, o, &aw);
+    int aw = 0, blk = 256;
+    std::string src = gen_agg_kernels(s, a->types, a->base_name, o, &aw, &blk);
+    if (src.empty()) {
+        *err = "the accumulators of " + std::to_string(ngroups) + " groups do not fit in LDS";
+        return UPDEC_EUNSUPPORTED;
+    }
     ModuleP m = ModuleCache::get().compile(src, err);
     if (!m) return UPDEC_ECOMPILE;
-    a->by_ngroups[ngroups] = {m, src};
+    a->by_ngroups[ngroups] = {m, src, blk};
     a->last_src = src;
     *mod = m;
+    *block = blk;
+    return UPDEC_OK;
+}
+
+static void fill_args(updec_agg_t a, AggArgs* args, const uint8_t* const* d_cols, const int32_t* d_filter,
+                      const uint8_t* d_key0, const uint8_t* d_key1, int64_t rows)
+{
+    *args = AggArgs{};
+    for (size_t i = 0; i < a->types.size(); i++) {
+        args->col[i] = d_cols ? d_cols[i] : nullptr;
+        args->colbytes[i] = (usize)rows * (usize)compact_bytes(a->types[i].P);
+    }
+    args->rows = (usize)rows;
+    args->fcol = d_filter;
+    args->key0 = d_key0;
+    args->key1 = d_key1;
+    args->lut = a->d_lut;
+    args->partials = a->d_partials;
+}
+
+static int64_t agg_grid(hipFunction_t f, int64_t rows, int T)
+{
+    int64_t tiles = std::max<int64_t>(1, (rows + T - 1) / T);
+    int blocks_per_cu = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, f, T, 0) != hipSuccess || blocks_per_cu < 1)
+        blocks_per_cu = 2;
+    int64_t grid = std::min<int64_t>(tiles, (int64_t)device_cus() * blocks_per_cu);
+    const int64_t per_block = (int64_t)T << kAggRowsPerLaneLog2;
+    int64_t min_grid = (rows + per_block - 1) / per_block;  // every lane <= 2^14 rows
+    return std::max(grid, std::max<int64_t>(min_grid, 1));
+}
+
+// One pass: keys numbered per block in LDS, merged on the device; one
+// host synchronization at the end to return the group count and keys.
+// Sets *overflowed when some block met more distinct keys than its table
+// holds; the caller then runs the two-pass form.
+static int agg_run_onepass(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_filter, const uint8_t* d_key0,
+                           const uint8_t* d_key1, int64_t rows, uint8_t* d_images, int64_t* d_counts, int32_t max_groups,
+                           int32_t* ngroups_out, uint16_t* keys_out, hipStream_t st, bool* overflowed)
+{
+    *overflowed = false;
+    std::string err;
+    if (!a->op_mod) {
+        AggSpec s = a->spec;
+        s.onepass = true;
+        GenOptions o;
+        int aw = 0;
+        a->op_src = gen_agg_kernels(s, a->types, a->base_name, o, &aw, &a->op_block);
+        if (a->op_src.empty()) return fail(UPDEC_EUNSUPPORTED, "one-pass accumulators do not fit in LDS");
+        a->op_mod = ModuleCache::get().compile(a->op_src, &err);
+        if (!a->op_mod) return fail(UPDEC_ECOMPILE, err);
+    }
+    a->last_src = a->op_src;
+    hipFunction_t fmain = a->op_mod->fn(a->base_name + "_main", &err);
+    hipFunction_t ffin = fmain ? a->op_mod->fn(a->base_name + "_final", &err) : nullptr;
+    if (!fmain || !ffin) return fail(UPDEC_ECOMPILE, err);
+    const int T = a->op_block;
+    const int64_t grid = agg_grid(fmain, rows, T);
+    const size_t need = (size_t)grid * (size_t)agg_onepass_record_words(a->spec);
+    if (need > a->partial_words) {
+        if (a->d_partials) HIPCHK(hipFree(a->d_partials));
+        a->d_partials = nullptr;
+        HIPCHK(hipMalloc((void**)&a->d_partials, need * sizeof(u32)));
+        a->partial_words = need;
+    }
+    if (!a->d_meta) {
+        HIPCHK(hipMalloc((void**)&a->d_meta, (3 + 256) * sizeof(u32)));
+        HIPCHK(hipHostMalloc((void**)&a->h_meta, (3 + 256) * sizeof(u32)));
+    }
+    HIPCHK(hipMemsetAsync(a->d_meta, 0, sizeof(u32), st));
+    AggArgs args;
+    fill_args(a, &args, d_cols, d_filter, d_key0, d_key1, rows);
+    u32* ovf = a->d_meta;
+    if (!a->ev0) {
+        HIPCHK(hipEventCreate(&a->ev0));
+        HIPCHK(hipEventCreate(&a->ev1));
+    }
+    HIPCHK(hipEventRecord(a->ev0, st));
+    void* pm[] = {&args, &ovf};
+    HIPCHK(hipModuleLaunchKernel(fmain, (unsigned)grid, 1, 1, T, 1, 1, 0, st, pm, nullptr));
+    HIPCHK(hipEventRecord(a->ev1, st));
+    a->timed = true;
+    const u32* rec = a->d_partials;
+    u32 nb = (u32)grid, maxg = (u32)max_groups;
+    uint8_t* img = d_images;
+    uint64_t* cnt = reinterpret_cast<uint64_t*>(d_counts);
+    u32* meta = a->d_meta + 1;
+    void* pf[] = {&rec, &nb, &maxg, &img, &cnt, &meta};
+    const int NP = (int)a->spec.plans.size();
+    HIPCHK(hipModuleLaunchKernel(ffin, (unsigned)(max_groups * (NP + 1)), 1, 1, 256, 1, 1, 0, st, pf, nullptr));
+    HIPCHK(hipMemcpyAsync(a->h_meta, a->d_meta, (3 + (size_t)max_groups) * sizeof(u32), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (a->h_meta[0]) {
+        *overflowed = true;
+        return UPDEC_OK;
+    }
+    if (a->h_meta[2]) return fail(UPDEC_EINVAL, "more groups than max_groups");
+    const int ng = (int)a->h_meta[1];
+    *ngroups_out = ng;
+    if (keys_out)
+        for (int g = 0; g < ng; g++) {
+            const u32 k = a->h_meta[3 + g];
+            keys_out[g] = (uint16_t)(a->nkeys == 2 ? (((k >> 8) & 0xff) | ((k & 0xff) << 8)) : k);
+        }
     return UPDEC_OK;
 }
 
@@ -650,10 +774,18 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
                   int32_t* ngroups_out, uint16_t* keys_out, updec_stream_t stream)
 {
     if (!a || rows < 0 || !d_images || !d_counts || !ngroups_out) return fail(UPDEC_EINVAL, "bad argument");
+    if (max_groups < 1 || max_groups > 255) return fail(UPDEC_EINVAL, "max_groups must be in [1, 255]");
     if (a->has_filter && !d_filter) return fail(UPDEC_EINVAL, "filter column missing");
     if (a->nkeys >= 1 && !d_key0) return fail(UPDEC_EINVAL, "key column missing");
     if (a->nkeys == 2 && !d_key1) return fail(UPDEC_EINVAL, "second key column missing");
     hipStream_t st = S(stream);
+    if (a->onepass) {
+        bool overflowed = false;
+        int rc = agg_run_onepass(a, d_cols, d_filter, d_key0, d_key1, rows, d_images, d_counts, max_groups, ngroups_out,
+                                 keys_out, st, &overflowed);
+        if (rc != UPDEC_OK || !overflowed) return rc;
+        // more distinct keys in a block than the one-pass table holds: two-pass form
+    }
     int ngroups = 1;
     std::vector<uint16_t> keys;
     if (a->nkeys > 0) {
@@ -691,40 +823,25 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
     }
     std::string err;
     ModuleP mod;
-    if (agg_module(a, ngroups, &mod, &err) != UPDEC_OK) return fail(UPDEC_ECOMPILE, err);
+    int T = 256;
+    int mrc = agg_module(a, ngroups, &mod, &T, &err);
+    if (mrc != UPDEC_OK) return fail(mrc, err);
     hipFunction_t fmain = mod->fn(a->base_name + "_main", &err);
     hipFunction_t ffin = fmain ? mod->fn(a->base_name + "_final", &err) : nullptr;
     if (!fmain || !ffin) return fail(UPDEC_ECOMPILE, err);
 
-    const int T = 256;
     const int NP = (int)a->spec.plans.size();
-    int aw = 0;
-    aw = agg_acc_words(a->spec);
+    const int aw = agg_acc_words(a->spec);
     const size_t PW = (size_t)NP * ngroups * aw + 2 * (size_t)ngroups;
-    int64_t tiles = std::max<int64_t>(1, (rows + T - 1) / T);
-    int blocks_per_cu = 0;
-    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, fmain, T, 0) != hipSuccess || blocks_per_cu < 1)
-        blocks_per_cu = 2;
-    int64_t grid = std::min<int64_t>(tiles, (int64_t)device_cus() * blocks_per_cu);
-    const int64_t per_block = (int64_t)T << kAggRowsPerLaneLog2;
-    int64_t min_grid = (rows + per_block - 1) / per_block;  // every lane <= 2^14 rows
-    grid = std::max(grid, std::max<int64_t>(min_grid, 1));
+    const int64_t grid = agg_grid(fmain, rows, T);
     if ((size_t)grid * PW > a->partial_words) {
         if (a->d_partials) HIPCHK(hipFree(a->d_partials));
+        a->d_partials = nullptr;
+        HIPCHK(hipMalloc((void**)&a->d_partials, (size_t)grid * PW * sizeof(u32)));
         a->partial_words = (size_t)grid * PW;
-        HIPCHK(hipMalloc((void**)&a->d_partials, a->partial_words * sizeof(u32)));
     }
-    AggArgs args{};
-    for (size_t i = 0; i < a->types.size(); i++) {
-        args.col[i] = d_cols ? d_cols[i] : nullptr;
-        args.colbytes[i] = (usize)rows * (usize)compact_bytes(a->types[i].P);
-    }
-    args.rows = (usize)rows;
-    args.fcol = d_filter;
-    args.key0 = d_key0;
-    args.key1 = d_key1;
-    args.lut = a->d_lut;
-    args.partials = a->d_partials;
+    AggArgs args;
+    fill_args(a, &args, d_cols, d_filter, d_key0, d_key1, rows);
     if (!a->ev0) {
         HIPCHK(hipEventCreate(&a->ev0));
         HIPCHK(hipEventCreate(&a->ev1));
@@ -753,12 +870,16 @@ int updec_agg_last_kernel_ms(updec_agg_t a, float* ms)
 
 int updec_agg_source(updec_agg_t a, int32_t ngroups, char* buf, size_t cap, size_t* needed)
 {
-    if (!a || ngroups < 1 || ngroups > 255) return fail(UPDEC_EINVAL, "bad argument");
+    if (!a || ngroups < 0 || ngroups > 255) return fail(UPDEC_EINVAL, "bad argument");
+    if (ngroups == 0 && !a->onepass) return fail(UPDEC_EINVAL, "no one-pass form for this aggregation");
     AggSpec s = a->spec;
+    s.onepass = ngroups == 0;
     s.ngroups = ngroups;
     GenOptions o;
-    int aw = 0;
-    return copy_text(gen_agg_kernels(s, a->types, a->base_name, o, &aw), buf, cap, needed);
+    int aw = 0, blk = 0;
+    std::string src = gen_agg_kernels(s, a->types, a->base_name, o, &aw, &blk);
+    if (src.empty()) return fail(UPDEC_EUNSUPPORTED, "accumulators do not fit in LDS");
+    return copy_text(src, buf, cap, needed);
 }
 
 void updec_agg_free(updec_agg_t a) { delete a; }
