@@ -9,8 +9,10 @@ discovery, fused filter/expression/SUM kernel, final combine, the three
 AVG divisions) over the device-resident columns.
 
 Also reported: the plain-method workload (1,000,000 rows, SUM(p*(1-d)) and
-SUM(q*p)) and the precision sweep ((a+b)*c - a/d over 10,000,000 rows at
-P = 18, 38, 76, 150, 300), each with its kernel time and bytes.
+SUM(q*p)), the precision sweep ((a+b)*c - a/d over 10,000,000 rows at
+P = 18, 38, 76, 150, 300) and the high-precision series (sin(x) by a
+25-term Horner series over 1,000,000 DECIMAL(300,100) rows, then SUM), each
+with its kernel time and bytes.
 
 Prints one JSON line.
 """
@@ -163,6 +165,45 @@ def precision_sweep(U, W, steps, rows_override=None):
     return out
 
 
+def series(U, W, steps, cpu_rows, threads):
+    """sin(x) by the 25-term Horner series over x DECIMAL(300,100) in
+    [-pi, pi], SUM over 1,000,000 rows (one fused aggregation launch)"""
+    C = W.CONFIG4
+    rows = C["rows"]
+    dev = U.gen_compact_device(W.sin_desc(U, C["seed"]), rows)
+    bits = U.column_maxbits(dev, 300, rows)
+    cols = [("x", 300, 100, bits)]
+    t0 = time.time()
+    agg = U.Agg([W.sin_expr()], cols)
+    images = U.DeviceBuffer(133).zero()
+    counts = U.DeviceBuffer(8).zero()
+    agg.run([dev], rows, images, counts, max_groups=1)
+    U.device_sync()
+    first_s = time.time() - t0
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        agg.run([dev], rows, images, counts, max_groups=1)
+        kms.append(agg.last_kernel_ms())
+    U.device_sync()
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    kernel_ms = sum(kms) / len(kms)
+    res = {"rows": rows, "column_bits": bits, "ms_per_step": ms, "kernel_ms": kernel_ms,
+           "mrows_per_s": rows / ms / 1e3, "kernel_mrows_per_s": rows / kernel_ms / 1e3,
+           "first_call_s": first_s, "alg_bytes": rows * U.compact_bytes(300)}
+    if cpu_rows > 0:
+        from oracle import oracle as O
+        n = min(cpu_rows, rows)
+        host = dev.numpy(count=n * U.compact_bytes(300))
+        plan = O.Plan(U.Expr(W.sin_expr(), cols).plan)
+        t0 = time.perf_counter()
+        O.plan_sum(plan, [host], [300], [100], n, mode=O.EXTENDED, threads=threads)
+        cpu_s = time.perf_counter() - t0
+        res["cpu"] = {"rows": n, "seconds": cpu_s, "mrows_per_s": n / cpu_s / 1e6, "threads": threads,
+                      "kind": "port"}
+    return res
+
+
 def copy_bandwidth(U, nbytes=1 << 30, reps=5):
     a = U.DeviceBuffer(nbytes)
     b = U.DeviceBuffer(nbytes)
@@ -201,8 +242,9 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=2_000_000, help="rows of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="headline only")
-    ap.add_argument("--extra", default="copy,plain,sweep", help="comma list of extra workloads")
+    ap.add_argument("--extra", default="copy,plain,sweep,series", help="comma list of extra workloads")
     ap.add_argument("--sweep-rows", type=int, default=0)
+    ap.add_argument("--series-cpu-rows", type=int, default=50_000, help="rows of the series CPU sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -227,6 +269,9 @@ def main():
     if "sweep" in want:
         extra["sweep_10M"] = precision_sweep(U, W, max(3, args.steps // 4), args.sweep_rows or None)
         log("sweep:", extra["sweep_10M"])
+    if "series" in want:
+        extra["series_1M"] = series(U, W, max(2, args.steps // 5), args.series_cpu_rows, threads)
+        log("series:", extra["series_1M"])
     value = h["rows"] / h["ms_per_step"] / 1e3  # Mrows/s
     achieved = h["achieved_gbps"]
     cpu = h.get("cpu")

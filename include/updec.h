@@ -178,9 +178,12 @@ int updec_check_compile(const char* src, size_t len);
  *   UPDEC_GEN_RANGE:  magnitude = (lo + r mod (hi - lo + 1)) * step   (64-bit)
  *   UPDEC_GEN_DIGITS: magnitude uniform with exactly `digits` decimal digits
  *                     (leading digit nonzero: uniform in [10^(digits-1), 10^digits))
+ *   UPDEC_GEN_BELOW:  magnitude uniform in [0, bound), bound given as
+ *                     bound_words little-endian 32-bit words (<= 16)
  * neg_permille: probability of a negative sign in 1/1000. */
 #define UPDEC_GEN_RANGE 0
 #define UPDEC_GEN_DIGITS 1
+#define UPDEC_GEN_BELOW 2
 typedef struct updec_gen_desc {
     int32_t kind;
     int32_t precision;
@@ -189,6 +192,8 @@ typedef struct updec_gen_desc {
     uint64_t lo, hi, step;
     uint64_t seed;
     uint32_t column_id;
+    int32_t bound_words;
+    uint32_t bound[16];
 } updec_gen_desc;
 int updec_gen_compact(const updec_gen_desc* g, int64_t row0, int64_t rows, uint8_t* out, int on_device,
                       updec_stream_t stream);

@@ -45,22 +45,32 @@ static int bl32(uint32_t x) { return x ? 32 - __builtin_clz(x) : 0; }
 static int top_word(const uint32_t* v, int nw) { int i = nw - 1; while (i >= 0 && v[i] == 0) i--; return i; }
 static int bitlen_w(const uint32_t* v, int nw) { int t = top_word(v, nw); return t < 0 ? 0 : 32 * t + bl32(v[t]); }
 
+/* closed form past the table: bytes for bitlen(10^P - 1) magnitude bits plus
+ * the sign bit.  10^P - 1 has the bit length of 10^P for P >= 1 (10^P is not
+ * a power of two), so one pass of exact limb arithmetic fills every P. */
+#define OD_MAX_DIGITS 1856
+static int g_bytes_past[OD_MAX_DIGITS + 1];
+static pthread_once_t g_bytes_once = PTHREAD_ONCE_INIT;
+static void fill_bytes_past(void)
+{
+    static uint32_t v[OD_MAXW + 8];
+    int nw = OD_MAXW + 8;
+    memset(v, 0, sizeof v);
+    v[0] = 1;
+    for (int P = 1; P <= OD_MAX_DIGITS; P++) {
+        uint64_t c = 0;
+        for (int i = 0; i < nw; i++) { uint64_t t = (uint64_t)v[i] * 10u + c; v[i] = (uint32_t)t; c = t >> 32; }
+        g_bytes_past[P] = (bitlen_w(v, nw) + 1 + 7) / 8;
+    }
+}
+
 int od_compact_bytes(int P)
 {
     if (P < 0) return 0;
     if (P < 300) return PREC_BYTES[P];
-    /* closed form: bitlen(10^P - 1) computed exactly with limb arithmetic */
-    uint32_t v[OD_MAXW + 8];
-    memset(v, 0, sizeof v);
-    v[0] = 1;
-    int nw = OD_MAXW + 8;
-    for (int k = 0; k < P; k++) {
-        uint64_t c = 0;
-        for (int i = 0; i < nw; i++) { uint64_t t = (uint64_t)v[i] * 10u + c; v[i] = (uint32_t)t; c = t >> 32; }
-    }
-    /* 10^P - 1 */
-    for (int i = 0; i < nw; i++) { if (v[i]) { v[i]--; break; } v[i] = 0xffffffffu; }
-    return (bitlen_w(v, nw) + 1 + 7) / 8;
+    if (P > OD_MAX_DIGITS) return 0;
+    pthread_once(&g_bytes_once, fill_bytes_past);
+    return g_bytes_past[P];
 }
 
 void od_zero(od_dec* d) { memset(d, 0, sizeof *d); }
@@ -77,11 +87,25 @@ static uint32_t mul_small(uint32_t* v, int nw, uint32_t m, uint32_t a)
     return (uint32_t)c;
 }
 
-/* v = floor(v / d); returns remainder */
+static int top_word(const uint32_t* v, int nw);
+
+/* v = floor(v / d); returns remainder (words above the top nonzero one stay 0) */
 static uint32_t div_small(uint32_t* v, int nw, uint32_t d)
 {
     uint64_t r = 0;
-    for (int i = nw - 1; i >= 0; i--) { uint64_t t = (r << 32) | v[i]; v[i] = (uint32_t)(t / d); r = t % d; }
+    for (int i = top_word(v, nw); i >= 0; i--) { uint64_t t = (r << 32) | v[i]; v[i] = (uint32_t)(t / d); r = t % d; }
+    return (uint32_t)r;
+}
+
+/* div_small by the constant 10^9 (the compiler turns it into a multiply) */
+static uint32_t div_1e9(uint32_t* v, int nw)
+{
+    uint64_t r = 0;
+    for (int i = top_word(v, nw); i >= 0; i--) {
+        uint64_t t = (r << 32) | v[i];
+        v[i] = (uint32_t)(t / 1000000000u);
+        r = t % 1000000000u;
+    }
     return (uint32_t)r;
 }
 
@@ -94,7 +118,10 @@ static void mul10n_w(uint32_t* v, int nw, int n, int mode)
 {
     while (n > 0) {
         int k = n > 9 ? 9 : n;
-        if (mul_small(v, nw, POW10_9[k], 0)) flag_overflow(mode);
+        /* words above top + 1 stay zero: a word product carries at most one word up */
+        int used = top_word(v, nw) + 2;
+        if (used > nw) used = nw;
+        if (mul_small(v, used, POW10_9[k], 0) && used == nw) flag_overflow(mode);
         n -= k;
     }
 }
@@ -277,7 +304,7 @@ void od_shl10(od_dec* d, int n, int mode) { if (n > 0) mul10n_w(d->v, NW(mode), 
 void od_shr10(od_dec* d, int n, int mode)
 {
     int nw = NW(mode);
-    while (n > 9) { div_small(d->v, nw, 1000000000u); n -= 9; }
+    while (n > 9) { div_1e9(d->v, nw); n -= 9; }
     if (n > 0) div_small(d->v, nw, POW10_9[n]);
 }
 
@@ -302,12 +329,17 @@ void od_add(od_dec* a, const od_dec* b, int mode)
     int nw = NW(mode);
     od_dec t = *b;
     align_pair(&t, a, mode);
+    /* words above both operands' top words are zero and stay zero, except
+     * for one carry word */
+    int ta = top_word(a->v, nw), tb = top_word(t.v, nw);
+    int used = (ta > tb ? ta : tb) + 2;
+    if (used > nw) used = nw;
     if (a->sign == t.sign) {
-        if (add_w(a->v, a->v, t.v, nw)) flag_overflow(mode);
+        if (add_w(a->v, a->v, t.v, used) && used == nw) flag_overflow(mode);
     } else {
-        int r = cmp_w(a->v, t.v, nw);
-        if (r >= 0) sub_w(a->v, a->v, t.v, nw);
-        else sub_w(a->v, t.v, a->v, nw);
+        int r = cmp_w(a->v, t.v, used);
+        if (r >= 0) sub_w(a->v, a->v, t.v, used);
+        else sub_w(a->v, t.v, a->v, used);
         a->sign = (r > 0 && !b->sign) || (r < 0 && b->sign);
     }
 }
@@ -468,7 +500,7 @@ int od_to_string(const od_dec* d, char* buf, int cap)
 
 /* ---------------- plan ---------------- */
 
-enum { N_COL, N_CONST, N_NEG, N_ADD, N_SUB, N_MUL, N_DIV };
+enum { N_COL, N_CONST, N_NEG, N_ADD, N_SUB, N_MUL, N_DIV, N_CAST };
 
 typedef struct pnode {
     int kind, P, S, col, a, b;
@@ -534,6 +566,13 @@ static const char* parse_node(od_plan* p, const char* s, int* out)
         s = parse_node(p, s, &c);
         if (!s) return NULL;
         p->n[id].kind = N_NEG; p->n[id].a = c; p->n[id].P = p->n[c].P; p->n[id].S = p->n[c].S;
+    } else if (!strcmp(op, "CAST")) {
+        int c;
+        int P = (int)strtol(s, (char**)&s, 10);
+        int S = (int)strtol(s, (char**)&s, 10);
+        s = parse_node(p, s, &c);
+        if (!s) return NULL;
+        p->n[id].kind = N_CAST; p->n[id].a = c; p->n[id].P = P; p->n[id].S = S;
     } else {
         int kind = !strcmp(op, "ADD") ? N_ADD : !strcmp(op, "SUB") ? N_SUB : !strcmp(op, "MUL") ? N_MUL
                  : !strcmp(op, "DIV") ? N_DIV : -1;
@@ -585,6 +624,7 @@ static void eval_node(const evctx* c, int id, int64_t row, od_dec* out)
     }
     case N_CONST: *out = nd->k; return;
     case N_NEG: eval_node(c, nd->a, row, out); od_neg(out); return;
+    case N_CAST: eval_node(c, nd->a, row, out); od_cast(out, nd->P, nd->S, c->mode); return;
     default: break;
     }
     od_dec rb;

@@ -68,6 +68,7 @@ void od_clear_overflow(void);
 /* ---- plan evaluation (the rewritten expression tree, text form) ----
  * plan := '(' OP ' ' P ' ' S ' ' plan ' ' plan ')'  OP in ADD SUB MUL DIV
  *       | '(' 'NEG' ' ' plan ')'
+ *       | '(' 'CAST' ' ' P ' ' S ' ' plan ')'   od_cast to DECIMAL(P,S)
  *       | 'c' IDX ':' P ':' S          compact column operand
  *       | 'k' ('+'|'-') HEX ':' S      constant: big-endian hex magnitude, scale
  */

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libupdec.so")
 
 MODE_AUTO, MODE_FAITHFUL, MODE_EXTENDED = 0, 1, 2
-GEN_RANGE, GEN_DIGITS = 0, 1
+GEN_RANGE, GEN_DIGITS, GEN_BELOW = 0, 1, 2
 IMAGE_BYTES = 133  # nullable_type<Decimal>: flag, sign, prec, frac, error, v[32]
 
 
@@ -36,7 +36,8 @@ class AggDesc(ctypes.Structure):
 class GenDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("precision", ctypes.c_int32), ("digits", ctypes.c_int32),
                 ("neg_permille", ctypes.c_int32), ("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64),
-                ("step", ctypes.c_uint64), ("seed", ctypes.c_uint64), ("column_id", ctypes.c_uint32)]
+                ("step", ctypes.c_uint64), ("seed", ctypes.c_uint64), ("column_id", ctypes.c_uint32),
+                ("bound_words", ctypes.c_int32), ("bound", ctypes.c_uint32 * 16)]
 
 
 def _load():
@@ -348,8 +349,20 @@ def check_compile(src):
     _check(_lib.updec_check_compile(b, len(b)))
 
 
-def gen_desc(kind, precision, seed, column_id, digits=0, lo=0, hi=0, step=1, neg_permille=0):
-    return GenDesc(kind, precision, digits, neg_permille, lo, hi, step, seed, column_id)
+def gen_desc(kind, precision, seed, column_id, digits=0, lo=0, hi=0, step=1, neg_permille=0, bound=0):
+    """`bound` (GEN_BELOW): magnitudes are drawn uniformly from [0, bound)."""
+    d = GenDesc(kind, precision, digits, neg_permille, lo, hi, step, seed, column_id)
+    if kind == GEN_BELOW:
+        words = []
+        while bound:
+            words.append(bound & 0xFFFFFFFF)
+            bound >>= 32
+        if not 1 <= len(words) <= 16:
+            raise ValueError("bound must be a positive integer below 2^512")
+        d.bound_words = len(words)
+        for i, w in enumerate(words):
+            d.bound[i] = w
+    return d
 
 
 def gen_compact_host(desc, rows, row0=0):

@@ -349,6 +349,82 @@ template <int WO, int WI, int N> UPD_DEV Dec<WO> dscale(const Dec<WI>& a)
     return r;
 }
 
+// ---- truncating division by 10^N (operator>>, decimal.cu:2003-2026)
+//
+// floor(a / 10^N) = floor(floor(a / 2^N) / 5^N): a static bit shift, then
+// passes of one-word division by 5^13 (the largest power of five below
+// 2^31) with a compile-time Moller-Granlund reciprocal.  Each pass drops at
+// least 30 bits from the static bound, so later passes run on fewer limbs.
+
+constexpr u32 pow5_u32(int k) { return k == 0 ? 1u : 5u * pow5_u32(k - 1); }
+constexpr int clz_u32(u32 x) { return x >> 31 ? 0 : 1 + clz_u32(x << 1); }
+constexpr int floor_log2_u32(u32 x) { return 31 - clz_u32(x); }
+
+template <u32 D> struct WordDivisor {
+    static constexpr int S = clz_u32(D);
+    static constexpr u32 DN = D << S;
+    static constexpr u32 V = (u32)(~0ull / DN - (1ull << 32));
+};
+
+// (r * 2^32 + w) / D with r < D: returns the quotient word, r becomes the remainder
+template <u32 D> UPD_DEV u32 div_word_const(u32& r, u32 w)
+{
+    using C = WordDivisor<D>;
+    const u32 u1 = C::S ? ((r << C::S) | (w >> ((32 - C::S) & 31))) : r;
+    const u32 u0 = w << C::S;
+    u32 rr;
+    const u32 q = div_2by1(u1, u0, C::DN, C::V, rr);
+    r = rr >> C::S;
+    return q;
+}
+
+// one pass: x = floor(x / D)
+template <int W, u32 D> UPD_DEV void div_pass(Mag<W>& x)
+{
+    u32 r = 0;
+#pragma unroll
+    for (int i = W - 1; i >= 0; i--) x.w[i] = div_word_const<D>(r, x.w[i]);
+}
+
+// floor(a / 5^N), a < 2^(32 W).  The full 5^13 passes run as a loop (one
+// copy of the unrolled limb body keeps generated code, and the hipRTC
+// compile time of long expressions, small); the last partial pass follows.
+template <int W, int N> UPD_DEV Mag<W> div_pow5(Mag<W> x)
+{
+#pragma unroll 1
+    for (int p = 0; p < N / 13; p++) div_pass<W, pow5_u32(13)>(x);
+    if constexpr (N % 13 != 0) div_pass<W, pow5_u32(N % 13)>(x);
+    return x;
+}
+
+// floor(a / 2^N) for a static N
+template <int W, int N> UPD_DEV Mag<W> shr_static(const Mag<W>& a)
+{
+    Mag<W> r;
+    constexpr int WS = N / 32, BS = N % 32;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        const u32 lo = i + WS < W ? a.w[i + WS] : 0u;
+        const u32 hi = i + WS + 1 < W ? a.w[i + WS + 1] : 0u;
+        r.w[i] = BS ? ((lo >> BS) | (hi << ((32 - BS) & 31))) : lo;
+    }
+    return r;
+}
+
+// cast to a smaller scale (decimal.cu:471-533, scale-down branch): the
+// magnitude is floor-divided by 10^N and the sign is kept, so a negative
+// value that truncates to zero stays -0
+template <int WO, int WI, int N> UPD_DEV Dec<WO> dshr10(const Dec<WI>& a)
+{
+    constexpr int BI = 32 * WI - N;
+    constexpr int WM = BI <= 32 ? 1 : (BI + 31) / 32;
+    const Mag<WM> t = resize<WM>(shr_static<WI, N>(a.m));
+    Dec<WO> r;
+    r.m = resize<WO>(div_pow5<WM, N>(t));
+    r.neg = a.neg;
+    return r;
+}
+
 // Add of equally scaled operands with the reference's sign rules
 // (decimal.cu:1284-1323): same sign -> magnitudes add and the sign is kept
 // (so -0 + -0 = -0); different signs -> |larger| - |smaller| with the sign

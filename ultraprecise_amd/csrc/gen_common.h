@@ -28,12 +28,15 @@ UPD_HD uint64_t column_key(uint64_t seed, uint32_t col) { return mix(seed * 0x10
 
 UPD_HD uint64_t draw(uint64_t key, uint64_t row, uint32_t k) { return mix(key ^ mix((row << 8) | k)); }
 
-enum { GEN_RANGE = 0, GEN_DIGITS = 1 };
+enum { GEN_RANGE = 0, GEN_DIGITS = 1, GEN_BELOW = 2 };
+enum { GEN_MAX_BOUND_WORDS = 16 };
 
 struct Desc {
     int32_t kind, precision, digits, neg_permille;
     uint64_t lo, hi, step, seed;
     uint32_t column_id;
+    int32_t bound_words;
+    uint32_t bound[GEN_MAX_BOUND_WORDS];
 };
 
 // Write one compact row of `nbytes` bytes (the reference's layout: little
@@ -50,6 +53,28 @@ UPD_HD void gen_row(const Desc& d, uint64_t key, uint64_t row, uint8_t* out, int
         v *= d.step;
         m[0] = (uint32_t)v;
         m[1] = (uint32_t)(v >> 32);
+    } else if (d.kind == GEN_BELOW) {
+        // floor(R * bound / 2^(32 (n + 2))) for a random R of n + 2 words:
+        // uniform on [0, bound) up to a relative bias below 2^-64
+        const int n = d.bound_words;
+        uint32_t r[GEN_MAX_BOUND_WORDS + 2];
+        for (int i = 0; i < n + 2; i += 2) {
+            uint64_t x = draw(key, row, 1 + (uint32_t)(i / 2));
+            r[i] = (uint32_t)x;
+            if (i + 1 < n + 2) r[i + 1] = (uint32_t)(x >> 32);
+        }
+        uint32_t prod[2 * GEN_MAX_BOUND_WORDS + 2];
+        for (int i = 0; i < 2 * n + 2; i++) prod[i] = 0;
+        for (int i = 0; i < n + 2; i++) {
+            uint64_t c = 0;
+            for (int j = 0; j < n; j++) {
+                uint64_t t = (uint64_t)r[i] * d.bound[j] + prod[i + j] + c;
+                prod[i + j] = (uint32_t)t;
+                c = t >> 32;
+            }
+            prod[i + n] = (uint32_t)c;
+        }
+        for (int i = 0; i < n; i++) m[i] = prod[n + 2 + i];
     } else {
         // exactly `digits` decimal digits, leading digit nonzero: a top
         // chunk of t digits in [10^(t-1), 10^t), then 9-digit chunks

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <queue>
 #include <sstream>
 
@@ -329,6 +330,31 @@ struct Parser {
             size_t st = lx.i;
             while (lx.i < lx.s.size() && (isalnum((unsigned char)lx.s[lx.i]) || lx.s[lx.i] == '_')) lx.i++;
             std::string id = lx.s.substr(st, lx.i - st);
+            if (id == "cast" && lx.peek() == '(') {
+                lx.i++;
+                ExprP arg = expr();
+                if (!arg) return nullptr;
+                long P = 0, S = 0;
+                auto num = [&](long* v) -> bool {
+                    if (!lx.eat(',')) return false;
+                    lx.ws();
+                    size_t b = lx.i;
+                    while (lx.i < lx.s.size() && isdigit((unsigned char)lx.s[lx.i])) lx.i++;
+                    if (b == lx.i) return false;
+                    *v = std::stol(lx.s.substr(b, lx.i - b));
+                    return true;
+                };
+                if (!num(&P) || !num(&S) || !lx.eat(')') || S > P) {
+                    err = "cast expects cast(expr, P, S) with S <= P";
+                    return nullptr;
+                }
+                auto e = std::make_shared<Expr>();
+                e->kind = NK::Cast;
+                e->P = (int)P;
+                e->S = (int)S;
+                e->ch = {arg};
+                return e;
+            }
             for (size_t k = 0; k < names.size(); k++) {
                 if (names[k] == id) {
                     auto e = std::make_shared<Expr>();
@@ -378,6 +404,7 @@ void infer_frontend_types(Expr& e, Mode mode)
         e.S = t.frac;
         return;
     }
+    case NK::Cast: infer_frontend_types(*e.ch[0], mode); return;
     case NK::Calc: break;
     }
     for (auto& c : e.ch) infer_frontend_types(*c, mode);
@@ -416,6 +443,7 @@ struct BT {
     HostDec dval;
     int P = 0, S = 0;
     bool reverse = false;
+    ExprP fn;   // Cast: the cast node, its argument already rewritten
     std::vector<BT*> ch;
 };
 
@@ -427,6 +455,7 @@ struct MT {
     HostDec dval;
     int P = 0, S = 0;
     bool reverse = false;
+    ExprP fn;
     BT* root = nullptr;
     std::vector<MT*> ch;
 };
@@ -434,12 +463,20 @@ struct MT {
 BT* Arena::bt() { bts.emplace_back(new BT()); return bts.back().get(); }
 MT* Arena::mt() { mts.emplace_back(new MT()); return mts.back().get(); }
 
-BT* clone_bt(Arena& A, const Expr& e)
+// A cast is a leaf of the rewrite: its argument is rewritten as a tree of
+// its own first (AlignOptimize rewrites every CALC subtree found below a
+// non-CALC node separately, AriesEngineShell.cpp:646-685).
+BT* clone_bt(Arena& A, const Expr& e, Mode mode, bool* ok)
 {
     BT* b = A.bt();
     b->kind = e.kind; b->op = e.op; b->col = e.col; b->ival = e.ival; b->dval = e.dval;
     b->P = e.P; b->S = e.S; b->reverse = e.reverse;
-    for (auto& c : e.ch) b->ch.push_back(clone_bt(A, *c));
+    if (e.kind == NK::Cast) {
+        b->fn = align_optimize(std::make_shared<Expr>(e), mode);
+        if (!b->fn) *ok = false;
+        return b;
+    }
+    for (auto& c : e.ch) b->ch.push_back(clone_bt(A, *c, mode, ok));
     return b;
 }
 
@@ -447,7 +484,7 @@ MT* mt_from_bt(Arena& A, BT* b)
 {
     MT* m = A.mt();
     m->root = b; m->kind = b->kind; m->op = b->op; m->col = b->col; m->ival = b->ival; m->dval = b->dval;
-    m->P = b->P; m->S = b->S; m->reverse = b->reverse;
+    m->P = b->P; m->S = b->S; m->reverse = b->reverse; m->fn = b->fn;
     return m;
 }
 
@@ -538,7 +575,7 @@ bool is_decimal_calc(const MT* m)
     // only CALC nodes of decimal result type are rewritten; a tree with a
     // division or a decimal operand anywhere below is decimal-typed
     std::function<bool(const BT*)> has_dec = [&](const BT* b) -> bool {
-        if (b->kind == NK::Column || b->kind == NK::Dec) return true;
+        if (b->kind == NK::Column || b->kind == NK::Dec || b->kind == NK::Cast) return true;
         if (b->kind == NK::Calc && b->op == Op::Div) return true;
         for (auto* c : b->ch)
             if (has_dec(c)) return true;
@@ -583,6 +620,7 @@ bool align_opt(Arena& A, MT* top, Mode mode, std::string* err)
                         MT* c = node->ch[index];
                         node->kind = c->kind; node->op = c->op; node->col = c->col; node->ival = c->ival;
                         node->dval = c->dval; node->P = c->P; node->S = c->S; node->reverse = c->reverse;
+                        node->fn = c->fn;
                         node->ch = c->ch;
                         node->root = c->root;
                     } else {
@@ -716,7 +754,7 @@ BT* leaf_from_mt(Arena& A, const MT* m)
 {
     BT* b = A.bt();
     b->kind = m->kind; b->op = m->op; b->col = m->col; b->ival = m->ival; b->dval = m->dval;
-    b->P = m->P; b->S = m->S; b->reverse = m->reverse;
+    b->P = m->P; b->S = m->S; b->reverse = m->reverse; b->fn = m->fn;
     return b;
 }
 
@@ -775,6 +813,11 @@ BT* multi_to_bin(Arena& A, MT* m)
 ExprP to_expr(const BT* b)
 {
     auto e = std::make_shared<Expr>();
+    if (b->kind == NK::Cast) {
+        *e = *b->fn;
+        e->reverse = b->reverse;
+        return e;
+    }
     e->kind = b->kind; e->op = b->op; e->col = b->col; e->ival = b->ival; e->dval = b->dval;
     e->P = b->P; e->S = b->S; e->reverse = b->reverse;
     for (auto* c : b->ch) e->ch.push_back(to_expr(c));
@@ -785,9 +828,18 @@ ExprP to_expr(const BT* b)
 
 ExprP align_optimize(const ExprP& e, Mode mode)
 {
+    if (e && e->kind == NK::Cast) {
+        ExprP arg = align_optimize(e->ch[0], mode);
+        if (!arg) return nullptr;
+        auto c = std::make_shared<Expr>(*e);
+        c->ch = {arg};
+        return c;
+    }
     if (!e || e->kind != NK::Calc) return e;
     Arena A;
-    BT* bt = clone_bt(A, *e);
+    bool ok = true;
+    BT* bt = clone_bt(A, *e, mode, &ok);
+    if (!ok) return nullptr;
     modify_sub(bt);
     MT* mt = mt_from_bt(A, bt);
     bin_to_multi(A, bt, mt);
@@ -809,6 +861,19 @@ bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode,
 {
     Plan p;
     p.mode = mode;
+    // identical subtrees (same operation, type and operand nodes) become one
+    // node, so a repeated term such as x*x in a series is computed once
+    std::map<std::string, int> seen;
+    auto intern = [&](const PNode& n) -> int {
+        std::string key = std::to_string((int)n.kind) + "," + std::to_string(n.P) + "," + std::to_string(n.S) + "," +
+                          std::to_string(n.col) + "," + std::to_string(n.a) + "," + std::to_string(n.b);
+        if (n.kind == PK::Const) key += std::string(",") + (n.k.sign ? "-" : "+") + n.k.hex() + ":" + std::to_string(n.k.frac);
+        auto it = seen.find(key);
+        if (it != seen.end()) return it->second;
+        p.n.push_back(n);
+        seen[key] = (int)p.n.size() - 1;
+        return (int)p.n.size() - 1;
+    };
     std::function<int(const Expr&)> rec = [&](const Expr& e) -> int {
         PNode n{};
         int id = -1;
@@ -829,6 +894,17 @@ bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode,
         } else if (e.kind == NK::Int) {
             *err = "integer operand left in a decimal calculation after rewrite";
             return -1;
+        } else if (e.kind == NK::Cast) {
+            int a = rec(*e.ch[0]);
+            if (a < 0) return -1;
+            const PNode& A = p.n[a];
+            n.kind = PK::Cast;
+            n.a = a;
+            n.P = e.P;
+            n.S = e.S;
+            if (n.S > A.S) n.bits = A.bits + pow10_bits(n.S - A.S);
+            else if (n.S < A.S) n.bits = std::max(1, A.bits - (pow10_bits(A.S - n.S) - 1));
+            else n.bits = A.bits;
         } else {
             int a = rec(*e.ch[0]);
             if (a < 0) return -1;
@@ -865,8 +941,7 @@ bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode,
         }
         if (mode == Mode::Faithful) n.bits = std::min(n.bits, 1024);
         n.W = words_for(n.bits);
-        p.n.push_back(n);
-        id = (int)p.n.size() - 1;
+        id = intern(n);
         if (e.reverse) {
             PNode g{};
             g.kind = PK::Neg;
@@ -875,8 +950,7 @@ bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode,
             g.S = n.S;
             g.bits = n.bits;
             g.W = n.W;
-            p.n.push_back(g);
-            id = (int)p.n.size() - 1;
+            id = intern(g);
         }
         return id;
     };
@@ -900,6 +974,7 @@ std::string Plan::text() const
         case PK::Col: snprintf(buf, sizeof buf, "c%d:%d:%d", x.col, x.P, x.S); return buf;
         case PK::Const: return std::string("k") + (x.k.sign ? "-" : "+") + x.k.hex() + ":" + std::to_string(x.S);
         case PK::Neg: return "(NEG " + rec(x.a) + ")";
+        case PK::Cast: snprintf(buf, sizeof buf, "(CAST %d %d ", x.P, x.S); return buf + rec(x.a) + ")";
         default: break;
         }
         const char* op = x.kind == PK::Add ? "ADD" : x.kind == PK::Sub ? "SUB" : x.kind == PK::Mul ? "MUL" : "DIV";
@@ -913,11 +988,15 @@ bool plan_is_faithful_envelope(const Plan& p, const std::vector<ColumnType>& col
 {
     for (auto& c : cols)
         if (c.P > 299 || c.P < 0 || c.S > 255) return false;
+    // the materialized result needs a row of the 300-entry byte table
+    if (p.outP > 299) return false;
     for (const PNode& x : p.n) {
         if (x.S > 255) return false;
         if (x.kind == PK::Add || x.kind == PK::Sub) {
             if (x.S - p.n[x.a].S > 38 || x.S - p.n[x.b].S > 38) return false;
         }
+        // operator<< reads one row of the 38-row 10^n table (decimal.cu:2028-2058)
+        if (x.kind == PK::Cast && x.S - p.n[x.a].S > 38) return false;
         if (x.kind == PK::Div) {
             int s = p.n[x.b].S + 4;
             if (s > 38) return false;
@@ -1003,9 +1082,60 @@ std::string emit_plan(Emit& E, const Plan& p, const std::vector<ColumnType>& col
             E.os << ind << "const Dec<" << x.W << "> " << v << " = ddiv<" << x.W << ", " << p.n[x.a].W << ", " << p.n[x.b].W
                  << ", " << (p.n[x.b].S + 4) << ", " << (faith ? 1 : 0) << ">(" << var[x.a] << ", " << var[x.b] << ");\n";
             break;
+        case PK::Cast: {
+            const PNode& A = p.n[x.a];
+            E.os << ind << "const Dec<" << x.W << "> " << v << " = ";
+            if (x.S > A.S) E.os << "dscale<" << x.W << ", " << A.W << ", " << (x.S - A.S) << ">(";
+            else if (x.S < A.S) E.os << "dshr10<" << x.W << ", " << A.W << ", " << (A.S - x.S) << ">(";
+            else E.os << "dresize<" << x.W << ">(";
+            E.os << var[x.a] << ");\n";
+            break;
+        }
         }
     }
     return var[p.root];
+}
+
+// Plans with many operations (a long series, say) are evaluated by an
+// out-of-line __noinline__ function: the loop around the row evaluation
+// stays small, which keeps LLVM's loop passes, and with them the hipRTC
+// compile time, linear in the plan length.  Short plans stay inline.
+constexpr int kOutOfLineOps = 16;
+
+bool plan_out_of_line(const Plan& p)
+{
+    int ops = 0;
+    for (const PNode& x : p.n) ops += (x.kind != PK::Col && x.kind != PK::Const) ? 1 : 0;
+    return ops > kOutOfLineOps;
+}
+
+// definition of the out-of-line evaluator `fname` (any args struct whose
+// first members are col[] and colbytes[])
+void emit_plan_function(Emit& E, const Plan& p, const std::vector<ColumnType>& cols, const std::string& fname)
+{
+    const int W = p.n[p.root].W;
+    E.os << "template <class A> __device__ __noinline__ Dec<" << W << "> " << fname << "(const A& a, usize row)\n{\n";
+    std::vector<std::string> loaded(cols.size());
+    std::string rv = emit_plan(E, p, cols, loaded, "  ", "");
+    E.os << "  return " << rv << ";\n}\n";
+}
+
+// evaluate plan p for `row`: inline, or a call of the out-of-line evaluator
+std::string emit_plan_any(Emit& E, const Plan& p, const std::vector<ColumnType>& cols, std::vector<std::string>& loaded,
+                          const std::string& ind, const std::string& prefix, const std::string& fname)
+{
+    if (!plan_out_of_line(p)) return emit_plan(E, p, cols, loaded, ind, prefix);
+    const std::string v = prefix + "r";
+    E.os << ind << "const Dec<" << p.n[p.root].W << "> " << v << " = " << fname << "(a, row);\n";
+    return v;
+}
+
+std::string plan_fn_name(int i) { return "upd_plan" + std::to_string(i); }
+
+void emit_plan_functions(Emit& E, const std::vector<Plan>& plans, const std::vector<ColumnType>& cols)
+{
+    for (int i = 0; i < (int)plans.size(); i++)
+        if (plan_out_of_line(plans[i])) emit_plan_function(E, plans[i], cols, plan_fn_name(i));
 }
 
 const char* kArgsStruct =
@@ -1023,6 +1153,7 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
     const int OB = p.outB;
     E.os << "#include \"dec_device.h\"\nusing namespace updec;\n" << kArgsStruct;
     E.os << "// plan: " << p.text() << "\n";
+    emit_plan_functions(E, {p}, cols);
     E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name << "(ColArgs a)\n{\n";
     E.os << "  __shared__ u32 tile[(" << T << " * " << OB << " + 3) / 4];\n";
     E.os << "  u8* t8 = reinterpret_cast<u8*>(tile);\n";
@@ -1030,7 +1161,7 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
     E.os << "    const usize row = base + threadIdx.x;\n";
     E.os << "    if (row < a.rows) {\n";
     std::vector<std::string> loaded(cols.size());
-    std::string rv = emit_plan(E, p, cols, loaded, "      ", "");
+    std::string rv = emit_plan_any(E, p, cols, loaded, "      ", "", plan_fn_name(0));
     E.os << "      const Mag<" << (OB + 3) / 4 << "> enc = encode_compact<" << OB << ", " << p.n[p.root].W << ">(" << rv << ");\n";
     E.os << "#pragma unroll\n      for (int b = 0; b < " << OB << "; b++) t8[threadIdx.x * " << OB
          << " + b] = (u8)(enc.w[b >> 2] >> (8 * (b & 3)));\n";
@@ -1162,7 +1293,7 @@ static void emit_agg_row(Emit& E, const AggSpec& s, const std::vector<ColumnType
     for (int i = 0; i < (int)s.plans.size(); i++) {
         const Plan& p = s.plans[i];
         const int W = p.n[p.root].W;
-        std::string rv = emit_plan(E, p, cols, loaded, ind, "p" + std::to_string(i) + "_");
+        std::string rv = emit_plan_any(E, p, cols, loaded, ind, "p" + std::to_string(i) + "_", plan_fn_name(i));
         E.os << ind << "slot_add<" << W << ">(&aslot[sbase + " << i << " * AWR], stored<" << p.outB << ", " << W << ">(" << rv
              << "));\n";
     }
@@ -1209,6 +1340,7 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     *block_out = c.T;
     const char* fops[] = {"<=", "<", ">=", ">", "==", "!="};
     emit_agg_prelude(E, s, c, MAXS);
+    emit_plan_functions(E, s.plans, cols);
     E.os << "// accumulation: one pass, LDS key hash + 64-bit atomic slots\n";
     E.os << "#define MAXS " << MAXS << "\n#define HT 64\n#define EW " << EW << "\n";
     // key -> slot: open addressing over HT entries {key+1 << 8 | slot};
@@ -1332,6 +1464,7 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
     const int PW = NP * NG * AW2 + 2 * NG;  // partial words per block
     const char* fops[] = {"<=", "<", ">=", ">", "==", "!="};
     emit_agg_prelude(E, s, c, NG);
+    emit_plan_functions(E, s.plans, cols);
     E.os << "// accumulation: " << (reg ? "registers" : "LDS 64-bit atomic slots") << "\n";
     E.os << "#define NG " << NG << "\n#define PW " << PW << "\n";
     // a lane accumulates at most 2^14 rows (kAggRowsPerLaneLog2)
@@ -1357,7 +1490,7 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
         for (int i = 0; i < NP; i++) {
             const Plan& p = s.plans[i];
             const int W = p.n[p.root].W;
-            std::string rv = emit_plan(E, p, cols, loaded, "    ", "p" + std::to_string(i) + "_");
+            std::string rv = emit_plan_any(E, p, cols, loaded, "    ", "p" + std::to_string(i) + "_", plan_fn_name(i));
             E.os << "    acc_add<AW1, " << W << ">(acc" << i << ", stored<" << p.outB << ", " << W << ">(" << rv << "));\n";
         }
         E.os << "    cnt += 1u;\n  }\n";

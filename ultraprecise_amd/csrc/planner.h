@@ -45,7 +45,10 @@ struct ColumnType {
 
 // Expression tree, mirroring the reference's BinaryTree/MultipleTree
 // (AriesEngineShell.cpp:19-95) closely enough to apply the same rewrite.
-enum class NK { Column, Int, Dec, Calc };
+// Cast is cast(expr, P, S): Decimal(P, S).cast(value) (decimal.cu:471-533),
+// multiply up or truncate down to scale S.  It is a function node: the
+// rewrite treats it as an operand and rewrites its argument on its own.
+enum class NK { Column, Int, Dec, Calc, Cast };
 enum class Op { Add = 0, Sub = 1, Mul = 2, Div = 3, Mod = 4 };
 
 struct Expr {
@@ -75,7 +78,7 @@ void infer_frontend_types(Expr& e, Mode mode);
 ExprP align_optimize(const ExprP& e, Mode mode);
 
 // Typed evaluation plan
-enum class PK { Col, Const, Neg, Add, Sub, Mul, Div };
+enum class PK { Col, Const, Neg, Add, Sub, Mul, Div, Cast };
 struct PNode {
     PK kind;
     int P = 0, S = 0;

@@ -1044,7 +1044,14 @@ int updec_gen_compact(const updec_gen_desc* g, int64_t row0, int64_t rows, uint8
     int nb = compact_bytes(g->precision);
     if (nb <= 0 || nb > 128) return fail(UPDEC_EINVAL, "precision out of generator range");
     if (g->kind == UPDEC_GEN_DIGITS && (g->digits < 1 || g->digits > 460)) return fail(UPDEC_EINVAL, "digits out of range");
-    updec_gen::Desc d{g->kind, g->precision, g->digits, g->neg_permille, g->lo, g->hi, g->step, g->seed, g->column_id};
+    if (g->kind == UPDEC_GEN_BELOW && (g->bound_words < 1 || g->bound_words > updec_gen::GEN_MAX_BOUND_WORDS))
+        return fail(UPDEC_EINVAL, "bound_words out of range");
+    if (g->kind < UPDEC_GEN_RANGE || g->kind > UPDEC_GEN_BELOW) return fail(UPDEC_EINVAL, "unknown generator kind");
+    updec_gen::Desc d{};
+    d.kind = g->kind; d.precision = g->precision; d.digits = g->digits; d.neg_permille = g->neg_permille;
+    d.lo = g->lo; d.hi = g->hi; d.step = g->step; d.seed = g->seed; d.column_id = g->column_id;
+    d.bound_words = g->kind == UPDEC_GEN_BELOW ? g->bound_words : 0;
+    for (int i = 0; i < d.bound_words; i++) d.bound[i] = g->bound[i];
     uint64_t key = updec_gen::column_key(g->seed, g->column_id);
     if (on_device) {
         if (rows == 0) return UPDEC_OK;

@@ -3,7 +3,7 @@
 Each column is drawn by the library's counter-based generator keyed by
 (seed, column id), so host and device produce identical bytes.
 """
-GEN_RANGE, GEN_DIGITS = 0, 1
+GEN_RANGE, GEN_DIGITS, GEN_BELOW = 0, 1, 2
 
 # DECIMAL(15,2) columns of the plain-method workload (1,000,000 rows, seed 0)
 # and of the grouped aggregation (60,000,000 rows, seed 2): magnitudes in
@@ -41,6 +41,55 @@ CONFIG2 = {
     "precisions": [(18, 4), (38, 10), (76, 20), (150, 50), (300, 100)],
     "expr": "(a+b)*c - a/d",
 }
+
+# division-heavy high-precision series: 1,000,000 rows, seed 3, x DECIMAL(300,100)
+# uniform in [-pi, pi]; sin(x) by a 25-term alternating Taylor series in Horner
+# form with the factorial reciprocals as scale-100 constants and every product
+# truncated back to scale 100, then SUM over all rows
+CONFIG4 = {"rows": 1_000_000, "seed": 3, "P": 300, "S": 100, "terms": 25}
+
+
+def pi_scaled(S):
+    """floor(pi * 10^S), from Machin's formula pi = 16 atan(1/5) - 4 atan(1/239)
+    in fixed point with 20 guard digits."""
+    one = 10 ** (S + 20)
+
+    def atan_inv(n):
+        total, term, k, sign = 0, one // n, 1, 1
+        while term:
+            total += sign * (term // k)
+            term //= n * n
+            k += 2
+            sign = -sign
+        return total
+
+    return (16 * atan_inv(5) - 4 * atan_inv(239)) // 10 ** 20
+
+
+def sin_constants(terms, S):
+    """1/(2k+1)! truncated to S fractional digits, as decimal literals"""
+    out, f = [], 1
+    for k in range(terms):
+        if k:
+            f *= (2 * k) * (2 * k + 1)
+        m = 10 ** S // f
+        out.append(f"{m // 10 ** S}.{m % 10 ** S:0{S}d}")
+    return out
+
+
+def sin_expr(col="x", terms=25, P=300, S=100):
+    """Horner form of the series: s = c_k - cast(y * s), y = cast(x*x), result cast(x * s)"""
+    c = sin_constants(terms, S)
+    y = f"cast({col}*{col}, {P}, {S})"
+    s = c[terms - 1]
+    for k in range(terms - 2, -1, -1):
+        s = f"{c[k]} - cast({y}*({s}), {P}, {S})"
+    return f"cast({col}*({s}), {P}, {S})"
+
+
+def sin_desc(U, seed, col_id=0, P=300, S=100):
+    return U.gen_desc(GEN_BELOW, P, seed, col_id, bound=pi_scaled(S) + 1, neg_permille=500)
+
 
 # column ids of the generator streams
 COL_FLAG1, COL_FLAG2, COL_SHIP = 100, 101, 102
