@@ -171,6 +171,46 @@ void updec_module_free(updec_module_t m);
  * returns UPDEC_ECOMPILE with the compiler log on failure */
 int updec_check_compile(const char* src, size_t len);
 
+/* ---------------- interpreter and batches ----------------
+ * The same plans as the expression path, run by an interpreter compiled once
+ * with the library (no per-query hipRTC compile): the alternative to
+ * AriesDynamicKernelManager::CompileKernels / CallKernel
+ * (src/CudaAcc/DynamicKernel.h:50-99) for a stream of distinct queries, with
+ * the SUM / COUNT of AEExprAggFunctionNode::RunKernelFunction
+ * (src/AriesEngine/AriesExprCalcNode.cpp:1623-1700) over all rows.
+ *
+ * updec_vm_compile: parse, rewrite and type `text` exactly as
+ * updec_expr_compile does, then build a register program.  Every register
+ * holds W limbs, W in {2, 4, 8, 16, 32} the smallest that holds every
+ * intermediate exactly; UPDEC_EUNSUPPORTED when none does.
+ *
+ * A batch collects queries (program, device columns, rows, output image and
+ * count) and runs them with one set of launches: persistent kernels per
+ * register width take row ranges from a work counter; with use_graph the
+ * launches are one HIP graph (captured at the first launch).  Results are
+ * written to each query's 133-byte SUM image and int64 COUNT.  A batch is
+ * not thread-safe across launches on different streams: use one stream per
+ * batch object.  No counterpart in the reference (its queries launch one
+ * kernel each, synchronously). */
+typedef struct updec_vm_s* updec_vm_t;
+typedef struct updec_batch_s* updec_batch_t;
+int updec_vm_compile(const char* text, const updec_column_desc* cols, int ncols, int mode, updec_vm_t* out);
+int updec_vm_info(updec_vm_t v, int32_t* width, int32_t* nins, int32_t* nregs, int32_t* precision, int32_t* scale,
+                  int32_t* faithful);
+int updec_vm_listing(updec_vm_t v, char* buf, size_t cap, size_t* needed);
+void updec_vm_free(updec_vm_t v);
+int updec_batch_create(int32_t max_queries, int32_t max_items, updec_batch_t* out);
+/* per width class (0..4 for W = 2..32): rows per work item and resident
+ * blocks per compute unit of the persistent kernel */
+int updec_batch_tuning(updec_batch_t b, int32_t width_class, int64_t rows_per_item, int32_t blocks_per_cu);
+/* *index_out (optional): the query's position in the pending batch */
+int updec_batch_add(updec_batch_t b, updec_vm_t v, const uint8_t* const* d_cols, int64_t rows, uint8_t* d_image,
+                    int64_t* d_count, int32_t* index_out);
+/* number of queries added since the last launch */
+int updec_batch_pending(updec_batch_t b, int32_t* n);
+int updec_batch_launch(updec_batch_t b, int use_graph, updec_stream_t stream);
+void updec_batch_free(updec_batch_t b);
+
 /* ---------------- seeded synthetic columns ----------------
  * Counter-based generator (splitmix64 keyed by seed, column id and row), so
  * any row range can be produced independently and identically on host and

@@ -90,6 +90,17 @@ def _load():
         "updec_gen_compact": (ctypes.c_int, [ctypes.POINTER(GenDesc), I64, I64, P, ctypes.c_int, P]),
         "updec_gen_int32": (ctypes.c_int, [U64, ctypes.c_uint32, I32, I32, I64, I64, P, ctypes.c_int, P]),
         "updec_gen_byte": (ctypes.c_int, [U64, ctypes.c_uint32, P, I32, I64, I64, P, ctypes.c_int, P]),
+        "updec_vm_compile": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ColumnDesc), ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(P)]),
+        "updec_vm_info": (ctypes.c_int, [P] + [ctypes.POINTER(I32)] * 6),
+        "updec_vm_listing": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
+        "updec_vm_free": (None, [P]),
+        "updec_batch_create": (ctypes.c_int, [I32, I32, ctypes.POINTER(P)]),
+        "updec_batch_tuning": (ctypes.c_int, [P, I32, I64, I32]),
+        "updec_batch_add": (ctypes.c_int, [P, P, ctypes.POINTER(P), I64, P, P, ctypes.POINTER(I32)]),
+        "updec_batch_pending": (ctypes.c_int, [P, ctypes.POINTER(I32)]),
+        "updec_batch_launch": (ctypes.c_int, [P, ctypes.c_int, P]),
+        "updec_batch_free": (None, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -109,7 +120,9 @@ EXPORTED = ["updec_last_error", "updec_version", "updec_init", "updec_compact_by
             "updec_expr_compile", "updec_expr_info", "updec_expr_plan", "updec_expr_source", "updec_expr_calc",
             "updec_expr_free", "updec_agg_compile", "updec_agg_run", "updec_agg_last_kernel_ms", "updec_agg_source",
             "updec_agg_free", "updec_div_int64", "updec_segsum_compact", "updec_compile", "updec_launch",
-            "updec_module_free", "updec_check_compile", "updec_gen_compact", "updec_gen_int32", "updec_gen_byte"]
+            "updec_module_free", "updec_check_compile", "updec_gen_compact", "updec_gen_int32", "updec_gen_byte",
+            "updec_vm_compile", "updec_vm_info", "updec_vm_listing", "updec_vm_free", "updec_batch_create",
+            "updec_batch_tuning", "updec_batch_add", "updec_batch_pending", "updec_batch_launch", "updec_batch_free"]
 
 
 def lib():
@@ -408,3 +421,77 @@ def gen_byte_device(seed, column_id, choices, rows, row0=0, stream=None):
     out = DeviceBuffer(max(4, rows))
     _check(_lib.updec_gen_byte(seed, column_id, ch, len(choices), row0, rows, _ptr(out), 1, _stream(stream)))
     return out
+
+
+VM_CLASS_WIDTHS = (2, 4, 8, 16, 32)
+
+
+class VmProgram:
+    """An expression compiled for the precompiled interpreter (no hipRTC):
+    the same parse / rewrite / typing as Expr, run as a register program."""
+
+    def __init__(self, text, cols, mode=MODE_AUTO):
+        self.text = text
+        self.cols = [tuple(c) for c in cols]
+        arr, keep = _col_array(self.cols)
+        h = ctypes.c_void_p()
+        _check(_lib.updec_vm_compile(text.encode(), arr, len(self.cols), mode, ctypes.byref(h)))
+        self._h = h
+        v = [ctypes.c_int32() for _ in range(6)]
+        _check(_lib.updec_vm_info(self._h, *[ctypes.byref(x) for x in v]))
+        self.width, self.nins, self.nregs, self.precision, self.scale, f = [x.value for x in v]
+        self.faithful = bool(f)
+        self.width_class = VM_CLASS_WIDTHS.index(self.width)
+
+    @property
+    def listing(self):
+        n = ctypes.c_size_t()
+        _check(_lib.updec_vm_listing(self._h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(_lib.updec_vm_listing(self._h, buf, n.value, ctypes.byref(n)))
+        return buf.value.decode()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.updec_vm_free(self._h)
+            self._h = None
+
+
+class Batch:
+    """Queries collected for one set of interpreter launches.  add() stages
+    a query; launch() runs every staged query (as one HIP graph when
+    graph=True) and writes each query's SUM image and COUNT."""
+
+    def __init__(self, max_queries=1024, max_items=65536):
+        h = ctypes.c_void_p()
+        _check(_lib.updec_batch_create(max_queries, max_items, ctypes.byref(h)))
+        self._h = h
+        self._keep = []
+
+    def tune(self, width_class, rows_per_item, blocks_per_cu):
+        _check(_lib.updec_batch_tuning(self._h, width_class, rows_per_item, blocks_per_cu))
+
+    def add(self, prog, columns, rows, image, count, image_offset=0, count_offset=0):
+        """columns: DeviceBuffers in descriptor order; image: 133 bytes at
+        image_offset; count: one int64 at count_offset."""
+        ptrs = (ctypes.c_void_p * max(1, len(columns)))(*[c.ptr for c in columns])
+        idx = ctypes.c_int32()
+        _check(_lib.updec_batch_add(self._h, prog._h, ptrs, rows, ctypes.c_void_p(image.ptr + image_offset),
+                                    ctypes.c_void_p(count.ptr + count_offset), ctypes.byref(idx)))
+        self._keep.append(prog)
+        return idx.value
+
+    @property
+    def pending(self):
+        n = ctypes.c_int32()
+        _check(_lib.updec_batch_pending(self._h, ctypes.byref(n)))
+        return n.value
+
+    def launch(self, graph=True, stream=None):
+        _check(_lib.updec_batch_launch(self._h, 1 if graph else 0, _stream(stream)))
+        self._keep = []
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.updec_batch_free(self._h)
+            self._h = None

@@ -425,6 +425,204 @@ template <int WO, int WI, int N> UPD_DEV Dec<WO> dshr10(const Dec<WI>& a)
     return r;
 }
 
+// ---- run-time scale variants (the precompiled interpreter, where the
+// shift counts are data).  Same results as the compile-time forms above;
+// every value is held at one register width W.
+
+// 10^n for n <= 9
+UPD_DEV u32 pow10_word(u32 n)
+{
+    u32 p = 1u;
+    for (u32 k = 0; k < n; k++) p *= 10u;
+    return p;
+}
+
+// a = a * m mod 2^(32 W)
+template <int W> UPD_DEV void mul_word(Mag<W>& a, u32 m)
+{
+    u64 c = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        u64 t = (u64)a.w[i] * m + c;
+        a.w[i] = (u32)t;
+        c = t >> 32;
+    }
+}
+
+// a * 10^n mod 2^(32 W), n at run time (chunks of 10^9: the product mod
+// 2^(32 W) does not depend on how 10^n is split)
+template <int W> UPD_DEV Mag<W> mul_p10_rt(Mag<W> a, u32 n)
+{
+    for (; n >= 9u; n -= 9u) mul_word<W>(a, 1000000000u);
+    if (n) mul_word<W>(a, pow10_word(n));
+    return a;
+}
+
+// x = floor(x / 10^r), 1 <= r <= 8, with a run-time divisor word
+template <int W> UPD_DEV void div_small_rt(Mag<W>& x, u32 r)
+{
+    const u32 d = pow10_word(r);
+    const int s = __builtin_clz(d);
+    const u32 dn = d << s;
+    const u32 v = recip_word(dn);
+    u32 rem = 0;
+#pragma unroll
+    for (int i = W - 1; i >= 0; i--) {
+        const u32 u1 = (rem << s) | (x.w[i] >> ((32 - s) & 31));
+        const u32 u0 = x.w[i] << s;
+        u32 rr;
+        x.w[i] = div_2by1(u1, u0, dn, v, rr);
+        rem = rr >> s;
+    }
+}
+
+// floor(a / 10^n), n at run time
+template <int W> UPD_DEV Mag<W> div_p10_rt(Mag<W> a, u32 n)
+{
+    for (; n >= 9u; n -= 9u) div_pass<W, 1000000000u>(a);
+    if (n) div_small_rt<W>(a, n);
+    return a;
+}
+
+// DivOrMod at a run-time S (see ddiv).  The caller guarantees that
+// |a| * 10^S fits in W limbs.
+template <int W> UPD_DEV Dec<W> ddiv_rt(const Dec<W>& a, const Dec<W>& b, u32 S, bool faith)
+{
+    const bool azero = is_zero(a.m);
+    const bool bzero = is_zero(b.m);
+    Mag<W> bs = b.m;
+    if (bzero) bs.w[0] = 1u;
+    const Mag<W> X = mul_p10_rt<W>(a.m, S);
+    Mag<W + 1> Q;
+    Mag<W> R;
+    divmod<W, W>(X, bs, Q, R);
+    Mag<W> T;
+    sub_to<W>(T, bs, R);
+    u32 rnd = cmp<W>(R, T) >= 0 ? 1u : 0u;
+    if (faith) {
+        const u32 bla = bitlen(a.m), blb = bitlen(bs);
+        const bool path_i = (bla + (u32)est_shift_bits((int)S) - 1u <= 64u) && blb <= 64u;
+        bool top0 = true;
+#pragma unroll
+        for (int i = 1; i < W; i++) top0 = top0 && bs.w[i] == 0u;
+        const bool path_ii = !path_i && top0;
+        const bool path_iii_one = !path_i && !path_ii && (bitlen(X) + 1u == blb);
+        if (path_ii) rnd = ((u32)(R.w[0] * 2u) >= bs.w[0]) ? 1u : 0u;
+        if (path_iii_one) {
+            Q = mzero<W + 1>();
+            Q.w[0] = 1u;
+            rnd = 0u;
+        }
+        if (path_i) {
+            Mag<2> dvs = mul_p10_rt<2>(resize<2>(a.m), S);
+            Mag<2> dvt = resize<2>(bs);
+            Mag<3> q3;
+            Mag<2> r2;
+            divmod<2, 2>(dvs, dvt, q3, r2);
+            Mag<2> r2d;
+            r2d.w[0] = r2.w[0] << 1;
+            r2d.w[1] = (r2.w[1] << 1) | (r2.w[0] >> 31);
+            rnd = cmp<2>(r2d, dvt) >= 0 ? 1u : 0u;
+            Q = resize<W + 1>(resize<2>(q3));
+        }
+    }
+    u64 c = rnd;
+#pragma unroll
+    for (int i = 0; i < W + 1; i++) {
+        u64 s = (u64)Q.w[i] + c;
+        Q.w[i] = (u32)s;
+        c = s >> 32;
+    }
+    Dec<W> r;
+    if (azero) {
+        r.m = mzero<W>();
+        r.neg = 0u;
+    } else if (bzero) {
+        r.m = a.m;
+        r.neg = a.neg ^ b.neg;
+    } else {
+        r.m = resize<W>(Q);
+        r.neg = a.neg ^ b.neg;
+    }
+    return r;
+}
+
+// Decode one compact row of B bytes (B at run time) into W limbs.  The
+// caller guarantees the magnitude fits W limbs; when B > 4 W only the low
+// 4 W bytes and the sign byte are read.
+template <int W> UPD_DEV Dec<W> load_compact_rt(const u8* __restrict__ col, usize row, u32 B, usize colbytes)
+{
+    const usize off = row * (usize)B;
+    const u32 nb = B < 4u * W ? B : 4u * W;
+    const usize d0 = off >> 2;
+    const u32 sh = (u32)(off & 3u);
+    const u32 nd = (sh + nb + 3u) >> 2;   // dwords covering the magnitude bytes
+    u32 dw[W + 2];
+    const bool fast = (d0 + nd) * 4u <= colbytes;
+    const u32* p = reinterpret_cast<const u32*>(col) + d0;
+#pragma unroll
+    for (int i = 0; i < W + 1; i++) {
+        u32 x = 0;
+        if ((u32)i < nd) {
+            if (fast) {
+                x = __builtin_nontemporal_load(p + i);
+            } else {
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const usize o = (d0 + (usize)i) * 4u + (usize)b;
+                    x |= (o < colbytes ? (u32)col[o] : 0u) << (8 * b);
+                }
+            }
+        }
+        dw[i] = x;
+    }
+    dw[W + 1] = 0u;
+    Dec<W> r;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        u32 x = __builtin_amdgcn_alignbyte(dw[i + 1], dw[i], sh);
+        // keep bytes below nb only
+        const u32 lo = 4u * (u32)i;
+        if (lo >= nb) x = 0u;
+        else if (lo + 4u > nb) x &= (1u << (8u * (nb - lo))) - 1u;
+        r.m.w[i] = x;
+    }
+    // sign: bit 7 of byte B - 1
+    if (B <= 4u * W) {
+        const u32 sb = 8u * (B - 1u);
+#pragma unroll
+        for (int i = 0; i < W; i++) {
+            if ((u32)i == (sb >> 5)) {
+                r.neg = (r.m.w[i] >> (sb & 31u) >> 7) & 1u;
+                r.m.w[i] &= ~(0x80u << (sb & 31u));
+            }
+        }
+    } else {
+        r.neg = (u32)col[off + B - 1u] >> 7;
+    }
+    return r;
+}
+
+// the compact round trip of a SUM input (see `stored` in the generated
+// kernels): keep 8 B - 1 magnitude bits, OR the bit at 8 B - 1 into the sign
+template <int W> UPD_DEV Dec<W> stored_rt(const Dec<W>& v, u32 B)
+{
+    Dec<W> r = v;
+    const u32 top = 8u * B - 1u;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        const u32 lo = 32u * (u32)i;
+        if (lo > top) {
+            r.m.w[i] = 0u;
+        } else if (lo + 32u > top) {
+            const u32 bit = top - lo;
+            r.neg |= (v.m.w[i] >> bit) & 1u;
+            r.m.w[i] &= bit == 0u ? 0u : (0xffffffffu >> (32u - bit));
+        }
+    }
+    return r;
+}
+
 // Add of equally scaled operands with the reference's sign rules
 // (decimal.cu:1284-1323): same sign -> magnitudes add and the sign is kept
 // (so -0 + -0 = -0); different signs -> |larger| - |smaller| with the sign

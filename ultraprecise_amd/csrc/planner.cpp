@@ -8,6 +8,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <queue>
 #include <sstream>
 
@@ -31,18 +32,21 @@ static const uint8_t kPrecBytes[300] = {
     105, 105, 105, 106, 106, 107, 107, 108, 108, 108, 109, 109, 110, 110, 110, 111, 111, 112, 112, 113, 113, 113, 114, 114, 115,
     115, 115, 116, 116, 117, 117, 118, 118, 118, 119, 119, 120, 120, 120, 121, 121, 122, 122, 123, 123, 123, 124, 124, 125, 125};
 
+// bitlen(10^n), from a table grown on demand (exact limb arithmetic)
 static int pow10_bits(int n)
 {
     if (n <= 0) return n == 0 ? 1 : 0;
-    std::vector<uint32_t> v(n / 9 + 2, 0);
-    v[0] = 1;
-    for (int k = 0; k < n; k++) {
+    static std::mutex mu;
+    static std::vector<int> table{1};
+    static std::vector<uint32_t> v{1u};   // 10^(table.size() - 1)
+    std::lock_guard<std::mutex> g(mu);
+    while ((int)table.size() <= n) {
         uint64_t c = 0;
         for (auto& w : v) { uint64_t t = (uint64_t)w * 10u + c; w = (uint32_t)t; c = t >> 32; }
+        if (c) v.push_back((uint32_t)c);
+        table.push_back(32 * ((int)v.size() - 1) + (32 - __builtin_clz(v.back())));
     }
-    int t = (int)v.size() - 1;
-    while (t > 0 && !v[t]) t--;
-    return 32 * t + (32 - __builtin_clz(v[t]));
+    return table[n];
 }
 
 int compact_bytes(int P)
@@ -857,6 +861,28 @@ ExprP align_optimize(const ExprP& e, Mode mode)
 
 static int words_for(int bits) { return std::max(1, (bits + 31) / 32); }
 
+// Magnitude bound of an operation node from its operands' bounds ba, bb
+// (value < 2^bits).  Leaves carry their own bound.
+static int op_bits(const PNode& n, const PNode& A, const PNode* B, int ba, int bb)
+{
+    switch (n.kind) {
+    case PK::Neg: return ba;
+    case PK::Add:
+    case PK::Sub: {
+        int xa = ba + (n.S > A.S ? pow10_bits(n.S - A.S) : 0);
+        int xb = bb + (n.S > B->S ? pow10_bits(n.S - B->S) : 0);
+        return std::max(xa, xb) + 1;
+    }
+    case PK::Mul: return ba + bb;
+    case PK::Div: return std::max(64, ba + pow10_bits(B->S + 4) + 1);
+    case PK::Cast:
+        if (n.S > A.S) return ba + pow10_bits(n.S - A.S);
+        if (n.S < A.S) return std::max(1, ba - (pow10_bits(A.S - n.S) - 1));
+        return ba;
+    default: return n.bits;
+    }
+}
+
 bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode, Plan* out, std::string* err)
 {
     Plan p;
@@ -897,14 +923,10 @@ bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode,
         } else if (e.kind == NK::Cast) {
             int a = rec(*e.ch[0]);
             if (a < 0) return -1;
-            const PNode& A = p.n[a];
             n.kind = PK::Cast;
             n.a = a;
             n.P = e.P;
             n.S = e.S;
-            if (n.S > A.S) n.bits = A.bits + pow10_bits(n.S - A.S);
-            else if (n.S < A.S) n.bits = std::max(1, A.bits - (pow10_bits(A.S - n.S) - 1));
-            else n.bits = A.bits;
         } else {
             int a = rec(*e.ch[0]);
             if (a < 0) return -1;
@@ -917,27 +939,29 @@ bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode,
             n.P = e.P;
             switch (e.op) {
             case Op::Add:
-            case Op::Sub: {
+            case Op::Sub:
                 n.kind = e.op == Op::Add ? PK::Add : PK::Sub;
                 n.S = std::max(A.S, B.S);
-                int ba = A.bits + (n.S > A.S ? pow10_bits(n.S - A.S) : 0);
-                int bb = B.bits + (n.S > B.S ? pow10_bits(n.S - B.S) : 0);
-                n.bits = std::max(ba, bb) + 1;
                 break;
-            }
             case Op::Mul:
                 n.kind = PK::Mul;
                 n.S = A.S + B.S;
-                n.bits = A.bits + B.bits;
                 break;
             case Op::Div:
                 n.kind = PK::Div;
                 n.S = A.S + 4;
-                n.bits = std::max(64, A.bits + pow10_bits(B.S + 4) + 1);
                 break;
             case Op::Mod: *err = "MOD is not implemented on the HIP path"; return -1;
             }
             if (n.S != e.S) { *err = "internal: runtime scale differs from node type"; return -1; }
+        }
+        if (n.a >= 0) {
+            const PNode& A = p.n[n.a];
+            const PNode* B = n.b >= 0 ? &p.n[n.b] : nullptr;
+            n.bits = op_bits(n, A, B, A.bits, B ? B->bits : 0);
+            n.ubits = op_bits(n, A, B, A.ubits, B ? B->ubits : 0);
+        } else {
+            n.ubits = n.bits;
         }
         if (mode == Mode::Faithful) n.bits = std::min(n.bits, 1024);
         n.W = words_for(n.bits);
@@ -949,6 +973,7 @@ bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode,
             g.P = n.P;
             g.S = n.S;
             g.bits = n.bits;
+            g.ubits = n.ubits;
             g.W = n.W;
             id = intern(g);
         }
@@ -1008,6 +1033,32 @@ bool plan_is_faithful_envelope(const Plan& p, const std::vector<ColumnType>& col
         }
     }
     return true;
+}
+
+bool plan_expression(const std::string& text, const std::vector<std::string>& names,
+                     const std::vector<ColumnType>& types, int mode, Plan* out, std::string* err)
+{
+    for (int attempt = 0; attempt < 2; attempt++) {
+        Mode m;
+        if (mode == 1) m = Mode::Faithful;
+        else if (mode == 2) m = Mode::Extended;
+        else m = attempt == 0 ? Mode::Faithful : Mode::Extended;
+        ExprP e = parse_expr(text, names, types, m, err);
+        if (!e) return false;
+        ExprP rw = align_optimize(e, m);
+        if (!rw) { *err = "expression rewrites to a constant or integer"; return false; }
+        Plan p;
+        if (!build_plan(rw, types, m, &p, err)) return false;
+        bool inside = plan_is_faithful_envelope(p, types);
+        if (mode == 0 && m == Mode::Faithful && !inside) continue;
+        if (mode == 1 && !inside) {
+            *err = "expression leaves the reference's defined envelope; use extended mode";
+            return false;
+        }
+        *out = std::move(p);
+        return true;
+    }
+    return false;
 }
 
 // ------------------------------------------------------------------ code generation
@@ -1096,17 +1147,38 @@ std::string emit_plan(Emit& E, const Plan& p, const std::vector<ColumnType>& col
     return var[p.root];
 }
 
-// Plans with many operations (a long series, say) are evaluated by an
-// out-of-line __noinline__ function: the loop around the row evaluation
-// stays small, which keeps LLVM's loop passes, and with them the hipRTC
-// compile time, linear in the plan length.  Short plans stay inline.
+// Plans with many operations (a long series) or wide limb arithmetic (a
+// division of 500-bit operands) are evaluated by an out-of-line
+// __noinline__ function: the loop around the row evaluation stays small,
+// which keeps LLVM's loop passes, and with them the hipRTC compile time,
+// close to linear in the plan size (inline in the aggregation loop, a
+// 16-limb division alone took more than 15 minutes to compile).  Short,
+// narrow plans stay inline.
 constexpr int kOutOfLineOps = 16;
+constexpr int kOutOfLineCost = 600;   // limb products, roughly
+
+int plan_cost(const Plan& p)
+{
+    int cost = 0;
+    for (const PNode& x : p.n) {
+        const int wa = x.a >= 0 ? p.n[x.a].W : 0, wb = x.b >= 0 ? p.n[x.b].W : 0;
+        switch (x.kind) {
+        case PK::Mul: cost += wa * wb; break;
+        case PK::Div: cost += 4 * (wa + words_for(pow10_bits(p.n[x.b].S + 4)) + 1) * wb; break;
+        case PK::Add:
+        case PK::Sub: cost += x.W; break;
+        case PK::Cast: cost += wa * (1 + std::abs(x.S - p.n[x.a].S) / 9); break;
+        default: break;
+        }
+    }
+    return cost;
+}
 
 bool plan_out_of_line(const Plan& p)
 {
     int ops = 0;
     for (const PNode& x : p.n) ops += (x.kind != PK::Col && x.kind != PK::Const) ? 1 : 0;
-    return ops > kOutOfLineOps;
+    return ops > kOutOfLineOps || plan_cost(p) > kOutOfLineCost;
 }
 
 // definition of the out-of-line evaluator `fname` (any args struct whose
@@ -1551,6 +1623,195 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
     E.os << kBlockReduceCount;
     E.os << "    if (tid == 0) { u64 t = 0; for (int wv = 0; wv < 4; wv++) t += (u64)red[2 * wv] | ((u64)red[2 * wv + 1] << 32); counts[gg] = t; }\n  }\n}\n";
     return E.os.str();
+}
+
+// ------------------------------------------------------------------ interpreter programs
+
+namespace {
+
+struct VmBuilder {
+    const Plan& p;
+    const std::vector<ColumnType>& cols;
+    VmProgram& out;
+    std::string err;
+    bool used[updec_vm::kMaxRegs] = {};
+    std::map<int, int> const_index;   // plan node -> constant slot
+    std::vector<int> need_memo;
+
+    VmBuilder(const Plan& pl, const std::vector<ColumnType>& c, VmProgram& o) : p(pl), cols(c), out(o) {}
+
+    // registers needed to evaluate node `id` into a register (leaves: 0,
+    // they are read in place)
+    int need(int id)
+    {
+        if (need_memo[id] >= 0) return need_memo[id];
+        const PNode& x = p.n[id];
+        int r = 0;
+        switch (x.kind) {
+        case PK::Col:
+        case PK::Const: r = 0; break;
+        case PK::Neg:
+        case PK::Cast: r = std::max(1, need(x.a)); break;
+        default: {
+            int l = need(x.a), q = need(x.b);
+            r = l == q ? std::max(1, l + (l > 0 ? 1 : 0)) : std::max(l, q);
+            break;
+        }
+        }
+        need_memo[id] = r;
+        return r;
+    }
+
+    int alloc()
+    {
+        for (int r = 0; r < updec_vm::kMaxRegs; r++)
+            if (!used[r]) { used[r] = true; out.nregs = std::max(out.nregs, r + 1); return r; }
+        err = "expression needs more than 8 interpreter registers";
+        return -1;
+    }
+    void release(const updec_vm::Operand& o)
+    {
+        if (o.kind == updec_vm::K_REG) used[o.idx] = false;
+    }
+
+    bool constant(int id, updec_vm::Operand* o)
+    {
+        auto it = const_index.find(id);
+        int k;
+        if (it != const_index.end()) {
+            k = it->second;
+        } else {
+            k = (int)const_index.size();
+            if (k >= 256) { err = "too many constants"; return false; }
+            const_index[id] = k;
+            const HostDec& d = p.n[id].k;
+            out.consts.push_back(d.sign);
+            for (int w = 0; w < out.W; w++) out.consts.push_back(w < HostDec::NW ? d.v[w] : 0u);
+        }
+        *o = {updec_vm::K_CONST, (uint8_t)k, 0};
+        return true;
+    }
+
+    bool emit(updec_vm::Op op, int dst, int arg, updec_vm::Operand a, updec_vm::Operand b)
+    {
+        if ((int)out.ins.size() >= updec_vm::kMaxIns) { err = "interpreter program too long"; return false; }
+        updec_vm::Ins I{};
+        I.op = op;
+        I.dst = (uint8_t)dst;
+        I.arg = (uint16_t)arg;
+        I.a = a;
+        I.b = b;
+        out.ins.push_back(I);
+        return true;
+    }
+
+    // evaluate node `id`; the result is a register, a column or a constant
+    bool gen(int id, updec_vm::Operand* res)
+    {
+        const PNode& x = p.n[id];
+        if (x.kind == PK::Col) {
+            if (x.col >= updec_vm::kMaxCols) { err = "too many columns for the interpreter"; return false; }
+            *res = {updec_vm::K_COL, (uint8_t)x.col, 0};
+            return true;
+        }
+        if (x.kind == PK::Const) return constant(id, res);
+        // b stays a (never read) constant operand for the unary operations
+        updec_vm::Operand a{}, b{updec_vm::K_CONST, 0, 0};
+        const bool unary = x.kind == PK::Neg || x.kind == PK::Cast;
+        if (unary) {
+            if (!gen(x.a, &a)) return false;
+        } else if (need(x.b) > need(x.a)) {
+            if (!gen(x.b, &b) || !gen(x.a, &a)) return false;
+        } else {
+            if (!gen(x.a, &a) || !gen(x.b, &b)) return false;
+        }
+        release(a);
+        if (!unary) release(b);
+        const int dst = alloc();
+        if (dst < 0) return false;
+        const PNode& A = p.n[x.a];
+        bool ok = true;
+        switch (x.kind) {
+        case PK::Neg: ok = emit(updec_vm::OP_NEG, dst, 0, a, {}); break;
+        case PK::Cast:
+            if (x.S > A.S) ok = emit(updec_vm::OP_UP, dst, x.S - A.S, a, {});
+            else if (x.S < A.S) ok = emit(updec_vm::OP_DOWN, dst, A.S - x.S, a, {});
+            else ok = emit(updec_vm::OP_MOV, dst, 0, a, {});
+            break;
+        case PK::Add:
+        case PK::Sub: {
+            const PNode& B = p.n[x.b];
+            a.shift = (uint16_t)(x.S - A.S);
+            b.shift = (uint16_t)(x.S - B.S);
+            ok = emit(x.kind == PK::Add ? updec_vm::OP_ADD : updec_vm::OP_SUB, dst, 0, a, b);
+            break;
+        }
+        case PK::Mul: ok = emit(updec_vm::OP_MUL, dst, 0, a, b); break;
+        case PK::Div: ok = emit(updec_vm::OP_DIV, dst, p.n[x.b].S + 4, a, b); break;
+        default: err = "internal: unexpected plan node"; return false;
+        }
+        *res = {updec_vm::K_REG, (uint8_t)dst, 0};
+        return ok;
+    }
+};
+
+}  // namespace
+
+bool build_vm_program(const Plan& p, const std::vector<ColumnType>& cols, VmProgram* out, std::string* err)
+{
+    VmProgram prog;
+    // the register width holds every node's exact bound, so no value wraps
+    // and no intermediate (aligned operand, scaled dividend) is cut
+    int words = 1;
+    for (const PNode& x : p.n) {
+        words = std::max(words, words_for(x.ubits));
+        if (x.S > 65535) { *err = "scale out of the interpreter's range"; return false; }
+    }
+    for (const PNode& x : p.n)
+        if (x.kind == PK::Col) words = std::max(words, words_for(x.bits));
+    prog.wclass = -1;
+    for (int c = 0; c < updec_vm::kNumClasses; c++)
+        if (updec_vm::kClassW[c] >= words) { prog.wclass = c; break; }
+    if (prog.wclass < 0) {
+        *err = "exact intermediates exceed the interpreter's 32-limb registers";
+        return false;
+    }
+    prog.W = updec_vm::kClassW[prog.wclass];
+    prog.faithful = p.mode == Mode::Faithful;
+    prog.outP = p.outP;
+    prog.outS = p.outS;
+    prog.outB = p.outB;
+    VmBuilder b(p, cols, prog);
+    b.need_memo.assign(p.n.size(), -1);
+    updec_vm::Operand r{};
+    if (!b.gen(p.root, &r)) { *err = b.err; return false; }
+    if (r.kind != updec_vm::K_REG) {
+        const int dst = b.alloc();
+        if (dst < 0 || !b.emit(updec_vm::OP_MOV, dst, 0, r, {})) { *err = b.err; return false; }
+        r = {updec_vm::K_REG, (uint8_t)dst, 0};
+    }
+    prog.root = r.idx;
+    *out = std::move(prog);
+    return true;
+}
+
+std::string VmProgram::text() const
+{
+    static const char* names[] = {"MOV", "NEG", "ADD", "SUB", "MUL", "DIV", "UP", "DOWN"};
+    auto opnd = [](const updec_vm::Operand& o) {
+        std::string s = (o.kind == updec_vm::K_REG ? "r" : o.kind == updec_vm::K_COL ? "c" : "k") + std::to_string(o.idx);
+        if (o.shift) s += "<<" + std::to_string(o.shift);
+        return s;
+    };
+    std::ostringstream os;
+    os << "W" << W << " regs " << nregs << " root r" << root << "\n";
+    for (const auto& I : ins) {
+        os << names[I.op] << " r" << (int)I.dst << " " << opnd(I.a);
+        if (I.op >= updec_vm::OP_ADD && I.op <= updec_vm::OP_DIV) os << " " << opnd(I.b);
+        if (I.op == updec_vm::OP_DIV || I.op == updec_vm::OP_UP || I.op == updec_vm::OP_DOWN) os << " #" << I.arg;
+        os << "\n";
+    }
+    return os.str();
 }
 
 }  // namespace updec

@@ -7,6 +7,8 @@
 #include <string>
 #include <vector>
 
+#include "vm.h"
+
 namespace updec {
 
 enum class Mode { Faithful = 0, Extended = 1 };
@@ -85,7 +87,8 @@ struct PNode {
     int col = -1;
     HostDec k;               // Const
     int a = -1, b = -1;      // children
-    int bits = 0;            // magnitude bound: value < 2^bits
+    int bits = 0;            // magnitude bound: value < 2^bits (faithful mode: <= 1024, values wrap)
+    int ubits = 0;           // the same bound without the 32-limb wrap
     int W = 1;               // limbs
 };
 struct Plan {
@@ -103,6 +106,12 @@ int compact_bytes(int P);   // __PREC_BYTE_ARRAY (decimal.cu:92), closed form pa
 // HIP path does not implement (MOD).
 bool build_plan(const ExprP& rewritten, const std::vector<ColumnType>& cols, Mode requested, Plan* out,
                 std::string* err);
+
+// parse -> AlignOptimize -> typed plan.  mode: 0 = pick faithful when the
+// plan stays inside the reference's envelope, else extended; 1 = faithful
+// (an error outside the envelope); 2 = extended.
+bool plan_expression(const std::string& text, const std::vector<std::string>& names,
+                     const std::vector<ColumnType>& types, int mode, Plan* out, std::string* err);
 
 // Classification: true when the plan stays inside the reference's defined
 // envelope (every shift <= 38, scales <= 255, precisions <= 299, divisions
@@ -155,5 +164,20 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
                             const GenOptions& o, int* acc_words, int* block_out);
 // limbs of one block partial of the fused aggregation
 int agg_acc_words(const AggSpec& s);
+
+// ---- the precompiled interpreter (vm.h) ----
+// Register program for one plan: the plan tree is expanded (shared nodes
+// are recomputed, so no value stays live longer than in the tree), operands
+// are ordered by register need (Sethi-Ullman) and every register holds W
+// limbs, W the smallest class that holds every node's exact bound.
+struct VmProgram {
+    std::vector<updec_vm::Ins> ins;
+    std::vector<uint32_t> consts;   // W + 1 words per constant: sign, limbs
+    int wclass = 0, W = 0, nregs = 0, root = 0;
+    int outB = 0, outP = 0, outS = 0;
+    bool faithful = false;
+    std::string text() const;       // one line per instruction
+};
+bool build_vm_program(const Plan& p, const std::vector<ColumnType>& cols, VmProgram* out, std::string* err);
 
 }  // namespace updec

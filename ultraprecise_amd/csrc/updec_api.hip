@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/updec.h"
+#include "api_internal.h"
 #include "dec_device.h"
 #include "gen_common.h"
 #include "planner.h"
@@ -174,27 +175,7 @@ std::vector<ColumnType> to_types(const updec_column_desc* cols, int n, std::vect
 bool make_plan(const std::string& text, const std::vector<std::string>& names, const std::vector<ColumnType>& types,
                int mode, Plan* out, std::string* err)
 {
-    for (int attempt = 0; attempt < 2; attempt++) {
-        Mode m;
-        if (mode == UPDEC_MODE_FAITHFUL) m = Mode::Faithful;
-        else if (mode == UPDEC_MODE_EXTENDED) m = Mode::Extended;
-        else m = attempt == 0 ? Mode::Faithful : Mode::Extended;
-        ExprP e = parse_expr(text, names, types, m, err);
-        if (!e) return false;
-        ExprP rw = align_optimize(e, m);
-        if (!rw) { *err = "expression rewrites to a constant or integer"; return false; }
-        Plan p;
-        if (!build_plan(rw, types, m, &p, err)) return false;
-        bool inside = plan_is_faithful_envelope(p, types);
-        if (mode == UPDEC_MODE_AUTO && m == Mode::Faithful && !inside) continue;
-        if (mode == UPDEC_MODE_FAITHFUL && !inside) {
-            *err = "expression leaves the reference's defined envelope; use extended mode";
-            return false;
-        }
-        *out = std::move(p);
-        return true;
-    }
-    return false;
+    return plan_expression(text, names, types, mode, out, err);
 }
 
 int copy_text(const std::string& s, char* buf, size_t cap, size_t* needed)
@@ -211,6 +192,20 @@ int copy_text(const std::string& s, char* buf, size_t cap, size_t* needed)
 std::atomic<uint64_t> g_kernel_id{0};
 
 }  // namespace
+
+namespace updec {
+int api_fail(int code, const std::string& msg) { return fail(code, msg); }
+int api_device_cus() { return device_cus(); }
+std::vector<ColumnType> api_types(const updec_column_desc* cols, int n, std::vector<std::string>* names)
+{
+    return to_types(cols, n, names);
+}
+bool api_make_plan(const std::string& text, const std::vector<std::string>& names,
+                   const std::vector<ColumnType>& types, int mode, Plan* out, std::string* err)
+{
+    return make_plan(text, names, types, mode, out, err);
+}
+}  // namespace updec
 
 // ------------------------------------------------------------------ static kernels
 

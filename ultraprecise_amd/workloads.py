@@ -91,6 +91,59 @@ def sin_desc(U, seed, col_id=0, P=300, S=100):
     return U.gen_desc(GEN_BELOW, P, seed, col_id, bound=pi_scaled(S) + 1, neg_permille=500)
 
 
+# batched serving: 10,000 queries, seed 4, Poisson arrivals at 200/400/800
+# queries/s; rows log-uniform in [1e3, 1e7]; P from the list with S = P/3; a
+# random + - * / tree of depth 2..6 over 2..4 fresh uniform-digit columns,
+# then SUM.  Trees are redrawn (from the same seeded stream) until every
+# exact intermediate fits in 1024 bits, the 32-limb Decimal of the
+# reference, so each query runs on the interpreter's register classes.
+CONFIG5 = {"queries": 10_000, "seed": 4, "rates": (200, 400, 800), "rows": (1e3, 1e7),
+           "precisions": (15, 18, 38, 76, 150, 300), "depth": (2, 6), "ncols": (2, 4)}
+
+
+def random_tree(rng, names, depth):
+    """fully parenthesized random tree of exactly `depth` operator levels"""
+    if depth == 0:
+        return rng.choice(names)
+    op = rng.choice("+-*/")
+    deep = gen_side = rng.random() < 0.5
+    left = random_tree(rng, names, depth - 1 if deep else rng.randrange(0, depth))
+    right = random_tree(rng, names, rng.randrange(0, depth) if deep else depth - 1)
+    return f"({left}{op}{right})"
+
+
+def random_query(seed, i, accept=None, config=None):
+    """Query i of the serving stream: {"text", "cols", "rows", "P", "S",
+    "col_ids", "tries"}.  `accept(text, cols)` -> bool decides whether a
+    drawn tree is kept (the interpreter's width check)."""
+    import math
+    import random
+    C = config or CONFIG5
+    rng = random.Random(seed * 1_000_003 + i)
+    lo, hi = C["rows"]
+    rows = int(round(10 ** rng.uniform(math.log10(lo), math.log10(hi))))
+    P = rng.choice(C["precisions"])
+    S = P // 3
+    ncols = rng.randint(*C["ncols"])
+    names = [chr(ord("a") + k) for k in range(ncols)]
+    cols = [(n, P, S) for n in names]
+    tries = 0
+    while True:
+        tries += 1
+        depth = rng.randint(*C["depth"])
+        text = random_tree(rng, names, depth)
+        if accept is None or accept(text, cols):
+            break
+    return {"text": text, "cols": cols, "rows": rows, "P": P, "S": S,
+            "col_ids": [1000 + 16 * (i % 4096) + k for k in range(ncols)], "tries": tries}
+
+
+def query_column_desc(U, seed, q, k):
+    """generator of the k-th column of a serving query: P uniform digits,
+    sign 50/50 (never zero, so divisions are defined)"""
+    return U.gen_desc(GEN_DIGITS, q["P"], seed, q["col_ids"][k], digits=q["P"], neg_permille=500)
+
+
 # column ids of the generator streams
 COL_FLAG1, COL_FLAG2, COL_SHIP = 100, 101, 102
 
