@@ -204,6 +204,65 @@ def series(U, W, steps, cpu_rows, threads):
     return res
 
 
+def serving_bench(U, W, nq, rates, budget_gb):
+    """the batched serving workload: nq seeded queries over resident columns,
+    saturated (all queued at once) and at Poisson arrival rates, batched
+    (HIP graph) against one plain launch per query"""
+    import numpy as np
+    from ultraprecise_amd import serving
+
+    def ok(text, cols):
+        try:
+            U.VmProgram(text, cols)
+            return True
+        except U.UpdecError as e:
+            if "exceed" in str(e):
+                return False
+            raise
+
+    C = W.CONFIG5
+    t0 = time.time()
+    qs = [W.random_query(C["seed"], i, ok) for i in range(nq)]
+    rows_max = int(C["rows"][1])
+    store = serving.ColumnStore(U, budget_gb << 30, seed=C["seed"], rows=rows_max)
+    keys = sorted({k for q in qs for k in q["keys"]})
+    for k in keys:   # resident columns, generated once before the stream starts
+        if store.acquire([k]) is None:
+            raise RuntimeError("the column budget cannot hold the resident set")
+        store.release([k])
+    U.device_sync()
+    setup_s = time.time() - t0
+    imgs = {m: U.DeviceBuffer(133 * nq).zero() for m in ("batched", "plain")}
+    cnts = {m: U.DeviceBuffer(8 * nq).zero() for m in ("batched", "plain")}
+    out = {"queries": nq, "rows": sum(q["rows"] for q in qs), "setup_s": setup_s,
+           "columns_resident_gb": store.used / 2 ** 30,
+           "width_classes": {w: sum(1 for q in qs if U.VmProgram(q["text"], q["cols"]).width == w)
+                             for w in U.VM_CLASS_WIDTHS}}
+    zero = np.zeros(nq)
+    sat = {}
+    for mode in ("plain", "batched"):
+        srv = serving.Server(U, store, batched=(mode == "batched"))
+        st = srv.run(qs, zero, imgs[mode], cnts[mode])
+        sat[mode] = {k: v for k, v in st.items() if k != "latency_s"}
+        log(f"serving saturated {mode}:", sat[mode])
+    same = bytes(imgs["batched"].numpy()) == bytes(imgs["plain"].numpy())
+    out["saturated"] = sat
+    out["saturated_speedup"] = sat["batched"]["mrows_per_s"] / sat["plain"]["mrows_per_s"]
+    out["identical_results"] = same
+    pois = {}
+    for rate in rates:
+        arr = serving.poisson_arrivals(nq, rate, seed=C["seed"] + rate)
+        pois[rate] = {}
+        for mode in ("batched", "plain"):
+            srv = serving.Server(U, store, batched=(mode == "batched"))
+            st = srv.run(qs, arr, imgs[mode], cnts[mode])
+            pois[rate][mode] = {k: v for k, v in st.items() if k != "latency_s"}
+            log(f"serving {rate}/s {mode}:", pois[rate][mode])
+        pois[rate]["identical_results"] = bytes(imgs["batched"].numpy()) == bytes(imgs["plain"].numpy())
+    out["poisson"] = pois
+    return out
+
+
 def copy_bandwidth(U, nbytes=1 << 30, reps=5):
     a = U.DeviceBuffer(nbytes)
     b = U.DeviceBuffer(nbytes)
@@ -242,9 +301,12 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=2_000_000, help="rows of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="headline only")
-    ap.add_argument("--extra", default="copy,plain,sweep,series", help="comma list of extra workloads")
+    ap.add_argument("--extra", default="copy,plain,sweep,series,serving", help="comma list of extra workloads")
     ap.add_argument("--sweep-rows", type=int, default=0)
     ap.add_argument("--series-cpu-rows", type=int, default=50_000, help="rows of the series CPU sample")
+    ap.add_argument("--serve-queries", type=int, default=2000, help="queries per serving run (workload: 10,000)")
+    ap.add_argument("--serve-rates", default="200,400,800")
+    ap.add_argument("--serve-budget-gb", type=int, default=24, help="device memory for resident columns")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -272,6 +334,9 @@ def main():
     if "series" in want:
         extra["series_1M"] = series(U, W, max(2, args.steps // 5), args.series_cpu_rows, threads)
         log("series:", extra["series_1M"])
+    if "serving" in want:
+        rates = [int(r) for r in args.serve_rates.split(",")]
+        extra["serving"] = serving_bench(U, W, args.serve_queries, rates, args.serve_budget_gb)
     value = h["rows"] / h["ms_per_step"] / 1e3  # Mrows/s
     achieved = h["achieved_gbps"]
     cpu = h.get("cpu")

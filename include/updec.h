@@ -24,6 +24,7 @@ extern "C" {
 #define UPDEC_ECOMPILE 2     /* parse, rewrite or hipRTC compile failure */
 #define UPDEC_EHIP 3         /* HIP runtime error */
 #define UPDEC_EUNSUPPORTED 4 /* construct outside what the HIP path implements */
+#define UPDEC_EFULL 5        /* batch has no room for this query: launch it first */
 
 #define UPDEC_MODE_AUTO 0     /* faithful inside the reference's envelope, else extended */
 #define UPDEC_MODE_FAITHFUL 1 /* 32-limb wrap and DivOrMod path quirks */
@@ -69,6 +70,9 @@ int updec_device_info(int32_t* cus, int32_t* clock_khz, int64_t* mem_bytes, char
 int updec_event_create(void** ev);
 int updec_event_record(void* ev, updec_stream_t stream);
 int updec_event_elapsed(void* ev0, void* ev1, float* ms);
+/* *done = 1 when the work before the event has finished (non-blocking) */
+int updec_event_query(void* ev, int32_t* done);
+int updec_event_sync(void* ev);
 int updec_event_destroy(void* ev);
 
 /* Plain streaming copy kernel (16-byte vector loads/stores), used to
@@ -203,7 +207,9 @@ int updec_batch_create(int32_t max_queries, int32_t max_items, updec_batch_t* ou
 /* per width class (0..4 for W = 2..32): rows per work item and resident
  * blocks per compute unit of the persistent kernel */
 int updec_batch_tuning(updec_batch_t b, int32_t width_class, int64_t rows_per_item, int32_t blocks_per_cu);
-/* *index_out (optional): the query's position in the pending batch */
+/* *index_out (optional): the query's position in the pending batch.
+ * UPDEC_EFULL (nothing staged) when the batch's query, work-item or program
+ * space would overflow: launch the batch and add the query to the next. */
 int updec_batch_add(updec_batch_t b, updec_vm_t v, const uint8_t* const* d_cols, int64_t rows, uint8_t* d_image,
                     int64_t* d_count, int32_t* index_out);
 /* number of queries added since the last launch */

@@ -48,19 +48,22 @@ static int bitlen_w(const uint32_t* v, int nw) { int t = top_word(v, nw); return
 /* closed form past the table: bytes for bitlen(10^P - 1) magnitude bits plus
  * the sign bit.  10^P - 1 has the bit length of 10^P for P >= 1 (10^P is not
  * a power of two), so one pass of exact limb arithmetic fills every P. */
-#define OD_MAX_DIGITS 1856
+#define OD_MAX_DIGITS 20000
+#define OD_TABLE_WORDS 2100   /* 10^20000 < 2^66440 */
 static int g_bytes_past[OD_MAX_DIGITS + 1];
 static pthread_once_t g_bytes_once = PTHREAD_ONCE_INIT;
 static void fill_bytes_past(void)
 {
-    static uint32_t v[OD_MAXW + 8];
-    int nw = OD_MAXW + 8;
+    static uint32_t v[OD_TABLE_WORDS];
+    int nw = OD_TABLE_WORDS;
     memset(v, 0, sizeof v);
     v[0] = 1;
+    int top = 1;   /* words in use */
     for (int P = 1; P <= OD_MAX_DIGITS; P++) {
         uint64_t c = 0;
-        for (int i = 0; i < nw; i++) { uint64_t t = (uint64_t)v[i] * 10u + c; v[i] = (uint32_t)t; c = t >> 32; }
-        g_bytes_past[P] = (bitlen_w(v, nw) + 1 + 7) / 8;
+        for (int i = 0; i < top + 1 && i < nw; i++) { uint64_t t = (uint64_t)v[i] * 10u + c; v[i] = (uint32_t)t; c = t >> 32; }
+        if (top < nw && v[top]) top++;
+        g_bytes_past[P] = (bitlen_w(v, top) + 1 + 7) / 8;
     }
 }
 
@@ -221,10 +224,11 @@ void od_from_compact(od_dec* d, const uint8_t* buf, int P, int S)
     d->prec = P; d->frac = S;
     int len = od_compact_bytes(P);
     uint8_t* mv = (uint8_t*)d->v;
-    memcpy(mv, buf, (size_t)len);
+    int keep = len < (int)sizeof d->v ? len : (int)sizeof d->v;
+    memcpy(mv, buf, (size_t)keep);
     if (len > 0) {
-        d->sign = (mv[len - 1] >> 7) & 1;
-        mv[len - 1] &= 0x7f;
+        d->sign = (buf[len - 1] >> 7) & 1;
+        if (len == keep) mv[len - 1] &= 0x7f;
     }
 }
 
@@ -232,7 +236,9 @@ void od_from_compact(od_dec* d, const uint8_t* buf, int P, int S)
  * into bit 7 of the last byte (a magnitude bit there is kept and merged). */
 void od_to_compact(const od_dec* d, uint8_t* buf, int len)
 {
-    memcpy(buf, (const uint8_t*)d->v, (size_t)len);
+    int keep = len < (int)sizeof d->v ? len : (int)sizeof d->v;
+    memcpy(buf, (const uint8_t*)d->v, (size_t)keep);
+    if (len > keep) memset(buf + keep, 0, (size_t)(len - keep));
     if (len > 0) buf[len - 1] |= (uint8_t)(d->sign << 7);
 }
 
@@ -666,8 +672,11 @@ static void eval_stored(const evctx* c, int64_t row, od_dec* r)
 {
     const pnode* root = &c->p->n[c->p->root];
     eval_node(c, c->p->root, row, r);
-    uint8_t buf[OD_MAXW * 4];
     int len = od_compact_bytes(root->P);
+    /* wider than the value array: every magnitude bit survives, the round
+     * trip only keeps the sign */
+    if (len > OD_MAXW * 4) return;
+    uint8_t buf[OD_MAXW * 4];
     od_to_compact(r, buf, len);
     od_from_compact(r, buf, root->P, root->S);
 }

@@ -66,6 +66,8 @@ def _load():
         "updec_event_record": (ctypes.c_int, [P, P]),
         "updec_event_elapsed": (ctypes.c_int, [P, P, ctypes.POINTER(ctypes.c_float)]),
         "updec_event_destroy": (ctypes.c_int, [P]),
+        "updec_event_query": (ctypes.c_int, [P, ctypes.POINTER(I32)]),
+        "updec_event_sync": (ctypes.c_int, [P]),
         "updec_column_maxbits": (ctypes.c_int, [P, ctypes.c_int, I64, ctypes.POINTER(I32), P]),
         "updec_expr_compile": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ColumnDesc), ctypes.c_int, ctypes.c_int,
                                               ctypes.POINTER(P)]),
@@ -116,7 +118,7 @@ EXPORTED = ["updec_last_error", "updec_version", "updec_init", "updec_compact_by
             "updec_malloc", "updec_free", "updec_memcpy_htod", "updec_memcpy_dtoh", "updec_memset",
             "updec_stream_create", "updec_stream_destroy", "updec_stream_sync", "updec_device_sync",
             "updec_device_info", "updec_event_create", "updec_event_record", "updec_event_elapsed",
-            "updec_event_destroy", "updec_copy_kernel",
+            "updec_event_destroy", "updec_event_query", "updec_event_sync", "updec_copy_kernel",
             "updec_expr_compile", "updec_expr_info", "updec_expr_plan", "updec_expr_source", "updec_expr_calc",
             "updec_expr_free", "updec_agg_compile", "updec_agg_run", "updec_agg_last_kernel_ms", "updec_agg_source",
             "updec_agg_free", "updec_div_int64", "updec_segsum_compact", "updec_compile", "updec_launch",
@@ -216,6 +218,14 @@ class Event:
 
     def record(self, stream=None):
         _check(_lib.updec_event_record(ctypes.c_void_p(self.handle), _stream(stream)))
+
+    def done(self):
+        d = ctypes.c_int32()
+        _check(_lib.updec_event_query(ctypes.c_void_p(self.handle), ctypes.byref(d)))
+        return bool(d.value)
+
+    def sync(self):
+        _check(_lib.updec_event_sync(ctypes.c_void_p(self.handle)))
 
     def elapsed_ms(self, end):
         ms = ctypes.c_float(0)
@@ -387,6 +397,23 @@ def gen_compact_host(desc, rows, row0=0):
     return out
 
 
+class DeviceView:
+    """A range of a DeviceBuffer (no ownership): usable wherever a buffer's
+    .ptr is read."""
+
+    def __init__(self, base, offset, nbytes):
+        self.base = base
+        self.ptr = base.ptr + offset
+        self.nbytes = nbytes
+
+
+def gen_compact_into(desc, rows, dst, row0=0, stream=None):
+    """generate a compact column into an existing device buffer or view"""
+    if dst.nbytes < rows * compact_bytes(desc.precision):
+        raise ValueError("destination too small")
+    _check(_lib.updec_gen_compact(ctypes.byref(desc), row0, rows, _ptr(dst), 1, _stream(stream)))
+
+
 def gen_compact_device(desc, rows, row0=0, stream=None):
     nb = compact_bytes(desc.precision)
     out = DeviceBuffer(max(4, rows * nb + 16))
@@ -424,6 +451,7 @@ def gen_byte_device(seed, column_id, choices, rows, row0=0, stream=None):
 
 
 VM_CLASS_WIDTHS = (2, 4, 8, 16, 32)
+EFULL = 5   # UPDEC_EFULL: the batch has no room for the query
 
 
 class VmProgram:
@@ -459,8 +487,9 @@ class VmProgram:
 
 class Batch:
     """Queries collected for one set of interpreter launches.  add() stages
-    a query; launch() runs every staged query (as one HIP graph when
-    graph=True) and writes each query's SUM image and COUNT."""
+    a query (None when the batch is full: launch it first); launch() runs
+    every staged query (as one HIP graph when graph=True) and writes each
+    query's SUM image and COUNT."""
 
     def __init__(self, max_queries=1024, max_items=65536):
         h = ctypes.c_void_p()
@@ -476,8 +505,11 @@ class Batch:
         image_offset; count: one int64 at count_offset."""
         ptrs = (ctypes.c_void_p * max(1, len(columns)))(*[c.ptr for c in columns])
         idx = ctypes.c_int32()
-        _check(_lib.updec_batch_add(self._h, prog._h, ptrs, rows, ctypes.c_void_p(image.ptr + image_offset),
-                                    ctypes.c_void_p(count.ptr + count_offset), ctypes.byref(idx)))
+        rc = _lib.updec_batch_add(self._h, prog._h, ptrs, rows, ctypes.c_void_p(image.ptr + image_offset),
+                                  ctypes.c_void_p(count.ptr + count_offset), ctypes.byref(idx))
+        if rc == EFULL:
+            return None
+        _check(rc)
         self._keep.append(prog)
         return idx.value
 

@@ -422,6 +422,7 @@ void infer_frontend_types(Expr& e, Mode mode)
     case Op::Mod: P = Pb; S = 0; break;
     }
     if (mode == Mode::Faithful) { P &= 0xff; S &= 0xff; }  // Decimal::prec/frac are uint8
+    else P = std::max(P, 1);
     e.P = P;
     e.S = S;
 }
@@ -763,11 +764,16 @@ BT* leaf_from_mt(Arena& A, const MT* m)
 }
 
 // ListToBinTree (AriesEngineShell.cpp:90-144): left-deep chain with the
-// per-node result types.  Types are uint16 like AriesDataType.
-BT* list_to_bin(Arena& A, std::vector<BT*>& L, int index, Op op)
+// per-node result types.  Faithful mode keeps the reference's uint16 type
+// arithmetic (a DIV whose rule gives a negative precision wraps to ~65500);
+// extended mode computes the same rules in plain integers and keeps the
+// precision at least 1, so the materialized width stays a real type.
+static int ty(int v, Mode mode) { return mode == Mode::Faithful ? (int)(uint16_t)v : v; }
+
+BT* list_to_bin(Arena& A, std::vector<BT*>& L, int index, Op op, Mode mode)
 {
     if (index == 0) return L[0];
-    BT* l = list_to_bin(A, L, index - 1, op);
+    BT* l = list_to_bin(A, L, index - 1, op, mode);
     BT* r = L[index];
     BT* t = A.bt();
     t->kind = NK::Calc;
@@ -775,43 +781,44 @@ BT* list_to_bin(Arena& A, std::vector<BT*>& L, int index, Op op)
     if (op == Op::Add) {
         if (r->reverse) { r->reverse = false; t->op = Op::Sub; }
         else t->op = Op::Add;
-        int P = (uint16_t)(std::max(P0 - S0, P1 - S1) + 1);
+        int P = ty(std::max(P0 - S0, P1 - S1) + 1, mode);
         int S = std::max(S0, S1);
-        t->P = (uint16_t)(P + S);
+        t->P = ty(P + S, mode);
         t->S = S;
     } else if (op == Op::Mul) {
         t->op = Op::Mul;
-        int P = (uint16_t)((P0 - S0) + (P1 - S1));
-        int S = (uint16_t)(S0 + S1);
-        t->P = (uint16_t)(P + S);
+        int P = ty((P0 - S0) + (P1 - S1), mode);
+        int S = ty(S0 + S1, mode);
+        t->P = ty(P + S, mode);
         t->S = S;
     } else if (op == Op::Div) {
         t->op = Op::Div;
-        t->P = (uint16_t)(P0 - P1 + S1 + 4 + 1);
-        t->S = (uint16_t)(S0 + 4);
+        t->P = ty(P0 - P1 + S1 + 4 + 1, mode);
+        t->S = ty(S0 + 4, mode);
     } else {
         t->op = Op::Mod;
         t->P = P1;
         t->S = 0;
     }
+    if (mode == Mode::Extended) t->P = std::max(t->P, 1);
     t->ch = {l, r};
     return t;
 }
 
 // MultiToBinTree (AriesEngineShell.cpp:146-182)
-BT* multi_to_bin(Arena& A, MT* m)
+BT* multi_to_bin(Arena& A, MT* m, Mode mode)
 {
     std::vector<BT*> tmp;
     for (MT* c : m->ch) {
         if (c->kind != NK::Calc) tmp.push_back(leaf_from_mt(A, c));
-        else tmp.push_back(multi_to_bin(A, c));
+        else tmp.push_back(multi_to_bin(A, c, mode));
     }
     if (m->ch.empty()) return leaf_from_mt(A, m);
     if (m->kind == NK::Calc && m->ch.size() == 1) {
-        if (m->ch[0]->kind == NK::Calc) return list_to_bin(A, tmp, (int)tmp.size() - 1, m->op);
+        if (m->ch[0]->kind == NK::Calc) return list_to_bin(A, tmp, (int)tmp.size() - 1, m->op, mode);
         return leaf_from_mt(A, m->ch[0]);
     }
-    return list_to_bin(A, tmp, (int)tmp.size() - 1, m->op);
+    return list_to_bin(A, tmp, (int)tmp.size() - 1, m->op, mode);
 }
 
 ExprP to_expr(const BT* b)
@@ -851,7 +858,7 @@ ExprP align_optimize(const ExprP& e, Mode mode)
     if (!align_opt(A, mt, mode, &err)) return nullptr;
     mt = remove_zero(mt);
     if (mt->kind == NK::Int) return nullptr;
-    BT* out = multi_to_bin(A, mt);
+    BT* out = multi_to_bin(A, mt, mode);
     // AlignOptimizeNode (AriesEngineShell.cpp:628-644): root precision capped at 288
     if (mode == Mode::Faithful && out->P > 288) out->P = 288;
     return to_expr(out);

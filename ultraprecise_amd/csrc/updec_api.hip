@@ -497,6 +497,26 @@ int updec_event_elapsed(void* ev0, void* ev1, float* ms)
     HIPCHK(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(ev0), reinterpret_cast<hipEvent_t>(ev1)));
     return UPDEC_OK;
 }
+int updec_event_query(void* ev, int32_t* done)
+{
+    if (!ev || !done) return fail(UPDEC_EINVAL, "bad argument");
+    hipError_t e = hipEventQuery(reinterpret_cast<hipEvent_t>(ev));
+    if (e == hipSuccess) {
+        *done = 1;
+        return UPDEC_OK;
+    }
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return UPDEC_OK;
+    }
+    return fail(UPDEC_EHIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+}
+int updec_event_sync(void* ev)
+{
+    if (!ev) return fail(UPDEC_EINVAL, "bad argument");
+    HIPCHK(hipEventSynchronize(reinterpret_cast<hipEvent_t>(ev)));
+    return UPDEC_OK;
+}
 int updec_event_destroy(void* ev)
 {
     if (ev) HIPCHK(hipEventDestroy(reinterpret_cast<hipEvent_t>(ev)));

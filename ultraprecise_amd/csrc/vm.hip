@@ -160,8 +160,9 @@ struct updec_vm_s {
 
 struct updec_batch_s {
     int max_queries = 0, max_items = 0, max_prog_words = 0, max_const_words = 0;
-    int64_t rows_per_item[kNumClasses] = {1 << 16, 1 << 16, 1 << 15, 1 << 14, 1 << 13};
-    int blocks_per_cu[kNumClasses] = {4, 4, 4, 4, 2};
+    // defaults sized so one 10^7-row query alone makes >= 4 items per CU
+    int64_t rows_per_item[kNumClasses] = {8192, 8192, 4096, 2048, 1024};
+    int blocks_per_cu[kNumClasses] = {4, 4, 4, 4, 4};
     Header* h_hdr = nullptr;
     Query* h_q = nullptr;
     Item* h_items = nullptr;
@@ -179,6 +180,9 @@ struct updec_batch_s {
     hipEvent_t fork = nullptr, join[kNumClasses] = {};
     hipGraphExec_t graph = nullptr;
     std::vector<Pending> pending;
+    // space the pending queries take at launch
+    uint32_t pend_items = 0, pend_prog_words = 0, pend_const_words = 0;
+    std::map<const VmProgram*, int> pend_programs;
     std::mutex mu;
 
     ~updec_batch_s()
@@ -202,6 +206,13 @@ struct updec_batch_s {
         (void)hipFree(d_prog);
         (void)hipFree(d_cst);
         (void)hipFree(d_partials);
+    }
+
+    void clear_pending()
+    {
+        pending.clear();
+        pend_items = pend_prog_words = pend_const_words = 0;
+        pend_programs.clear();
     }
 
     int grid(int cls, uint32_t nitems) const
@@ -330,6 +341,7 @@ int updec_batch_tuning(updec_batch_t b, int32_t width_class, int64_t rows_per_it
         blocks_per_cu > 64)
         return api_fail(UPDEC_EINVAL, "bad argument");
     std::lock_guard<std::mutex> g(b->mu);
+    if (!b->pending.empty()) return api_fail(UPDEC_EINVAL, "tuning changes while queries are staged");
     b->rows_per_item[width_class] = rows_per_item;
     b->blocks_per_cu[width_class] = blocks_per_cu;
     if (b->graph) {
@@ -345,7 +357,16 @@ int updec_batch_add(updec_batch_t b, updec_vm_t v, const uint8_t* const* d_cols,
     if (!b || !v || rows < 0 || !d_image || !d_count || (!d_cols && !v->colbytes.empty()))
         return api_fail(UPDEC_EINVAL, "bad argument");
     std::lock_guard<std::mutex> g(b->mu);
-    if ((int)b->pending.size() >= b->max_queries) return api_fail(UPDEC_EINVAL, "batch is full");
+    const VmProgram& pr = *v->prog;
+    const int64_t ch = b->rows_per_item[pr.wclass];
+    const uint32_t items = (uint32_t)std::max<int64_t>(1, (rows + ch - 1) / ch);
+    const bool new_prog = b->pend_programs.find(&pr) == b->pend_programs.end();
+    const uint32_t pw = new_prog ? 3u * (uint32_t)pr.ins.size() : 0u, cw = new_prog ? (uint32_t)pr.consts.size() : 0u;
+    if ((int)b->pending.size() >= b->max_queries || b->pend_items + items > (uint32_t)b->max_items ||
+        b->pend_prog_words + pw > (uint32_t)b->max_prog_words || b->pend_const_words + cw > (uint32_t)b->max_const_words) {
+        if (b->pending.empty()) return api_fail(UPDEC_EINVAL, "query does not fit an empty batch");
+        return api_fail(UPDEC_EFULL, "batch is full");
+    }
     Pending p{};
     p.prog = v->prog;
     for (size_t c = 0; c < v->colbytes.size(); c++) {
@@ -357,6 +378,10 @@ int updec_batch_add(updec_batch_t b, updec_vm_t v, const uint8_t* const* d_cols,
     p.image = d_image;
     p.count = d_count;
     b->pending.push_back(p);
+    b->pend_items += items;
+    b->pend_prog_words += pw;
+    b->pend_const_words += cw;
+    b->pend_programs[&pr] = 1;
     if (index_out) *index_out = (int32_t)b->pending.size() - 1;
     return UPDEC_OK;
 }
@@ -392,7 +417,7 @@ int updec_batch_launch(updec_batch_t b, int use_graph, updec_stream_t stream)
         total += nitems[c];
     }
     if (total > (uint32_t)b->max_items) {
-        b->pending.clear();
+        b->clear_pending();
         return api_fail(UPDEC_EINVAL, "batch needs more work items than it was created for");
     }
     std::map<const VmProgram*, std::pair<uint32_t, uint32_t>> placed;  // program -> (ins_off, const_off)
@@ -412,7 +437,7 @@ int updec_batch_launch(updec_batch_t b, int use_graph, updec_stream_t stream)
         if (it == placed.end()) {
             const uint32_t pw = 3u * (uint32_t)pr.ins.size(), cw = (uint32_t)pr.consts.size();
             if (prog_words + pw > (uint32_t)b->max_prog_words || const_words + cw > (uint32_t)b->max_const_words) {
-                b->pending.clear();
+                b->clear_pending();
                 return api_fail(UPDEC_EINVAL, "batch programs exceed the staging space");
             }
             for (size_t k = 0; k < pr.ins.size(); k++) {
@@ -460,7 +485,7 @@ int updec_batch_launch(updec_batch_t b, int use_graph, updec_stream_t stream)
         fill[c] += n;
     }
     const size_t nq = b->pending.size();
-    b->pending.clear();
+    b->clear_pending();
     VMCHK(hipMemcpyAsync(b->d_hdr, b->h_hdr, sizeof(Header), hipMemcpyHostToDevice, s));
     VMCHK(hipMemcpyAsync(b->d_q, b->h_q, sizeof(Query) * nq, hipMemcpyHostToDevice, s));
     VMCHK(hipMemcpyAsync(b->d_items, b->h_items, sizeof(Item) * total, hipMemcpyHostToDevice, s));

@@ -134,8 +134,14 @@ def random_query(seed, i, accept=None, config=None):
         text = random_tree(rng, names, depth)
         if accept is None or accept(text, cols):
             break
-    return {"text": text, "cols": cols, "rows": rows, "P": P, "S": S,
+    # resident columns used by the serving loop: distinct ids out of
+    # SERVE_COLUMNS_PER_P per precision
+    keys = [(P, c) for c in rng.sample(range(SERVE_COLUMNS_PER_P), ncols)]
+    return {"text": text, "cols": cols, "rows": rows, "P": P, "S": S, "keys": keys,
             "col_ids": [1000 + 16 * (i % 4096) + k for k in range(ncols)], "tries": tries}
+
+
+SERVE_COLUMNS_PER_P = 6
 
 
 def query_column_desc(U, seed, q, k):
