@@ -220,7 +220,12 @@ def serving_bench(U, W, nq, rates, budget_gb):
                 return False
             raise
 
+    from ultraprecise_amd import autotune
     C = W.CONFIG5
+    t0 = time.time()
+    tuning = autotune.tune(U, use_cache=False)
+    tune_s = time.time() - t0
+    log("serving: autotuned (rows per item, blocks per CU) per width class:", tuning, f"in {tune_s:.1f} s")
     t0 = time.time()
     qs = [W.random_query(C["seed"], i, ok) for i in range(nq)]
     rows_max = int(C["rows"][1])
@@ -234,14 +239,15 @@ def serving_bench(U, W, nq, rates, budget_gb):
     setup_s = time.time() - t0
     imgs = {m: U.DeviceBuffer(133 * nq).zero() for m in ("batched", "plain")}
     cnts = {m: U.DeviceBuffer(8 * nq).zero() for m in ("batched", "plain")}
-    out = {"queries": nq, "rows": sum(q["rows"] for q in qs), "setup_s": setup_s,
+    out = {"queries": nq, "rows": sum(q["rows"] for q in qs), "setup_s": setup_s, "tune_s": tune_s,
+           "tuning": {str(U.VM_CLASS_WIDTHS[k]): v for k, v in tuning.items()},
            "columns_resident_gb": store.used / 2 ** 30,
            "width_classes": {w: sum(1 for q in qs if U.VmProgram(q["text"], q["cols"]).width == w)
                              for w in U.VM_CLASS_WIDTHS}}
     zero = np.zeros(nq)
     sat = {}
     for mode in ("plain", "batched"):
-        srv = serving.Server(U, store, batched=(mode == "batched"))
+        srv = serving.Server(U, store, batched=(mode == "batched"), tuning=tuning)
         st = srv.run(qs, zero, imgs[mode], cnts[mode])
         sat[mode] = {k: v for k, v in st.items() if k != "latency_s"}
         log(f"serving saturated {mode}:", sat[mode])
@@ -254,7 +260,7 @@ def serving_bench(U, W, nq, rates, budget_gb):
         arr = serving.poisson_arrivals(nq, rate, seed=C["seed"] + rate)
         pois[rate] = {}
         for mode in ("batched", "plain"):
-            srv = serving.Server(U, store, batched=(mode == "batched"))
+            srv = serving.Server(U, store, batched=(mode == "batched"), tuning=tuning)
             st = srv.run(qs, arr, imgs[mode], cnts[mode])
             pois[rate][mode] = {k: v for k, v in st.items() if k != "latency_s"}
             log(f"serving {rate}/s {mode}:", pois[rate][mode])

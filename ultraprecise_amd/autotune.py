@@ -1,0 +1,90 @@
+"""Autotuner for the interpreter's persistent kernels.
+
+Per register-width class (W = 2, 4, 8, 16, 32 limbs) it times a
+representative query of that width -- a seeded serving query whose program
+lands in the class -- over every candidate (rows per work item, resident
+blocks per compute unit) and keeps the fastest.  Results are cached per
+device name in a JSON file so a process pays the search once.
+"""
+import json
+import os
+
+DEFAULT_CANDIDATES = ((1024, 4), (2048, 4), (4096, 2), (4096, 4), (8192, 4), (8192, 8), (16384, 4), (32768, 4))
+
+
+def cache_path(device_name):
+    base = os.environ.get("UPDEC_TUNE_DIR", os.path.join(os.path.expanduser("~"), ".cache", "ultraprecise_amd"))
+    safe = "".join(ch if ch.isalnum() else "_" for ch in device_name)
+    return os.path.join(base, f"vm_tuning_{safe}.json")
+
+
+def representative_queries(U, W, per_class=2, seed=77, max_draws=4000):
+    """seeded serving queries, `per_class` of each width class"""
+    def ok(text, cols):
+        try:
+            U.VmProgram(text, cols)
+            return True
+        except U.UpdecError as e:
+            if "exceed" in str(e):
+                return False
+            raise
+
+    found = {c: [] for c in range(len(U.VM_CLASS_WIDTHS))}
+    for i in range(max_draws):
+        q = W.random_query(seed, i, ok)
+        cls = U.VmProgram(q["text"], q["cols"]).width_class
+        if len(found[cls]) < per_class:
+            found[cls].append(q)
+        if all(len(v) >= per_class for v in found.values()):
+            break
+    return found
+
+
+def tune(U, rows=2_000_000, candidates=DEFAULT_CANDIDATES, reps=3, use_cache=True):
+    """returns {class: (rows_per_item, blocks_per_cu)}"""
+    from . import workloads as W
+    name = U.device_info()["name"]
+    path = cache_path(name)
+    if use_cache and os.path.exists(path):
+        with open(path) as f:
+            return {int(k): tuple(v) for k, v in json.load(f).items()}
+    reps = max(1, reps)
+    result = {}
+    for cls, qs in representative_queries(U, W).items():
+        if not qs:
+            continue
+        progs, cols = [], []
+        for q in qs:
+            progs.append(U.VmProgram(q["text"], q["cols"]))
+            cols.append([U.gen_compact_device(U.gen_desc(U.GEN_DIGITS, q["P"], 77, 900 + k, digits=q["P"],
+                                                         neg_permille=500), rows) for k in range(len(q["cols"]))])
+        img = U.DeviceBuffer(133 * len(qs))
+        cnt = U.DeviceBuffer(8 * len(qs))
+        best = None
+        for rpi, bpc in candidates:
+            b = U.Batch(max_queries=len(qs), max_items=1 << 16)
+            b.tune(cls, rpi, bpc)
+
+            def launch():
+                for k, (p, c) in enumerate(zip(progs, cols)):
+                    if b.add(p, c, rows, img, cnt, image_offset=133 * k, count_offset=8 * k) is None:
+                        raise RuntimeError("tuning batch overflow")
+                b.launch(graph=False)
+
+            launch()
+            U.device_sync()
+            e0, e1 = U.Event(), U.Event()
+            e0.record()
+            for _ in range(reps):
+                launch()
+            e1.record()
+            ms = e0.elapsed_ms(e1) / reps
+            if best is None or ms < best[0]:
+                best = (ms, rpi, bpc)
+        result[cls] = (best[1], best[2])
+        del cols
+    if use_cache:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path, "w") as f:
+            json.dump({str(k): list(v) for k, v in result.items()}, f)
+    return result

@@ -1798,6 +1798,10 @@ bool build_vm_program(const Plan& p, const std::vector<ColumnType>& cols, VmProg
         r = {updec_vm::K_REG, (uint8_t)dst, 0};
     }
     prog.root = r.idx;
+    if (prog.nregs > updec_vm::kClassRegs[prog.wclass]) {
+        *err = "expression needs more interpreter registers than its width class holds";
+        return false;
+    }
     *out = std::move(prog);
     return true;
 }

@@ -34,6 +34,9 @@ constexpr int kMaxIns = 512;
 constexpr int kNumClasses = 5;                  // W = 2, 4, 8, 16, 32
 constexpr int kClassW[kNumClasses] = {2, 4, 8, 16, 32};
 constexpr int kMaxAccWords = 34;                // accumulator limbs of the widest class (W + 2)
+// registers per class: the register file is held in LDS, so the wide
+// classes get fewer registers to keep four blocks resident per CU
+constexpr int kClassRegs[kNumClasses] = {8, 8, 8, 6, 4};
 
 struct Operand {
     uint8_t kind;    // Kind

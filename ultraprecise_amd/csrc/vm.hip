@@ -50,7 +50,9 @@ __global__ void __launch_bounds__(T)
     constexpr int NWAVE = T / 64;
     __shared__ u32 s_item;
     __shared__ u32 red[NWAVE * AW];
+    __shared__ u32 regs[kClassRegs[CLS] * (W + 1) * T];
     const u32 tid = threadIdx.x;
+    RegFile<W, T> R{regs + tid};
     const u32 n = h->class_nitems[CLS];
     const u32 i0 = h->class_item0[CLS];
     for (;;) {
@@ -74,7 +76,7 @@ __global__ void __launch_bounds__(T)
         const u32 ob = q->outB;
         Mag<AW> acc = mzero<AW>();
         for (usize row = I.row0 + tid; row < I.row1; row += T) {
-            const Dec<W> v = vm_eval<W>(q, qp, qc, row);
+            const Dec<W> v = vm_eval<W>(q, qp, qc, row, R);
             acc_add<AW, W>(acc, stored_rt<W>(v, ob));
         }
         wave_reduce<AW>(acc);
