@@ -11,16 +11,17 @@ C15 = [("q", 15, 2), ("p", 15, 2), ("d", 15, 2)]
 def test_plain_method_programs(U):
     v = U.VmProgram("p*(1-d)", C15)
     # plan (MUL 31 4 c1 (SUB 16 2 k+64:2 c2)): the constant 1.00 is k0
-    assert v.listing == "W4 regs 1 root r0\nSUB r0 k0 c2\nMUL r0 c1 r0\n"
+    # the SUB result is forwarded to the MUL without a register write
+    assert v.listing == "W4 regs 1 root acc\nSUB - k0 c2\nMUL - c1 acc\n"
     assert (v.precision, v.scale, v.faithful) == (31, 4, True)
-    assert U.VmProgram("q", C15).listing == "W2 regs 1 root r0\nMOV r0 c0\n"
+    assert U.VmProgram("q", C15).listing == "W2 regs 1 root acc\nMOV - c0\n"
 
 
 def test_sweep_program_alignment_and_registers(U):
     v = U.VmProgram("(a+b)*c - a/d", [(n, 38, 10) for n in "abcd"])
     # the division result (scale 14) is aligned to the product's scale 20
-    assert v.listing == ("W8 regs 2 root r0\nNEG r0 c0\nDIV r0 r0 c3 #14\nADD r1 c0 c1\nMUL r1 r1 c2\n"
-                         "ADD r0 r0<<6 r1\n")
+    assert v.listing == ("W8 regs 2 root acc\nNEG - c0\nDIV r0 acc c3 #14\nADD - c0 c1\nMUL - acc c2\n"
+                         "ADD - r0<<6 acc\n")
     v = U.VmProgram("(a+b)*c - a/d", [(n, 150, 50) for n in "abcd"])
     assert v.width == 32 and not v.faithful
     with pytest.raises(U.UpdecError, match="exceed"):
@@ -30,9 +31,9 @@ def test_sweep_program_alignment_and_registers(U):
 def test_series_program(U):
     v = U.VmProgram(W.sin_expr(terms=3), [("x", 300, 100, 334)])
     lines = v.listing.splitlines()
-    assert lines[0] == "W32 regs 2 root r0"
-    assert lines[1:3] == ["MUL r0 c0 c0", "DOWN r0 r0 #100"]
-    assert lines[-2:] == ["MUL r0 c0 r0", "DOWN r0 r0 #100"]
+    assert lines[0] == "W32 regs 2 root acc"
+    assert lines[1:3] == ["MUL - c0 c0", "DOWN r0 acc #100"]
+    assert lines[-2:] == ["MUL - c0 acc", "DOWN - acc #100"]
     # 25 terms: y = cast(x*x) is recomputed per use, registers stay at 2
     v = U.VmProgram(W.sin_expr(), [("x", 300, 100, 334)])
     # per Horner step: y (MUL, DOWN), MUL, DOWN, NEG, ADD; then x * s (MUL, DOWN)
@@ -49,7 +50,7 @@ def test_registers_bounded_by_tree_need(U):
     # the rewrite orders operands by scale: -d (scale 2) is evaluated
     # before the product (scale 4), so the product needs a second register
     assert U.VmProgram("((a+b)*c)-d", cols).listing == (
-        "W4 regs 2 root r0\nNEG r0 c3\nADD r1 c0 c1\nMUL r1 r1 c2\nADD r0 r0<<2 r1\n")
+        "W4 regs 2 root acc\nNEG r0 c3\nADD - c0 c1\nMUL - acc c2\nADD - r0<<2 acc\n")
 
 
 def test_serving_queries_are_seeded_and_fit(U):

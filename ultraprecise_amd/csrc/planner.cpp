@@ -1798,6 +1798,33 @@ bool build_vm_program(const Plan& p, const std::vector<ColumnType>& cols, VmProg
         r = {updec_vm::K_REG, (uint8_t)dst, 0};
     }
     prog.root = r.idx;
+    // forwarding: a result read only by the next instruction (or, for the
+    // last instruction, only as the root) stays in VGPRs and is not stored
+    auto& I = prog.ins;
+    for (size_t i = 0; i < I.size(); i++) {
+        const uint8_t d = I[i].dst;
+        std::vector<std::pair<size_t, int>> reads;   // (instruction, operand 0/1)
+        bool root_read = false;
+        size_t k = i + 1;
+        for (; k < I.size(); k++) {
+            const bool binary = I[k].op >= updec_vm::OP_ADD && I[k].op <= updec_vm::OP_DIV;
+            if (I[k].a.kind == updec_vm::K_REG && I[k].a.idx == d) reads.push_back({k, 0});
+            if (binary && I[k].b.kind == updec_vm::K_REG && I[k].b.idx == d) reads.push_back({k, 1});
+            if (I[k].dst == d) break;   // overwritten
+        }
+        if (k == I.size() && prog.root == d) root_read = true;
+        bool only_next = true;
+        for (auto& rd : reads) only_next = only_next && rd.first == i + 1;
+        if (root_read && i + 1 != I.size()) only_next = false;
+        if (!only_next || (reads.empty() && !root_read)) continue;
+        for (auto& rd : reads) {
+            updec_vm::Operand& o = rd.second == 0 ? I[rd.first].a : I[rd.first].b;
+            o.kind = updec_vm::K_LAST;
+            o.idx = 0;
+        }
+        if (root_read) prog.root = updec_vm::kNoStore;
+        I[i].dst = updec_vm::kNoStore;
+    }
     if (prog.nregs > updec_vm::kClassRegs[prog.wclass]) {
         *err = "expression needs more interpreter registers than its width class holds";
         return false;
@@ -1810,14 +1837,17 @@ std::string VmProgram::text() const
 {
     static const char* names[] = {"MOV", "NEG", "ADD", "SUB", "MUL", "DIV", "UP", "DOWN"};
     auto opnd = [](const updec_vm::Operand& o) {
-        std::string s = (o.kind == updec_vm::K_REG ? "r" : o.kind == updec_vm::K_COL ? "c" : "k") + std::to_string(o.idx);
+        std::string s = o.kind == updec_vm::K_LAST ? std::string("acc")
+                        : (o.kind == updec_vm::K_REG ? "r" : o.kind == updec_vm::K_COL ? "c" : "k") + std::to_string(o.idx);
         if (o.shift) s += "<<" + std::to_string(o.shift);
         return s;
     };
     std::ostringstream os;
-    os << "W" << W << " regs " << nregs << " root r" << root << "\n";
+    os << "W" << W << " regs " << nregs << " root " << (root == updec_vm::kNoStore ? std::string("acc") : "r" + std::to_string(root))
+       << "\n";
     for (const auto& I : ins) {
-        os << names[I.op] << " r" << (int)I.dst << " " << opnd(I.a);
+        os << names[I.op] << " " << (I.dst == updec_vm::kNoStore ? std::string("-") : "r" + std::to_string(I.dst)) << " "
+           << opnd(I.a);
         if (I.op >= updec_vm::OP_ADD && I.op <= updec_vm::OP_DIV) os << " " << opnd(I.b);
         if (I.op == updec_vm::OP_DIV || I.op == updec_vm::OP_UP || I.op == updec_vm::OP_DOWN) os << " #" << I.arg;
         os << "\n";

@@ -26,7 +26,11 @@ enum Op : uint8_t {
     OP_DOWN = 7,  // dst = floor(|a| / 10^arg), sign kept (cast to a smaller scale)
 };
 
-enum Kind : uint8_t { K_REG = 0, K_COL = 1, K_CONST = 2 };
+// K_LAST: the previous instruction's result, kept in VGPRs (the builder
+// forwards a value that only the next instruction reads and then does not
+// store it: dst = kNoStore; root = kNoStore reads the last result).
+enum Kind : uint8_t { K_REG = 0, K_COL = 1, K_CONST = 2, K_LAST = 3 };
+constexpr uint8_t kNoStore = 0xff;
 
 constexpr int kMaxRegs = 8;
 constexpr int kMaxCols = 8;

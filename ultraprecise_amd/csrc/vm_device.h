@@ -39,11 +39,13 @@ template <int W, int STRIDE> struct RegFile {
 
 // operand word: kind | idx << 8 | shift << 16
 template <int W, class RF>
-UPD_DEV Dec<W> fetch(u32 ow, const Query* q, const u32* cst, const RF& R, usize row)
+UPD_DEV Dec<W> fetch(u32 ow, const Query* q, const u32* cst, const RF& R, const Dec<W>& last, usize row)
 {
     const u32 kind = ow & 0xffu, idx = (ow >> 8) & 0xffu, shift = ow >> 16;
     Dec<W> v;
-    if (kind == K_REG) {
+    if (kind == K_LAST) {
+        v = last;
+    } else if (kind == K_REG) {
         v = R.get(idx);
     } else if (kind == K_COL) {
         const u32 B = q->colbytes[idx];
@@ -63,10 +65,13 @@ template <int W, class RF> UPD_DEV Dec<W> vm_eval(const Query* q, const u32* pro
 {
     const u32 n = q->nins;
     const bool faith = q->faithful != 0u;
+    Dec<W> last;
+    last.m = mzero<W>();
+    last.neg = 0u;
     for (u32 pc = 0; pc < n; pc++) {
         const u32 w0 = prog[3 * pc], w1 = prog[3 * pc + 1], w2 = prog[3 * pc + 2];
         const u32 op = w0 & 0xffu, dst = (w0 >> 8) & 0xffu, arg = w0 >> 16;
-        const Dec<W> a = fetch<W, RF>(w1, q, cst, R, row);
+        const Dec<W> a = fetch<W, RF>(w1, q, cst, R, last, row);
         Dec<W> r;
         if (op == OP_MOV) {
             r = a;
@@ -79,15 +84,16 @@ template <int W, class RF> UPD_DEV Dec<W> vm_eval(const Query* q, const u32* pro
             r.m = div_p10_rt<W>(a.m, arg);
             r.neg = a.neg;
         } else {
-            const Dec<W> b = fetch<W, RF>(w2, q, cst, R, row);
+            const Dec<W> b = fetch<W, RF>(w2, q, cst, R, last, row);
             if (op == OP_ADD) r = dadd<W, W, W>(a, b);
             else if (op == OP_SUB) r = dsub<W, W, W>(a, b);
             else if (op == OP_MUL) r = dmul<W, W, W>(a, b);
             else r = ddiv_rt<W>(a, b, arg, faith);
         }
-        R.set(dst, r);
+        if (dst != kNoStore) R.set(dst, r);
+        last = r;
     }
-    return R.get(q->root);
+    return q->root == kNoStore ? last : R.get(q->root);
 }
 
 }  // namespace updec_vm
