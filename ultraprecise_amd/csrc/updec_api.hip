@@ -3,6 +3,7 @@
 // statistics, group discovery, AVG division).
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <atomic>
@@ -754,10 +755,11 @@ static int agg_run_onepass(updec_agg_t a, const uint8_t* const* d_cols, const in
         HIPCHK(hipEventCreate(&a->ev0));
         HIPCHK(hipEventCreate(&a->ev1));
     }
-    HIPCHK(hipEventRecord(a->ev0, st));
+    // the events are attached to the dispatch itself: they time the kernel,
+    // not the host's enqueue latency in front of it
     void* pm[] = {&args, &ovf};
-    HIPCHK(hipModuleLaunchKernel(fmain, (unsigned)grid, 1, 1, T, 1, 1, 0, st, pm, nullptr));
-    HIPCHK(hipEventRecord(a->ev1, st));
+    HIPCHK(hipExtModuleLaunchKernel(fmain, (uint32_t)grid * (uint32_t)T, 1, 1, T, 1, 1, 0, st, pm, nullptr, a->ev0,
+                                    a->ev1, 0));
     a->timed = true;
     const u32* rec = a->d_partials;
     u32 nb = (u32)grid, maxg = (u32)max_groups;
@@ -861,10 +863,9 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
         HIPCHK(hipEventCreate(&a->ev0));
         HIPCHK(hipEventCreate(&a->ev1));
     }
-    HIPCHK(hipEventRecord(a->ev0, st));
     void* pm[] = {&args};
-    HIPCHK(hipModuleLaunchKernel(fmain, (unsigned)grid, 1, 1, T, 1, 1, 0, st, pm, nullptr));
-    HIPCHK(hipEventRecord(a->ev1, st));
+    HIPCHK(hipExtModuleLaunchKernel(fmain, (uint32_t)grid * (uint32_t)T, 1, 1, T, 1, 1, 0, st, pm, nullptr, a->ev0,
+                                    a->ev1, 0));
     a->timed = true;
     const u32* cp = a->d_partials;
     u32 nb = (u32)grid;
