@@ -308,7 +308,7 @@ def load_traffic():
     """HBM bytes per launch of the fused kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel
     calibration), when present."""
-    path = os.path.join(ROOT, "profiles", "agg_main_q1_60M_pmc.md")
+    path = os.path.join(ROOT, "profiles", "agg_keys_main_q1_60M_pmc.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -388,7 +388,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": load_traffic(),
-            "kernel": "agg_main", "kernel_ms": round(h["kernel_ms"], 4), "alg_bytes_per_launch": h["alg_bytes"],
+            "kernel": "agg_keys_main", "kernel_ms": round(h["kernel_ms"], 4), "alg_bytes_per_launch": h["alg_bytes"],
         },
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["mrows_per_s"], 4), "unit": "Mrows/s", "cores": cpu["threads"], "kind": "port",

@@ -366,7 +366,7 @@ struct updec_agg_s {
     AggSpec spec;
     std::vector<ColumnType> types;
     int nkeys = 0, has_filter = 0;
-    std::string base_name, last_src;
+    std::string last_src;
     struct GenMod {
         ModuleP mod;
         std::string src;
@@ -649,10 +649,17 @@ int updec_agg_compile(const updec_agg_desc* d, updec_agg_t* out)
     a->spec.filter_val = d->filter_value;
     a->nkeys = d->nkeys;
     a->spec.nkeys = d->nkeys;
-    a->base_name = "agg";
     a->onepass = agg_onepass_ok(a->spec, GenOptions());
     *out = a.release();
     return UPDEC_OK;
+}
+
+// kernel names by form, so profiles tell the forms apart: one-pass keyed
+// (agg_keys_*), two-pass through the key table (agg_lut_*), no grouping
+// (agg_sum_*)
+static std::string form_name(const updec_agg_s* a, bool onepass)
+{
+    return onepass ? "agg_keys" : (a->nkeys == 0 ? "agg_sum" : "agg_lut");
 }
 
 static int agg_module(updec_agg_t a, int ngroups, ModuleP* mod, int* block, std::string* err)
@@ -668,7 +675,7 @@ static int agg_module(updec_agg_t a, int ngroups, ModuleP* mod, int* block, std:
     s.ngroups = ngroups;
     GenOptions o;
     int aw = 0, blk = 256;
-    std::string src = gen_agg_kernels(s, a->types, a->base_name, o, &aw, &blk);
+    std::string src = gen_agg_kernels(s, a->types, form_name(a, false), o, &aw, &blk);
     if (src.empty()) {
         *err = "the accumulators of " + std::to_string(ngroups) + " groups do not fit in LDS";
         return UPDEC_EUNSUPPORTED;
@@ -725,14 +732,14 @@ static int agg_run_onepass(updec_agg_t a, const uint8_t* const* d_cols, const in
         s.onepass = true;
         GenOptions o;
         int aw = 0;
-        a->op_src = gen_agg_kernels(s, a->types, a->base_name, o, &aw, &a->op_block);
+        a->op_src = gen_agg_kernels(s, a->types, form_name(a, true), o, &aw, &a->op_block);
         if (a->op_src.empty()) return fail(UPDEC_EUNSUPPORTED, "one-pass accumulators do not fit in LDS");
         a->op_mod = ModuleCache::get().compile(a->op_src, &err);
         if (!a->op_mod) return fail(UPDEC_ECOMPILE, err);
     }
     a->last_src = a->op_src;
-    hipFunction_t fmain = a->op_mod->fn(a->base_name + "_main", &err);
-    hipFunction_t ffin = fmain ? a->op_mod->fn(a->base_name + "_final", &err) : nullptr;
+    hipFunction_t fmain = a->op_mod->fn(form_name(a, true) + "_main", &err);
+    hipFunction_t ffin = fmain ? a->op_mod->fn(form_name(a, true) + "_final", &err) : nullptr;
     if (!fmain || !ffin) return fail(UPDEC_ECOMPILE, err);
     const int T = a->op_block;
     const int64_t grid = agg_grid(fmain, rows, T);
@@ -843,8 +850,8 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
     int T = 256;
     int mrc = agg_module(a, ngroups, &mod, &T, &err);
     if (mrc != UPDEC_OK) return fail(mrc, err);
-    hipFunction_t fmain = mod->fn(a->base_name + "_main", &err);
-    hipFunction_t ffin = fmain ? mod->fn(a->base_name + "_final", &err) : nullptr;
+    hipFunction_t fmain = mod->fn(form_name(a, false) + "_main", &err);
+    hipFunction_t ffin = fmain ? mod->fn(form_name(a, false) + "_final", &err) : nullptr;
     if (!fmain || !ffin) return fail(UPDEC_ECOMPILE, err);
 
     const int NP = (int)a->spec.plans.size();
@@ -893,7 +900,7 @@ int updec_agg_source(updec_agg_t a, int32_t ngroups, char* buf, size_t cap, size
     s.ngroups = ngroups;
     GenOptions o;
     int aw = 0, blk = 0;
-    std::string src = gen_agg_kernels(s, a->types, a->base_name, o, &aw, &blk);
+    std::string src = gen_agg_kernels(s, a->types, form_name(a, s.onepass), o, &aw, &blk);
     if (src.empty()) return fail(UPDEC_EUNSUPPORTED, "accumulators do not fit in LDS");
     return copy_text(src, buf, cap, needed);
 }
