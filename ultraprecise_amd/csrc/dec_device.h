@@ -765,8 +765,13 @@ template <int B> UPD_DEV Dec<(B + 3) / 4> load_compact(const u8* __restrict__ co
     u32 dw[ND + 1];
     if ((d0 + ND) * 4u <= colbytes) {
         const u32* p = reinterpret_cast<const u32*>(col) + d0;
+        // the last dword is read only when the row reaches into it: a row
+        // that ends on a dword boundary would otherwise touch the next
+        // row's dword (and, at the end of a block's range, a cache line
+        // another block on another XCD fetches again)
+        const u32 nd = (sh + (u32)B + 3u) >> 2;
 #pragma unroll
-        for (int i = 0; i < ND; i++) dw[i] = __builtin_nontemporal_load(p + i);
+        for (int i = 0; i < ND; i++) dw[i] = (i < NW || (u32)i < nd) ? __builtin_nontemporal_load(p + i) : 0u;
     } else {
 #pragma unroll
         for (int i = 0; i < ND; i++) {

@@ -1217,7 +1217,16 @@ void emit_plan_functions(Emit& E, const std::vector<Plan>& plans, const std::vec
         if (plan_out_of_line(plans[i])) emit_plan_function(E, plans[i], cols, plan_fn_name(i));
 }
 
+// Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b mod 8)
+// and each XCD has its own L2.  xcd_block() renumbers the blocks so that XCD
+// x owns the logical range [x G/8, (x+1) G/8): the row ranges of one
+// grid-stride step are then contiguous per XCD, and a cache line that
+// straddles two blocks' ranges is fetched into one L2, not two.
 const char* kArgsStruct =
+    "__device__ __forceinline__ usize xcd_block() {\n"
+    "  const u32 g = gridDim.x, b = blockIdx.x;\n"
+    "  return (g & 7u) ? (usize)b : (usize)((b & 7u) * (g >> 3) + (b >> 3));\n"
+    "}\n"
     "#define UPD_MAXCOLS 16\n"
     "struct ColArgs { const u8* col[UPD_MAXCOLS]; usize colbytes[UPD_MAXCOLS]; usize rows; u8* out; };\n"
     "struct AggArgs { const u8* col[UPD_MAXCOLS]; usize colbytes[UPD_MAXCOLS]; usize rows;\n"
@@ -1236,7 +1245,7 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
     E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name << "(ColArgs a)\n{\n";
     E.os << "  __shared__ u32 tile[(" << T << " * " << OB << " + 3) / 4];\n";
     E.os << "  u8* t8 = reinterpret_cast<u8*>(tile);\n";
-    E.os << "  for (usize base = (usize)blockIdx.x * " << T << "; base < a.rows; base += (usize)gridDim.x * " << T << ") {\n";
+    E.os << "  for (usize base = xcd_block() * " << T << "; base < a.rows; base += (usize)gridDim.x * " << T << ") {\n";
     E.os << "    const usize row = base + threadIdx.x;\n";
     E.os << "    if (row < a.rows) {\n";
     std::vector<std::string> loaded(cols.size());
@@ -1366,9 +1375,30 @@ static void emit_agg_prelude(Emit& E, const AggSpec& s, const AccCfg& c, int nsl
 
 // per-row body shared by both forms: evaluate every plan and add it to the
 // slots at `sbase` (u64 index), then count the row
-static void emit_agg_row(Emit& E, const AggSpec& s, const std::vector<ColumnType>& cols, const std::string& ind)
+// Issue every column load of the row (filter value, keys, the decimal
+// columns of the inline plans) before anything depends on one of them, so
+// a row costs one memory round trip instead of a chain of filter -> key ->
+// value loads.  Rows the filter rejects have their lines read anyway.
+static void emit_row_preload(Emit& E, const AggSpec& s, const std::vector<ColumnType>& cols,
+                             std::vector<std::string>& loaded, const std::string& ind)
 {
-    std::vector<std::string> loaded(cols.size());
+    for (int i = 0; i < (int)s.plans.size(); i++) {
+        const Plan& p = s.plans[i];
+        if (plan_out_of_line(p)) continue;
+        for (const PNode& x : p.n) {
+            if (x.kind != PK::Col || !loaded[x.col].empty()) continue;
+            const int c = x.col;
+            const int B = compact_bytes(cols[c].P);
+            loaded[c] = "c" + std::to_string(c);
+            E.os << ind << "const Dec<" << (B + 3) / 4 << "> " << loaded[c] << " = load_compact<" << B << ">(a.col[" << c
+                 << "], row, a.colbytes[" << c << "]);\n";
+        }
+    }
+}
+
+static void emit_agg_row(Emit& E, const AggSpec& s, const std::vector<ColumnType>& cols,
+                         std::vector<std::string>& loaded, const std::string& ind)
+{
     for (int i = 0; i < (int)s.plans.size(); i++) {
         const Plan& p = s.plans[i];
         const int W = p.n[p.root].W;
@@ -1461,14 +1491,18 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     E.os << "  __syncthreads();\n";
     E.os << "  const u32 copy = wave * SPLIT + lane / (64 / SPLIT);\n";
     E.os << "  u32 ovf = 0;\n";
-    E.os << "  for (usize row = (usize)blockIdx.x * T + tid; row < a.rows; row += (usize)gridDim.x * T) {\n";
-    if (s.filter_col >= 0) E.os << "    if (!(a.fcol[row] " << fops[s.filter_op] << " (i64)" << s.filter_val << "LL)) continue;\n";
+    E.os << "  const usize lblk = xcd_block();\n";
+    E.os << "  for (usize row = lblk * T + tid; row < a.rows; row += (usize)gridDim.x * T) {\n";
+    if (s.filter_col >= 0) E.os << "    const i32 fv = a.fcol[row];\n";
     if (s.nkeys == 2) E.os << "    const u32 key = ((u32)a.key0[row] << 8) | a.key1[row];\n";
     else E.os << "    const u32 key = a.key0[row];\n";
+    std::vector<std::string> loaded(cols.size());
+    emit_row_preload(E, s, cols, loaded, "    ");
+    if (s.filter_col >= 0) E.os << "    if (!(fv " << fops[s.filter_op] << " (i64)" << s.filter_val << "LL)) continue;\n";
     E.os << "    const u32 g = slot_of(key, htab, &nslot, skey);\n";
     E.os << "    if (g >= MAXS) { ovf = 1u; continue; }\n";
     E.os << "    const u32 sbase = (copy * NS + g) * NP * AWR;\n";
-    emit_agg_row(E, s, cols, "    ");
+    emit_agg_row(E, s, cols, loaded, "    ");
     E.os << "    atomicAdd(&acnt[copy * NS + g], 1u);\n";
     E.os << "  }\n";
     E.os << "  if (ovf) atomicOr(overflow, 1u);\n";
@@ -1562,10 +1596,17 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
         for (int i = 0; i < NP; i++) E.os << "  Mag<AW1> acc" << i << " = mzero<AW1>();\n";
         E.os << "  u32 cnt = 0;\n";
     }
-    E.os << "  for (usize row = (usize)blockIdx.x * T + tid; row < a.rows; row += (usize)gridDim.x * T) {\n";
-    if (s.filter_col >= 0) E.os << "    if (!(a.fcol[row] " << fops[s.filter_op] << " (i64)" << s.filter_val << "LL)) continue;\n";
+    E.os << "  const usize lblk = xcd_block();\n";
+    E.os << "  for (usize row = lblk * T + tid; row < a.rows; row += (usize)gridDim.x * T) {\n";
+    if (s.filter_col >= 0) E.os << "    const i32 fv = a.fcol[row];\n";
+    std::vector<std::string> loaded(cols.size());
+    if (!reg) {
+        if (s.nkeys == 2) E.os << "    const u32 key = ((u32)a.key0[row] << 8) | a.key1[row];\n";
+        else E.os << "    const u32 key = a.key0[row];\n";
+    }
+    emit_row_preload(E, s, cols, loaded, "    ");
+    if (s.filter_col >= 0) E.os << "    if (!(fv " << fops[s.filter_op] << " (i64)" << s.filter_val << "LL)) continue;\n";
     if (reg) {
-        std::vector<std::string> loaded(cols.size());
         for (int i = 0; i < NP; i++) {
             const Plan& p = s.plans[i];
             const int W = p.n[p.root].W;
@@ -1574,10 +1615,9 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
         }
         E.os << "    cnt += 1u;\n  }\n";
     } else {
-        if (s.nkeys == 2) E.os << "    const u32 g = a.lut[((u32)a.key0[row] << 8) | a.key1[row]];\n";
-        else E.os << "    const u32 g = a.lut[a.key0[row]];\n";
+        E.os << "    const u32 g = a.lut[key];\n";
         E.os << "    const u32 sbase = (copy * NS + g) * NP * AWR;\n";
-        emit_agg_row(E, s, cols, "    ");
+        emit_agg_row(E, s, cols, loaded, "    ");
         E.os << "    atomicAdd(&acnt[copy * NS + g], 1u);\n  }\n";
     }
     E.os << "  __shared__ u32 red[NWAVE * PW];\n";
