@@ -67,3 +67,24 @@ def test_config1_fused_sum_bit_exact(U, O, gpu, rows):
         want, n = O.plan_sum(O.Plan(e.plan), host, [c[1] for c in cols], [c[2] for c in cols], rows)
         assert n == rows
         assert bytes(got[k]) == want, f"SUM({text}): {O.image_value(got[k])} vs {O.image_value(want)}"
+
+
+# output row widths: 4 (dword stores), 8 (dwordx2), 16 and 32 (dwordx4), 5
+# and 17 (LDS tile); each at a 16-byte aligned output and at one 4 bytes past
+# it, where only the tile path may run
+@pytest.mark.parametrize("P,S,ob", [(8, 2, 4), (17, 4, 8), (37, 9, 16), (75, 20, 32), (9, 2, 5), (38, 9, 17)])
+@pytest.mark.parametrize("offset", [0, 4])
+def test_calc_output_store_paths(U, O, gpu, P, S, ob, offset):
+    rows = 70_001
+    cols = [("a", P, S), ("b", P, S)]
+    dev = [U.gen_compact_device(W.digits_desc(U, 11, i, P), rows) for i in range(2)]
+    host = [t.numpy(count=rows * U.compact_bytes(P)) for t in dev]
+    e = U.Expr("a+b", cols)
+    assert e.out_bytes == ob
+    buf = U.DeviceBuffer(rows * ob + 64).zero()
+    e.calc(dev, rows, U.DeviceView(buf, offset, rows * ob))
+    full = buf.numpy()
+    got = full[offset: offset + rows * ob]
+    want = O.plan_eval(O.Plan(e.plan), host, [P, P], [S, S], rows, ob)
+    assert np.array_equal(got, want), f"first mismatch row {np.argmax(got != want) // ob}"
+    assert not full[:offset].any() and not full[offset + rows * ob:].any(), "kernel wrote outside the output"

@@ -796,6 +796,63 @@ template <int B> UPD_DEV Dec<(B + 3) / 4> load_compact(const u8* __restrict__ co
     return r;
 }
 
+// ---- wave-staged loads
+//
+// A wave's 64 consecutive rows of a B-byte compact column are 64 B
+// contiguous bytes (4 B 16-byte vectors).  Loading them with 16-byte vector
+// loads into LDS and extracting each lane's row there reads the column at
+// the streaming rate; the per-lane covering-dword loads above spread one
+// instruction over 64 rows B bytes apart and reach about 75% of it.
+
+// words of a wave's staging area for one column (2 words of slack for the
+// covering reads of the last lane)
+template <int B> constexpr int stage_words() { return 16 * B + 2; }
+
+typedef unsigned int u32x4 __attribute__((vector_size(16)));
+typedef unsigned int u32x2 __attribute__((vector_size(8)));
+
+// stage rows [base, base + 64) of the column; base is a multiple of 64 and
+// the column start is 16-byte aligned (checked by the caller), so the span
+// starts on a 16-byte boundary
+template <int B> UPD_DEV void stage_wave(const u8* __restrict__ col, usize base, u32* lds, u32 lane)
+{
+    constexpr int NV = 4 * B;   // 64 B bytes / 16
+    const u32x4* src = reinterpret_cast<const u32x4*>(col + base * (usize)B);
+#pragma unroll
+    for (int k = 0; k < (NV + 63) / 64; k++) {
+        const u32 i = lane + 64u * (u32)k;
+        if (i < (u32)NV) {
+            const u32x4 v = __builtin_nontemporal_load(src + i);
+            lds[4 * i] = v[0];
+            lds[4 * i + 1] = v[1];
+            lds[4 * i + 2] = v[2];
+            lds[4 * i + 3] = v[3];
+        }
+    }
+}
+
+// this lane's row from the staged bytes (same decoding as load_compact)
+template <int B> UPD_DEV Dec<(B + 3) / 4> load_staged(const u32* lds, u32 lane)
+{
+    constexpr int ND = (B + 6) / 4;
+    constexpr int NW = (B + 3) / 4;
+    const u32 off = lane * (u32)B;
+    const u32 d0 = off >> 2;
+    const u32 sh = off & 3u;
+    u32 dw[ND + 1];
+#pragma unroll
+    for (int i = 0; i < ND; i++) dw[i] = lds[d0 + i];
+    dw[ND] = 0u;
+    Dec<NW> r;
+#pragma unroll
+    for (int i = 0; i < NW; i++) r.m.w[i] = __builtin_amdgcn_alignbyte(dw[i + 1], dw[i], sh);
+    if constexpr ((B % 4) != 0) r.m.w[NW - 1] &= (1u << (8 * (B % 4))) - 1u;
+    constexpr int SB = 8 * ((B - 1) % 4) + 7;
+    r.neg = (r.m.w[NW - 1] >> SB) & 1u;
+    r.m.w[NW - 1] &= ~(1u << SB);
+    return r;
+}
+
 // Encode (decimal.cu:337-341): the low B bytes of the magnitude with the sign
 // OR-ed into bit 7 of the last byte.  Returns the bytes as words.
 template <int B, int W> UPD_DEV Mag<(B + 3) / 4> encode_compact(const Dec<W>& a)

@@ -621,7 +621,7 @@ int updec_expr_calc(updec_expr_t e, const uint8_t* const* d_cols, int64_t rows, 
     a.rows = (usize)rows;
     a.out = d_out;
     int64_t tiles = (rows + 255) / 256;
-    unsigned grid = (unsigned)std::min<int64_t>(tiles, (int64_t)device_cus() * 16);
+    unsigned grid = (unsigned)std::min<int64_t>(tiles, (int64_t)device_cus() * 32);
     void* params[] = {&a};
     HIPCHK(hipModuleLaunchKernel(e->fn, grid, 1, 1, 256, 1, 1, 0, S(stream), params, nullptr));
     return UPDEC_OK;
