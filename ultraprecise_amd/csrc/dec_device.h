@@ -853,6 +853,85 @@ template <int B> UPD_DEV Dec<(B + 3) / 4> load_staged(const u32* lds, u32 lane)
     return r;
 }
 
+// ---- row quads
+//
+// Four consecutive rows of a B-byte compact column are 4 B bytes: B whole
+// dwords, dword aligned when the column is.  A lane that owns rows
+// [4q, 4q + 4) loads those B dwords with 16-byte vector loads (consecutive
+// lanes 4 B bytes apart, every byte fetched belongs to the lane) and
+// decodes its four rows from registers at byte offsets fixed at compile
+// time once the caller's loop over the four rows is unrolled.
+
+typedef unsigned int u32x4u __attribute__((vector_size(16), aligned(4)));
+typedef unsigned int u32x2u __attribute__((vector_size(8), aligned(4)));
+
+template <int B> struct Quad {
+    u32 w[B];
+};
+
+// dwords [q B, q B + B) of a column of colbytes bytes; in the column's last,
+// partial quad the dwords past its end read as zero and the covering dword
+// of its last bytes is assembled byte by byte
+template <int B> UPD_DEV Quad<B> load_quad(const u8* __restrict__ col, usize q, usize colbytes)
+{
+    Quad<B> r;
+    const usize d0 = q * (usize)B;
+    const u32* p = reinterpret_cast<const u32*>(col) + d0;
+    if ((d0 + (usize)B) * 4u <= colbytes) {
+#pragma unroll
+        for (int i = 0; i + 4 <= B; i += 4) {
+            const u32x4u v = __builtin_nontemporal_load(reinterpret_cast<const u32x4u*>(p + i));
+            r.w[i] = v[0];
+            r.w[i + 1] = v[1];
+            r.w[i + 2] = v[2];
+            r.w[i + 3] = v[3];
+        }
+        if constexpr ((B & 3) >= 2) {
+            const u32x2u v = __builtin_nontemporal_load(reinterpret_cast<const u32x2u*>(p + (B & ~3)));
+            r.w[B & ~3] = v[0];
+            r.w[(B & ~3) + 1] = v[1];
+        }
+        if constexpr ((B & 1) != 0) r.w[B - 1] = __builtin_nontemporal_load(p + (B - 1));
+    } else {
+#pragma unroll
+        for (int i = 0; i < B; i++) {
+            u32 x = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const usize o = (d0 + (usize)i) * 4u + (usize)b;
+                x |= (o < colbytes ? (u32)col[o] : 0u) << (8 * b);
+            }
+            r.w[i] = x;
+        }
+    }
+    return r;
+}
+
+// row k (0..3) of a quad of a B-byte column (decoding as load_compact)
+template <int B> UPD_DEV Dec<(B + 3) / 4> quad_row(const Quad<B>& q, u32 k)
+{
+    constexpr int ND = (B + 6) / 4;
+    constexpr int NW = (B + 3) / 4;
+    const u32 off = k * (u32)B;
+    const u32 d0 = off >> 2;
+    const u32 sh = off & 3u;
+    u32 dw[ND + 1];
+#pragma unroll
+    for (int i = 0; i < ND; i++) dw[i] = (d0 + (u32)i < (u32)B) ? q.w[d0 + i] : 0u;
+    dw[ND] = 0u;
+    Dec<NW> r;
+#pragma unroll
+    for (int i = 0; i < NW; i++) r.m.w[i] = __builtin_amdgcn_alignbyte(dw[i + 1], dw[i], sh);
+    if constexpr ((B % 4) != 0) r.m.w[NW - 1] &= (1u << (8 * (B % 4))) - 1u;
+    constexpr int SB = 8 * ((B - 1) % 4) + 7;
+    r.neg = (r.m.w[NW - 1] >> SB) & 1u;
+    r.m.w[NW - 1] &= ~(1u << SB);
+    return r;
+}
+
+// byte k of the quad of a byte column
+UPD_DEV u32 quad_byte(const Quad<1>& q, u32 k) { return (q.w[0] >> (8u * k)) & 255u; }
+
 // Encode (decimal.cu:337-341): the low B bytes of the magnitude with the sign
 // OR-ed into bit 7 of the last byte.  Returns the bytes as words.
 template <int B, int W> UPD_DEV Mag<(B + 3) / 4> encode_compact(const Dec<W>& a)

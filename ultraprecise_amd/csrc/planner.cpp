@@ -1308,6 +1308,68 @@ static void emit_row_preload(Emit& E, const AggSpec& s, const std::vector<Column
     }
 }
 
+// quad registers of the inline columns, -1 when one is too wide
+static int quad_words(const std::vector<int>& incols, const std::vector<ColumnType>& cols)
+{
+    int w = 0;
+    for (int c : incols) {
+        const int B = compact_bytes(cols[c].P);
+        if (B > kQuadMaxBytes) return -1;
+        w += B;
+    }
+    return w;
+}
+
+// Opens the row loop of a generated kernel (needs tid, lblk, and the stage
+// declarations when sp stages columns).  The body that follows sees `row`,
+// `fv` (filter value), `key` (when keys) and the inline columns named in
+// `loaded`, and may `continue` to drop its row.  rps = 4: the lane's row
+// quad, columns loaded as quads and decoded per row.  Returns the body's
+// indentation.
+static std::string emit_row_loop_open(Emit& E, const AggSpec& s, const std::vector<ColumnType>& cols, int rps,
+                                      const StagePlan& sp, bool keys, std::vector<std::string>& loaded)
+{
+    if (rps == 1) {
+        E.os << "  for (usize row = lblk * T + tid; row < a.rows; row += (usize)gridDim.x * T) {\n";
+        if (s.filter_col >= 0) E.os << "    const i32 fv = a.fcol[row];\n";
+        if (keys) {
+            if (s.nkeys == 2) E.os << "    const u32 key = ((u32)a.key0[row] << 8) | a.key1[row];\n";
+            else E.os << "    const u32 key = a.key0[row];\n";
+        }
+        emit_row_preload(E, s, cols, sp, loaded, "    ");
+        return "    ";
+    }
+    E.os << "  for (usize q = lblk * T + tid; q * 4u < a.rows; q += (usize)gridDim.x * T) {\n";
+    if (s.filter_col >= 0)
+        E.os << "    const Quad<4> fq = load_quad<4>(reinterpret_cast<const u8*>(a.fcol), q, a.rows * 4u);\n";
+    if (keys) {
+        E.os << "    const Quad<1> kq0 = load_quad<1>(a.key0, q, a.rows);\n";
+        if (s.nkeys == 2) E.os << "    const Quad<1> kq1 = load_quad<1>(a.key1, q, a.rows);\n";
+    }
+    const std::vector<int> inl = inline_columns(s);
+    for (int c : inl) {
+        const int B = compact_bytes(cols[c].P);
+        E.os << "    const Quad<" << B << "> q" << c << " = load_quad<" << B << ">(a.col[" << c << "], q, a.colbytes[" << c
+             << "]);\n";
+    }
+    E.os << "#pragma unroll\n    for (u32 k = 0; k < 4u; k++) {\n";
+    E.os << "      const usize row = q * 4u + k;\n";
+    E.os << "      if (row >= a.rows) continue;\n";
+    if (s.filter_col >= 0) E.os << "      const i32 fv = (i32)fq.w[k];\n";
+    if (keys) {
+        if (s.nkeys == 2) E.os << "      const u32 key = (quad_byte(kq0, k) << 8) | quad_byte(kq1, k);\n";
+        else E.os << "      const u32 key = quad_byte(kq0, k);\n";
+    }
+    for (int c : inl) {
+        const int B = compact_bytes(cols[c].P);
+        loaded[c] = "c" + std::to_string(c);
+        E.os << "      const Dec<" << (B + 3) / 4 << "> " << loaded[c] << " = quad_row<" << B << ">(q" << c << ", k);\n";
+    }
+    return "      ";
+}
+
+static void emit_row_loop_close(Emit& E, int rps) { E.os << (rps == 4 ? "    }\n  }\n" : "  }\n"); }
+
 }  // namespace
 
 std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, const std::string& name, const GenOptions& o)
@@ -1369,6 +1431,16 @@ static int agg_value_bits(const AggSpec& s)
 }
 
 int agg_acc_words(const AggSpec& s) { return words_for(agg_value_bits(s) + 40 + 1); }
+
+int agg_rows_per_step(const AggSpec& s, const std::vector<ColumnType>& cols, const GenOptions& o)
+{
+    if (!o.row_quads) return 1;
+    const std::vector<int> inl = inline_columns(s);
+    const int w = quad_words(inl, cols);
+    const int extra = (s.filter_col >= 0 ? 4 : 0) + s.nkeys;
+    if (w < 0 || w + extra == 0 || w + extra > kQuadMaxWords) return 1;
+    return 4;
+}
 
 // rows that can reach one slot copy: 2^14 per lane times the lanes sharing it
 static int share_log_for(int split) { return kAggRowsPerLaneLog2 + (split >= 64 ? 0 : 6 - __builtin_ctz((unsigned)split)); }
@@ -1578,19 +1650,16 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     E.os << "  const u32 copy = wave * SPLIT + lane / (64 / SPLIT);\n";
     E.os << "  u32 ovf = 0;\n";
     E.os << "  const usize lblk = xcd_block();\n";
-    E.os << "  for (usize row = lblk * T + tid; row < a.rows; row += (usize)gridDim.x * T) {\n";
-    if (s.filter_col >= 0) E.os << "    const i32 fv = a.fcol[row];\n";
-    if (s.nkeys == 2) E.os << "    const u32 key = ((u32)a.key0[row] << 8) | a.key1[row];\n";
-    else E.os << "    const u32 key = a.key0[row];\n";
     std::vector<std::string> loaded(cols.size());
-    emit_row_preload(E, s, cols, sp, loaded, "    ");
-    if (s.filter_col >= 0) E.os << "    if (!(fv " << fops[s.filter_op] << " (i64)" << s.filter_val << "LL)) continue;\n";
-    E.os << "    const u32 g = slot_of(key, htab, &nslot, skey);\n";
-    E.os << "    if (g >= MAXS) { ovf = 1u; continue; }\n";
-    E.os << "    const u32 sbase = (copy * NS + g) * NP * AWR;\n";
-    emit_agg_row(E, s, cols, loaded, "    ");
-    E.os << "    atomicAdd(&acnt[copy * NS + g], 1u);\n";
-    E.os << "  }\n";
+    const int rps = agg_rows_per_step(s, cols, o);
+    const std::string bi = emit_row_loop_open(E, s, cols, rps, sp, true, loaded);
+    if (s.filter_col >= 0) E.os << bi << "if (!(fv " << fops[s.filter_op] << " (i64)" << s.filter_val << "LL)) continue;\n";
+    E.os << bi << "const u32 g = slot_of(key, htab, &nslot, skey);\n";
+    E.os << bi << "if (g >= MAXS) { ovf = 1u; continue; }\n";
+    E.os << bi << "const u32 sbase = (copy * NS + g) * NP * AWR;\n";
+    emit_agg_row(E, s, cols, loaded, bi);
+    E.os << bi << "atomicAdd(&acnt[copy * NS + g], 1u);\n";
+    emit_row_loop_close(E, rps);
     E.os << "  if (ovf) atomicOr(overflow, 1u);\n";
     E.os << "  __syncthreads();\n";
     E.os << "  u32* rec = a.partials + (usize)blockIdx.x * (MAXS * EW);\n";
@@ -1672,7 +1741,7 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
     E.os << "#define AW1 " << AW1 << "\n";
     E.os << "extern \"C\" __global__ void __launch_bounds__(" << c.T << ") " << name << "_main(AggArgs a)\n{\n";
     E.os << "  const u32 tid = threadIdx.x;\n  const u32 wave = tid >> 6, lane = tid & 63u;\n";
-    const StagePlan sp = reg ? stage_plan(s, cols) : StagePlan();
+    const StagePlan sp = (reg && agg_rows_per_step(s, cols, o) == 1) ? stage_plan(s, cols) : StagePlan();
     emit_stage_decl(E, sp, c.T / 64);
     if (!reg) {
         E.os << "  __shared__ u64 aslot[NCOPY * NS * NP * AWR];\n  __shared__ u32 acnt[NCOPY * NS];\n";
@@ -1685,29 +1754,25 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
         E.os << "  u32 cnt = 0;\n";
     }
     E.os << "  const usize lblk = xcd_block();\n";
-    E.os << "  for (usize row = lblk * T + tid; row < a.rows; row += (usize)gridDim.x * T) {\n";
-    if (s.filter_col >= 0) E.os << "    const i32 fv = a.fcol[row];\n";
     std::vector<std::string> loaded(cols.size());
-    if (!reg) {
-        if (s.nkeys == 2) E.os << "    const u32 key = ((u32)a.key0[row] << 8) | a.key1[row];\n";
-        else E.os << "    const u32 key = a.key0[row];\n";
-    }
-    emit_row_preload(E, s, cols, sp, loaded, "    ");
-    if (s.filter_col >= 0) E.os << "    if (!(fv " << fops[s.filter_op] << " (i64)" << s.filter_val << "LL)) continue;\n";
+    const int rps = agg_rows_per_step(s, cols, o);
+    const std::string bi = emit_row_loop_open(E, s, cols, rps, sp, !reg, loaded);
+    if (s.filter_col >= 0) E.os << bi << "if (!(fv " << fops[s.filter_op] << " (i64)" << s.filter_val << "LL)) continue;\n";
     if (reg) {
         for (int i = 0; i < NP; i++) {
             const Plan& p = s.plans[i];
             const int W = p.n[p.root].W;
-            std::string rv = emit_plan_any(E, p, cols, loaded, "    ", "p" + std::to_string(i) + "_", plan_fn_name(i));
-            E.os << "    acc_add<AW1, " << W << ">(acc" << i << ", stored<" << p.outB << ", " << W << ">(" << rv << "));\n";
+            std::string rv = emit_plan_any(E, p, cols, loaded, bi, "p" + std::to_string(i) + "_", plan_fn_name(i));
+            E.os << bi << "acc_add<AW1, " << W << ">(acc" << i << ", stored<" << p.outB << ", " << W << ">(" << rv << "));\n";
         }
-        E.os << "    cnt += 1u;\n  }\n";
+        E.os << bi << "cnt += 1u;\n";
     } else {
-        E.os << "    const u32 g = a.lut[key];\n";
-        E.os << "    const u32 sbase = (copy * NS + g) * NP * AWR;\n";
-        emit_agg_row(E, s, cols, loaded, "    ");
-        E.os << "    atomicAdd(&acnt[copy * NS + g], 1u);\n  }\n";
+        E.os << bi << "const u32 g = a.lut[key];\n";
+        E.os << bi << "const u32 sbase = (copy * NS + g) * NP * AWR;\n";
+        emit_agg_row(E, s, cols, loaded, bi);
+        E.os << bi << "atomicAdd(&acnt[copy * NS + g], 1u);\n";
     }
+    emit_row_loop_close(E, rps);
     E.os << "  __shared__ u32 red[NWAVE * PW];\n";
     if (!reg) {
         // combine the slot copies of the block: one thread per (group, plan)

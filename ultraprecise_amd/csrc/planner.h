@@ -124,7 +124,20 @@ struct GenOptions {
     // grouped aggregation: slot copies per wave for the LDS atomic
     // accumulators (1 = one set shared by the 64 lanes, 64 = per lane)
     int slot_split = 4;
+    // aggregation row loops take a lane's four consecutive rows per step
+    // ("row quads") when the columns they load are narrow enough
+    bool row_quads = true;
 };
+
+// Row quads (aggregation row loops): a lane owns rows [4q, 4q + 4) and
+// loads each narrow column as the 4 B whole dwords those rows occupy.  Used
+// when every column the row loop loads inline is at most kQuadMaxBytes wide
+// and their quads together (with the filter and key quads) take at most
+// kQuadMaxWords registers.  The calc kernel keeps one row per lane: its
+// rows of 8 and 16 bytes are already whole-dword loads and stores, and
+// output quads measured slower there.
+constexpr int kQuadMaxBytes = 16;
+constexpr int kQuadMaxWords = 40;
 
 // The fused aggregation launch keeps every lane at <= 2^kAggRowsPerLaneLog2
 // rows; the generated accumulator widths rely on it.
@@ -164,6 +177,8 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
                             const GenOptions& o, int* acc_words, int* block_out);
 // limbs of one block partial of the fused aggregation
 int agg_acc_words(const AggSpec& s);
+// rows a lane takes per step of the aggregation row loop (1 or 4)
+int agg_rows_per_step(const AggSpec& s, const std::vector<ColumnType>& cols, const GenOptions& o);
 
 // ---- the precompiled interpreter (vm.h) ----
 // Register program for one plan: the plan tree is expanded (shared nodes

@@ -705,15 +705,18 @@ static void fill_args(updec_agg_t a, AggArgs* args, const uint8_t* const* d_cols
     args->partials = a->d_partials;
 }
 
-static int64_t agg_grid(hipFunction_t f, int64_t rows, int T)
+// rps: rows a lane takes per loop step (1, or 4 for row quads)
+static int64_t agg_grid(hipFunction_t f, int64_t rows, int T, int rps)
 {
-    int64_t tiles = std::max<int64_t>(1, (rows + T - 1) / T);
+    const int64_t steps = (rows + rps - 1) / rps;
+    int64_t tiles = std::max<int64_t>(1, (steps + T - 1) / T);
     int blocks_per_cu = 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, f, T, 0) != hipSuccess || blocks_per_cu < 1)
         blocks_per_cu = 2;
     int64_t grid = std::min<int64_t>(tiles, (int64_t)device_cus() * blocks_per_cu);
-    const int64_t per_block = (int64_t)T << kAggRowsPerLaneLog2;
-    int64_t min_grid = (rows + per_block - 1) / per_block;  // every lane <= 2^14 rows
+    // every lane <= 2^14 rows: at most 2^12 quads (or 2^14 single rows)
+    const int64_t per_block = (int64_t)T << (kAggRowsPerLaneLog2 - 2);
+    const int64_t min_grid = ((rows + 3) / 4 + per_block - 1) / per_block;
     return std::max(grid, std::max<int64_t>(min_grid, 1));
 }
 
@@ -742,7 +745,7 @@ static int agg_run_onepass(updec_agg_t a, const uint8_t* const* d_cols, const in
     hipFunction_t ffin = fmain ? a->op_mod->fn(form_name(a, true) + "_final", &err) : nullptr;
     if (!fmain || !ffin) return fail(UPDEC_ECOMPILE, err);
     const int T = a->op_block;
-    const int64_t grid = agg_grid(fmain, rows, T);
+    const int64_t grid = agg_grid(fmain, rows, T, agg_rows_per_step(a->spec, a->types, GenOptions()));
     const size_t need = (size_t)grid * (size_t)agg_onepass_record_words(a->spec);
     if (need > a->partial_words) {
         if (a->d_partials) HIPCHK(hipFree(a->d_partials));
@@ -857,7 +860,7 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
     const int NP = (int)a->spec.plans.size();
     const int aw = agg_acc_words(a->spec);
     const size_t PW = (size_t)NP * ngroups * aw + 2 * (size_t)ngroups;
-    const int64_t grid = agg_grid(fmain, rows, T);
+    const int64_t grid = agg_grid(fmain, rows, T, agg_rows_per_step(a->spec, a->types, GenOptions()));
     if ((size_t)grid * PW > a->partial_words) {
         if (a->d_partials) HIPCHK(hipFree(a->d_partials));
         a->d_partials = nullptr;
