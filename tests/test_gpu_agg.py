@@ -112,13 +112,16 @@ def test_grouping_many_keys_and_fallback(U, O, gpu, nf1, nf2, nkeys):
         assert np.array_equal(got[:ng, k], want), text
 
 
-def test_div_int64_edges(U, O, gpu):
-    """AVG division on crafted sums: zero, negative, path (i) wrap region, wide sums, zero count"""
+@pytest.mark.parametrize("mode", ["faithful", "extended"])
+def test_div_int64_edges(U, O, gpu, mode):
+    """AVG division on crafted sums: zero, negative, path (i) wrap region,
+    sums of exactly 8 limbs and wider (the kernel's two division widths),
+    zero count"""
     rng = np.random.default_rng(5)
     imgs, cnts = [], []
-    for i in range(400):
+    for i in range(420):
         d = O.OdDec()
-        k = i % 5
+        k = i % 6
         if k == 0:
             m = 0
         elif k == 1:
@@ -127,6 +130,8 @@ def test_div_int64_edges(U, O, gpu):
             m = int(rng.integers(1 << 48, 1 << 51))
         elif k == 3:
             m = int.from_bytes(rng.bytes(40), "little")
+        elif k == 4:
+            m = int.from_bytes(rng.bytes(32), "little") | (1 << 255)
         else:
             m = int(rng.integers(1 << 60, 1 << 63))
         d.set_magnitude(m)
@@ -139,10 +144,11 @@ def test_div_int64_edges(U, O, gpu):
     src = U.DeviceBuffer.from_numpy(np.frombuffer(b"".join(imgs), dtype=np.uint8))
     cnt = U.DeviceBuffer.from_numpy(np.array(cnts, dtype=np.int64))
     out = U.DeviceBuffer(n * 133).zero()
-    U.div_int64(src, 133, cnt, out, n, mode=U.MODE_FAITHFUL)
+    um, om = (U.MODE_FAITHFUL, O.FAITHFUL) if mode == "faithful" else (U.MODE_EXTENDED, O.EXTENDED)
+    U.div_int64(src, 133, cnt, out, n, mode=um)
     got = out.numpy().reshape(n, 133)
     for i in range(n):
-        assert bytes(got[i]) == O.avg_from_sum(imgs[i], cnts[i], mode=O.FAITHFUL), i
+        assert bytes(got[i]) == O.avg_from_sum(imgs[i], cnts[i], mode=om), i
 
 
 @pytest.mark.parametrize("P,S", [(15, 2), (38, 10), (76, 20)])

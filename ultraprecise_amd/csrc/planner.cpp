@@ -1662,35 +1662,53 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     emit_row_loop_close(E, rps);
     E.os << "  if (ovf) atomicOr(overflow, 1u);\n";
     E.os << "  __syncthreads();\n";
-    E.os << "  u32* rec = a.partials + (usize)blockIdx.x * (MAXS * EW);\n";
+    // records are stored word-major: word f of slot sl of block b at
+    // partials[(sl * EW + f) * nblocks + b], so the final kernel's scans
+    // over the blocks read consecutive words
+    E.os << "  const usize nb = gridDim.x;\n";
     E.os << "  const u32 ns = nslot < MAXS ? nslot : MAXS;\n";
+    // overflow[1]: the most slots any block used (the final kernel scans only those)
+    E.os << "  if (tid == 0 && ns) atomicMax(overflow + 1, ns);\n";
     E.os << "  for (u32 k = tid; k < MAXS * (NP + 1); k += T) {\n";
     E.os << "    const u32 sl = k / (NP + 1), pp = k % (NP + 1);\n";
-    E.os << "    u32* e = rec + sl * EW;\n";
+    E.os << "    u32* e = a.partials + (usize)sl * EW * nb + blockIdx.x;\n";
     E.os << "    if (pp == 0) e[0] = sl < ns ? skey[sl] : 0u;\n";
     E.os << "    if (pp < NP) {\n      Mag<AW2> sm = mzero<AW2>();\n";
     E.os << "      if (sl < ns) for (u32 cc = 0; cc < NCOPY; cc++) add_to<AW2>(sm, sm, slot_value(&aslot[((cc * NS + sl) * NP + pp) * AWR]));\n";
-    E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) e[1 + pp * AW2 + w] = sm.w[w];\n";
+    E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) e[(usize)(1 + pp * AW2 + w) * nb] = sm.w[w];\n";
     E.os << "    } else {\n      u64 cnt = 0;\n";
     E.os << "      if (sl < ns) for (u32 cc = 0; cc < NCOPY; cc++) cnt += acnt[cc * NS + sl];\n";
-    E.os << "      e[1 + NP * AW2] = (u32)cnt; e[2 + NP * AW2] = (u32)(cnt >> 32);\n    }\n  }\n}\n";
+    E.os << "      e[(usize)(1 + NP * AW2) * nb] = (u32)cnt; e[(usize)(2 + NP * AW2) * nb] = (u32)(cnt >> 32);\n    }\n  }\n}\n";
 
     // final: meta[0] = ngroups, meta[1] = 1 when ngroups > maxg, meta[2..] = keys
+    E.os << "UPD_DEV usize rec_at(u32 i, u32 nblocks) { const u32 sl = i / nblocks; return (usize)sl * EW * nblocks + (i - sl * nblocks); }\n";
     E.os << "extern \"C\" __global__ void __launch_bounds__(256) " << name
-         << "_final(const u32* __restrict__ rec, u32 nblocks, u32 maxg, u8* __restrict__ images, u64* __restrict__ counts, u32* __restrict__ meta)\n{\n";
+         << "_final(const u32* __restrict__ rec, u32 nblocks, const u32* __restrict__ used, u32 maxg, u8* __restrict__ images, u64* __restrict__ counts, u32* __restrict__ meta)\n{\n";
     E.os << "  const u32 tid = threadIdx.x;\n";
-    E.os << "  __shared__ u32 bm[2048];\n  __shared__ u32 pre[2049];\n  __shared__ u32 part[256];\n";
+    E.os << "  __shared__ u32 bm[2048];\n  __shared__ u32 pre[2049];\n  __shared__ u32 part[4];\n";
     E.os << "  __shared__ u32 red[4 * AW2 + 8];\n  __shared__ u32 gkey;\n";
     E.os << "  for (u32 i = tid; i < 2048; i += 256) bm[i] = 0u;\n  __syncthreads();\n";
-    E.os << "  const u32 nrec = nblocks * MAXS;\n";
-    E.os << "  for (u32 i = tid; i < nrec; i += 256) { const u32 k1 = rec[(usize)i * EW]; if (k1) atomicOr(&bm[(k1 - 1u) >> 5], 1u << ((k1 - 1u) & 31u)); }\n";
+    // the scans load 16 record keys per thread before using any (the
+    // records sit in other XCDs' caches; one load at a time would leave
+    // each block waiting out every load's latency in turn).  Record i of
+    // the MAXS * nblocks is slot i / nblocks of block i % nblocks.
+    E.os << "  const u32 nrec = nblocks * used[0];\n";
+    E.os << "  for (u32 i0 = tid; i0 < nrec; i0 += 256u * 16u) {\n    u32 kk[16];\n";
+    E.os << "#pragma unroll\n    for (int j = 0; j < 16; j++) { const u32 i = i0 + 256u * j; kk[j] = i < nrec ? rec[rec_at(i, nblocks)] : 0u; }\n";
+    E.os << "#pragma unroll\n    for (int j = 0; j < 16; j++) {\n      const u32 k1 = kk[j];\n";
+    E.os << "      if (k1 && !(bm[(k1 - 1u) >> 5] & (1u << ((k1 - 1u) & 31u)))) atomicOr(&bm[(k1 - 1u) >> 5], 1u << ((k1 - 1u) & 31u));\n";
+    E.os << "    }\n  }\n";
     E.os << "  __syncthreads();\n";
     E.os << "  // prefix popcounts: thread t owns words [8t, 8t+8)\n";
     E.os << "  u32 loc = 0;\n#pragma unroll\n  for (int j = 0; j < 8; j++) loc += __popc(bm[tid * 8 + j]);\n";
-    E.os << "  part[tid] = loc;\n  __syncthreads();\n";
-    E.os << "  if (tid == 0) { u32 acc = 0; for (int i = 0; i < 256; i++) { const u32 x = part[i]; part[i] = acc; acc += x; } pre[2048] = acc; }\n";
-    E.os << "  __syncthreads();\n";
-    E.os << "  { u32 acc = part[tid];\n#pragma unroll\n    for (int j = 0; j < 8; j++) { pre[tid * 8 + j] = acc; acc += __popc(bm[tid * 8 + j]); } }\n";
+    // exclusive prefix of loc over the block: scan in each wave, then the
+    // wave totals
+    E.os << "  u32 inc = loc;\n";
+    E.os << "#pragma unroll\n  for (int off = 1; off < 64; off <<= 1) { const u32 y = __shfl_up(inc, off, 64); if ((tid & 63u) >= (u32)off) inc += y; }\n";
+    E.os << "  if ((tid & 63u) == 63u) part[tid >> 6] = inc;\n  __syncthreads();\n";
+    E.os << "  u32 wbase = 0;\n  for (u32 wv = 0; wv < (tid >> 6); wv++) wbase += part[wv];\n";
+    E.os << "  if (tid == 255u) pre[2048] = wbase + inc;\n";
+    E.os << "  { u32 acc = wbase + inc - loc;\n#pragma unroll\n    for (int j = 0; j < 8; j++) { pre[tid * 8 + j] = acc; acc += __popc(bm[tid * 8 + j]); } }\n";
     E.os << "  __syncthreads();\n";
     E.os << "  const u32 ngroups = pre[2048];\n";
     E.os << "  if (blockIdx.x == 0) {\n";
@@ -1704,12 +1722,19 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     E.os << "    while (x) { const u32 b = __ffs(x) - 1u; x &= x - 1u; if (r == g) gkey = w * 32u + b + 1u; r++; } }\n";
     E.os << "  __syncthreads();\n  const u32 want = gkey;\n";
     E.os << "  if (pp < NP) {\n    Mag<AW2> sm = mzero<AW2>();\n";
-    E.os << "    for (u32 i = tid; i < nrec; i += 256) {\n      const u32* e = rec + (usize)i * EW;\n      if (e[0] != want) continue;\n";
-    E.os << "      Mag<AW2> o;\n#pragma unroll\n      for (int w = 0; w < AW2; w++) o.w[w] = e[1 + pp * AW2 + w];\n      add_to<AW2>(sm, sm, o);\n    }\n";
+    E.os << "    for (u32 i0 = tid; i0 < nrec; i0 += 256u * 16u) {\n      u32 kk[16];\n      usize at[16];\n";
+    E.os << "#pragma unroll\n      for (int j = 0; j < 16; j++) { const u32 i = i0 + 256u * j; at[j] = i < nrec ? rec_at(i, nblocks) : 0; kk[j] = i < nrec ? rec[at[j]] : 0u; }\n";
+    E.os << "#pragma unroll\n      for (int j = 0; j < 16; j++) {\n        if (kk[j] != want) continue;\n";
+    E.os << "        Mag<AW2> o;\n#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = rec[at[j] + (usize)(1 + pp * AW2 + w) * nblocks];\n";
+    E.os << "        add_to<AW2>(sm, sm, o);\n      }\n    }\n";
     E.os << kBlockReduceAW2;
     emit_agg_final_image(E, s, "images + ((usize)g * NP + pp) * 133u");
     E.os << "    }\n  } else {\n    u64 c = 0;\n";
-    E.os << "    for (u32 i = tid; i < nrec; i += 256) { const u32* e = rec + (usize)i * EW; if (e[0] == want) c += (u64)e[1 + NP * AW2] | ((u64)e[2 + NP * AW2] << 32); }\n";
+    E.os << "    for (u32 i0 = tid; i0 < nrec; i0 += 256u * 16u) {\n      u32 kk[16];\n      usize at[16];\n";
+    E.os << "#pragma unroll\n      for (int j = 0; j < 16; j++) { const u32 i = i0 + 256u * j; at[j] = i < nrec ? rec_at(i, nblocks) : 0; kk[j] = i < nrec ? rec[at[j]] : 0u; }\n";
+    E.os << "#pragma unroll\n      for (int j = 0; j < 16; j++)\n";
+    E.os << "        if (kk[j] == want) c += (u64)rec[at[j] + (usize)(1 + NP * AW2) * nblocks] | ((u64)rec[at[j] + (usize)(2 + NP * AW2) * nblocks] << 32);\n";
+    E.os << "    }\n";
     E.os << kBlockReduceCount;
     E.os << "    if (tid == 0) { u64 t = 0; for (int wv = 0; wv < 4; wv++) t += (u64)red[2 * wv] | ((u64)red[2 * wv + 1] << 32); counts[g] = t; }\n  }\n}\n";
     return E.os.str();

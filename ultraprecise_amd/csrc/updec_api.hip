@@ -336,7 +336,14 @@ __global__ void __launch_bounds__(64) k_div_int64(const u8* sums, int64_t stride
     b.m.w[0] = (u32)uc;
     b.m.w[1] = (u32)(uc >> 32);
     u32 err = error;
-    Dec<32> q = ddiv<32, 32, 2, 4, FAITH>(a, b, &err);
+    // sums of up to 8 limbs (every sum of narrow columns) divide at that
+    // width: the quotient's value does not depend on the operand widths
+    bool narrow = true;
+#pragma unroll
+    for (int w = 8; w < 32; w++) narrow = narrow && a.m.w[w] == 0u;
+    Dec<32> q;
+    if (narrow) q = dresize<32>(ddiv<10, 8, 2, 4, FAITH>(dresize<8>(a), b, &err));
+    else q = ddiv<32, 32, 2, 4, FAITH>(a, b, &err);
     u8* o = out + i * 133;
     o[0] = s[0];
     o[1] = (u8)q.neg;
@@ -382,7 +389,7 @@ struct updec_agg_s {
     ModuleP op_mod;
     std::string op_src;
     int op_block = 256;
-    u32* d_meta = nullptr;  // [0] overflow flag, [1] ngroups, [2] too many, [3..] keys
+    u32* d_meta = nullptr;  // [0] overflow flag, [1] slots used, [2] ngroups, [3] too many, [4..] keys
     u32* h_meta = nullptr;  // pinned host copy
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
@@ -754,10 +761,10 @@ static int agg_run_onepass(updec_agg_t a, const uint8_t* const* d_cols, const in
         a->partial_words = need;
     }
     if (!a->d_meta) {
-        HIPCHK(hipMalloc((void**)&a->d_meta, (3 + 256) * sizeof(u32)));
-        HIPCHK(hipHostMalloc((void**)&a->h_meta, (3 + 256) * sizeof(u32)));
+        HIPCHK(hipMalloc((void**)&a->d_meta, (4 + 256) * sizeof(u32)));
+        HIPCHK(hipHostMalloc((void**)&a->h_meta, (4 + 256) * sizeof(u32)));
     }
-    HIPCHK(hipMemsetAsync(a->d_meta, 0, sizeof(u32), st));
+    HIPCHK(hipMemsetAsync(a->d_meta, 0, 2 * sizeof(u32), st));
     AggArgs args;
     fill_args(a, &args, d_cols, d_filter, d_key0, d_key1, rows);
     u32* ovf = a->d_meta;
@@ -775,22 +782,23 @@ static int agg_run_onepass(updec_agg_t a, const uint8_t* const* d_cols, const in
     u32 nb = (u32)grid, maxg = (u32)max_groups;
     uint8_t* img = d_images;
     uint64_t* cnt = reinterpret_cast<uint64_t*>(d_counts);
-    u32* meta = a->d_meta + 1;
-    void* pf[] = {&rec, &nb, &maxg, &img, &cnt, &meta};
+    const u32* used = a->d_meta + 1;
+    u32* meta = a->d_meta + 2;
+    void* pf[] = {&rec, &nb, &used, &maxg, &img, &cnt, &meta};
     const int NP = (int)a->spec.plans.size();
     HIPCHK(hipModuleLaunchKernel(ffin, (unsigned)(max_groups * (NP + 1)), 1, 1, 256, 1, 1, 0, st, pf, nullptr));
-    HIPCHK(hipMemcpyAsync(a->h_meta, a->d_meta, (3 + (size_t)max_groups) * sizeof(u32), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(a->h_meta, a->d_meta, (4 + (size_t)max_groups) * sizeof(u32), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (a->h_meta[0]) {
         *overflowed = true;
         return UPDEC_OK;
     }
-    if (a->h_meta[2]) return fail(UPDEC_EINVAL, "more groups than max_groups");
-    const int ng = (int)a->h_meta[1];
+    if (a->h_meta[3]) return fail(UPDEC_EINVAL, "more groups than max_groups");
+    const int ng = (int)a->h_meta[2];
     *ngroups_out = ng;
     if (keys_out)
         for (int g = 0; g < ng; g++) {
-            const u32 k = a->h_meta[3 + g];
+            const u32 k = a->h_meta[4 + g];
             keys_out[g] = (uint16_t)(a->nkeys == 2 ? (((k >> 8) & 0xff) | ((k & 0xff) << 8)) : k);
         }
     return UPDEC_OK;
