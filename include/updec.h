@@ -162,6 +162,36 @@ int updec_div_int64(const uint8_t* d_sums, int64_t stride_bytes, const int64_t* 
 int updec_segsum_compact(const uint8_t* d_col, int precision, int scale, int64_t rows, const int32_t* d_associated,
                          const int32_t* d_groups, int32_t ngroups, uint8_t* d_out, int mode, updec_stream_t stream);
 
+/* ---------------- codec: ingest and output of compact columns ----------------
+ * Bulk ingest of one text column into DECIMAL(P, S) compact rows: the
+ * per-value step of the CSV import (src/server/mysql/importCsvFile.cpp:640-660
+ * and :1883-1897), i.e. Decimal(P, S, text) (src/datatypes/decimal.cu:154-165:
+ * StringToDecimal :2980-3025, then cast :471-533) and ToCompactDecimal
+ * (:337-341).  Row r is d_chars[d_offsets[r] .. d_offsets[r+1]) (rows + 1
+ * offsets), cut at its first NUL.  The reference's character rules are kept
+ * for every byte (a non-digit c still contributes c - '0'); its reading past
+ * the terminator after a trailing '.' is not: a trailing '.' ends the row.
+ * The reference reports no error on this path (its overflow check is
+ * disabled, decimal.cu:400-425), so neither does this one.  B(P) must be
+ * at most 128 bytes (the reference's 32-limb Decimal). */
+int updec_parse_compact(const char* d_chars, const int64_t* d_offsets, int64_t rows, int32_t precision,
+                        int32_t scale, uint8_t* d_out, updec_stream_t stream);
+
+/* Bulk output of a compact column into fixed stride-byte slots, each
+ * NUL-terminated and zero-filled.  UPDEC_FORMAT_HEX is GetDecimal
+ * (src/datatypes/decimal.cu:362-397) of Decimal(compact, P, S) (:167-176),
+ * the string AriesDataBuffer::GetCompactDecimalAsString returns
+ * (src/CudaAcc/AriesEngineDef.cu:817-829): "+0000000C(002)".
+ * UPDEC_FORMAT_TEXT is the decimal text with exactly S fractional digits
+ * ("-123.4500"), the form the reference's decimal tests print
+ * (test/testcases/datatypes/test_decimal.cu).  stride: a multiple of 4 of
+ * at least updec_format_stride(); d_out 4-byte aligned. */
+#define UPDEC_FORMAT_HEX 0
+#define UPDEC_FORMAT_TEXT 1
+int updec_format_stride(int32_t precision, int32_t scale, int32_t form, int64_t* stride);
+int updec_format_compact(const uint8_t* d_col, int64_t rows, int32_t precision, int32_t scale, int32_t form,
+                         char* d_out, int64_t stride, updec_stream_t stream);
+
 /* ---------------- dynamic-code interface ----------------
  * AriesDynamicKernelManager::CompileKernels / CallKernel
  * (src/CudaAcc/DynamicKernel.h:50-99): compile caller-supplied kernel

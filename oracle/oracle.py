@@ -74,6 +74,9 @@ def _load():
         "od_segsum_compact": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I64, P, P, ctypes.c_int, P, ctypes.c_int,
                                              ctypes.c_int]),
         "od_avg_from_sum": (None, [P, I64, P, ctypes.c_int]),
+        "od_format_hex": (ctypes.c_int, [D, ctypes.c_char_p]),
+        "od_parse_compact": (None, [P, P, I64, ctypes.c_int, ctypes.c_int, P]),
+        "od_format_compact": (None, [P, I64, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, I64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -216,3 +219,35 @@ def image_value(img):
     img = bytes(img)
     m = int.from_bytes(img[5:133], "little")
     return img[1], img[2], img[3], img[4], m
+
+
+def format_hex(d):
+    """GetDecimal (decimal.cu:362-397) of an oracle value"""
+    buf = ctypes.create_string_buffer(8 * 32 + 8)
+    lib.od_format_hex(ctypes.byref(d), buf)
+    return buf.value.decode()
+
+
+def parse_compact(texts, P, S):
+    """Decimal(P, S, text) -> ToCompactDecimal for each text (bytes or str):
+    returns (rows * B(P) bytes as numpy uint8)"""
+    import numpy as np
+    enc = [t.encode("latin-1") if isinstance(t, str) else bytes(t) for t in texts]
+    chars = b"".join(enc)
+    offs = np.zeros(len(enc) + 1, dtype=np.int64)
+    offs[1:] = np.cumsum([len(t) for t in enc])
+    B = lib.od_compact_bytes(P)
+    out = np.zeros(len(enc) * B, dtype=np.uint8)
+    cbuf = ctypes.create_string_buffer(chars, len(chars) + 1)
+    lib.od_parse_compact(cbuf, offs.ctypes.data, len(enc), P, S, out.ctypes.data)
+    return out
+
+
+def format_compact(col, rows, P, S, hex_form, stride):
+    """stride-byte NUL-padded slots of each compact row, GetDecimal hex
+    (hex_form true) or decimal text"""
+    import numpy as np
+    out = np.zeros(rows * stride, dtype=np.uint8)
+    src = np.ascontiguousarray(col, dtype=np.uint8)
+    lib.od_format_compact(src.ctypes.data, rows, P, S, 1 if hex_form else 0, out.ctypes.data, stride)
+    return out

@@ -242,9 +242,26 @@ void od_to_compact(const od_dec* d, uint8_t* buf, int len)
     if (len > 0) buf[len - 1] |= (uint8_t)(d->sign << 7);
 }
 
-/* StringToDecimal (decimal.cu:2980-3025): digits accumulated as v = v*10 + d
- * in base 2^32 (mod 2^(32nw)); '-' first sets the sign; prec = number of
- * digits, frac = digits after '.'. */
+/* v += a (a sign-extended) mod 2^(32 nw) */
+static void add_signed_small(uint32_t* v, int nw, int32_t a)
+{
+    const uint32_t ext = a < 0 ? 0xffffffffu : 0u;
+    uint64_t c = 0;
+    for (int i = 0; i < nw; i++) {
+        uint64_t t = (uint64_t)v[i] + (i == 0 ? (uint32_t)a : ext) + c;
+        v[i] = (uint32_t)t;
+        c = t >> 32;
+    }
+}
+
+/* StringToDecimal (decimal.cu:2980-3025): every character c after an
+ * optional leading '-' is accumulated as v = v*10 + (c - '0') in base 2^32,
+ * mod 2^(32nw), with (c - '0') signed (a character that is not a digit
+ * still shifts and offsets the value, as in the reference); a '.' sets the
+ * fraction flag and the character after it is taken as a digit whatever it
+ * is; every character after the first '.' counts one fractional digit;
+ * prec = characters taken.  A '.' that ends the string ends it: the
+ * reference goes on to read its terminator as a digit and past it. */
 int od_from_string(od_dec* d, const char* s, int mode)
 {
     int nw = NW(mode);
@@ -254,7 +271,8 @@ int od_from_string(od_dec* d, const char* s, int mode)
     for (i = t; s[i] != '\0'; i++) {
         if (s[i] == '.') { dotflag = 1; i++; if (s[i] == '\0') break; }
         if (dotflag) dot++;
-        if (mul_small(d->v, nw, 10u, (uint32_t)(s[i] - '0'))) flag_overflow(mode);
+        if (mul_small(d->v, nw, 10u, 0u)) flag_overflow(mode);
+        add_signed_small(d->v, nw, (int32_t)(signed char)s[i] - '0');
     }
     d->sign = flag;
     d->prec = i - flag - dotflag;
@@ -476,6 +494,74 @@ static void from_nullable_image(od_dec* d, const uint8_t* in)
     od_zero(d);
     d->sign = in[1]; d->prec = in[2]; d->frac = in[3]; d->error = in[4];
     memcpy(d->v, in + 5, 32 * sizeof(uint32_t));
+}
+
+/* GetDecimal (decimal.cu:362-397): '+' or '-', then 8 upper-case hex
+ * digits per limb from the top nonzero limb of the reference's 32 down to
+ * limb 0 (none for a zero magnitude), then "(" three decimal digits of frac
+ * ")".  out needs 8 * 32 + 7 bytes; returns the length without the NUL. */
+int od_format_hex(const od_dec* d, char* out)
+{
+    static const char dict[] = "0123456789ABCDEF";
+    int idx = 0, flag = 0;
+    out[idx] = d->sign ? '-' : '+';
+    for (int i = 31; i >= 0; i--) {
+        if (d->v[i] != 0) flag = 1;
+        if (flag) {
+            uint32_t t = d->v[i];
+            idx += 8;
+            for (int j = 0; j < 8; j++) { out[idx - j] = dict[t & 0xF]; t >>= 4; }
+        }
+    }
+    idx++;
+    out[idx++] = '(';
+    out[idx++] = (char)(d->frac / 100 + '0');
+    out[idx++] = (char)(d->frac / 10 % 10 + '0');
+    out[idx++] = (char)(d->frac % 10 + '0');
+    out[idx++] = ')';
+    out[idx] = '\0';
+    return idx;
+}
+
+/* Bulk ingest of one text column (importCsvFile.cpp:640-660): row r is
+ * chars[offsets[r] .. offsets[r+1]), cut at its first NUL; each row is
+ * Decimal(P, S, text) (decimal.cu:154-165: StringToDecimal, then cast) in
+ * the reference's 32 limbs, written with ToCompactDecimal
+ * (decimal.cu:337-341) as B(P) bytes at out + r * B(P). */
+void od_parse_compact(const char* chars, const int64_t* offsets, int64_t rows, int P, int S, uint8_t* out)
+{
+    const int B = od_compact_bytes(P);
+    char* buf = NULL;
+    size_t cap = 0;
+    od_dec d;
+    for (int64_t r = 0; r < rows; r++) {
+        const size_t len = (size_t)(offsets[r + 1] - offsets[r]);
+        if (len + 1 > cap) { cap = len + 64; buf = (char*)realloc(buf, cap); }
+        memcpy(buf, chars + offsets[r], len);
+        buf[len] = '\0';
+        od_from_string(&d, buf, OD_FAITHFUL);
+        od_cast(&d, P, S, OD_FAITHFUL);
+        od_to_compact(&d, out + (size_t)r * (size_t)B, B);
+    }
+    free(buf);
+}
+
+/* Bulk output of one compact column: Decimal(compact, P, S)
+ * (decimal.cu:167-176) of each row, formatted by GetDecimal (hex = 1,
+ * decimal.cu:362-397) or as decimal text (hex = 0, od_to_string), into
+ * stride-byte slots, NUL-terminated and zero-filled. */
+void od_format_compact(const uint8_t* col, int64_t rows, int P, int S, int hex, char* out, int64_t stride)
+{
+    const int B = od_compact_bytes(P);
+    od_dec d;
+    char tmp[OD_MAXW * 10 + 16];
+    for (int64_t r = 0; r < rows; r++) {
+        od_from_compact(&d, col + (size_t)r * (size_t)B, P, S);
+        int n = hex ? od_format_hex(&d, tmp) : od_to_string(&d, tmp, (int)sizeof tmp);
+        char* o = out + (size_t)r * (size_t)stride;
+        memset(o, 0, (size_t)stride);
+        memcpy(o, tmp, (size_t)(n < stride ? n : stride - 1));
+    }
 }
 
 int od_to_string(const od_dec* d, char* buf, int cap)

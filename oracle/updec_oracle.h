@@ -60,6 +60,12 @@ int  od_bitlen(const od_dec* d);
 void od_to_nullable_image(const od_dec* d, uint8_t* out133);
 /* decimal string of value (for tests): "-123.4500" */
 int  od_to_string(const od_dec* d, char* buf, int cap);
+/* GetDecimal hex form (decimal.cu:362-397): "+0000000C(002)" */
+int  od_format_hex(const od_dec* d, char* out);
+/* bulk text -> compact (importCsvFile.cpp:640-660 via decimal.cu:154-165, 337-341) */
+void od_parse_compact(const char* chars, const int64_t* offsets, int64_t rows, int P, int S, uint8_t* out);
+/* bulk compact -> hex (hex = 1) or decimal text (hex = 0), stride-byte NUL-padded slots */
+void od_format_compact(const uint8_t* col, int64_t rows, int P, int S, int hex, char* out, int64_t stride);
 
 /* overflow accounting for OD_EXTENDED (set when a result would exceed OD_MAXW limbs) */
 int  od_overflow_flag(void);

@@ -121,3 +121,50 @@ def compact_decode(buf, S):
     sign = b[-1] >> 7
     b[-1] &= 0x7F
     return sign, int.from_bytes(bytes(b), "little"), S
+
+
+def string_to_decimal(raw):
+    """StringToDecimal (decimal.cu:2980-3025) over the bytes of one row, cut
+    at its first NUL: v = v*10 + (c - '0') mod 2^1024 for every byte taken,
+    (c - '0') signed (bytes are C chars, signed); after a '.', the next byte
+    is taken whatever it is; bytes after the first '.' are fractional.  A
+    '.' that ends the row ends it."""
+    raw = bytes(raw).split(b"\0", 1)[0]
+    sign, i, v, frac, dot = 0, 0, 0, 0, False
+    if raw[:1] == b"-":
+        sign, i = 1, 1
+    while i < len(raw):
+        c = raw[i]
+        if c == ord("."):
+            dot = True
+            i += 1
+            if i >= len(raw):
+                break
+            c = raw[i]
+        if dot:
+            frac += 1
+        d = (c - 256 if c >= 128 else c) - ord("0")
+        v = (v * 10 + d) % MOD1024
+        i += 1
+    return sign, v, frac
+
+
+def decimal_of_bytes(P, S, raw, nbytes):
+    """Decimal(P, S, text) (decimal.cu:154-165) then ToCompactDecimal (:337-341)"""
+    sign, m, s = string_to_decimal(raw)
+    if S >= s:
+        m = (m * 10 ** (S - s)) % MOD1024
+    else:
+        m = m // 10 ** (s - S)
+    return compact_encode((sign, m, S), nbytes)
+
+
+def get_decimal_hex(v):
+    """GetDecimal (decimal.cu:362-397): sign, 8 hex digits per limb from the
+    top nonzero one of 32, '(' three scale digits ')'"""
+    sign, m, s = v[0], v[1], v[2]
+    m %= MOD1024
+    limbs = [(m >> (32 * i)) & 0xFFFFFFFF for i in range(32)]
+    top = max((i for i in range(32) if limbs[i]), default=-1)
+    body = "".join("%08X" % limbs[i] for i in range(top, -1, -1))
+    return ("-" if sign else "+") + body + "(" + "%d%d%d" % (s // 100, s // 10 % 10, s % 10) + ")"

@@ -103,6 +103,9 @@ def _load():
         "updec_batch_pending": (ctypes.c_int, [P, ctypes.POINTER(I32)]),
         "updec_batch_launch": (ctypes.c_int, [P, ctypes.c_int, P]),
         "updec_batch_free": (None, [P]),
+        "updec_parse_compact": (ctypes.c_int, [P, P, I64, I32, I32, P, P]),
+        "updec_format_stride": (ctypes.c_int, [I32, I32, I32, ctypes.POINTER(I64)]),
+        "updec_format_compact": (ctypes.c_int, [P, I64, I32, I32, I32, P, I64, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -124,7 +127,8 @@ EXPORTED = ["updec_last_error", "updec_version", "updec_init", "updec_compact_by
             "updec_agg_free", "updec_div_int64", "updec_segsum_compact", "updec_compile", "updec_launch",
             "updec_module_free", "updec_check_compile", "updec_gen_compact", "updec_gen_int32", "updec_gen_byte",
             "updec_vm_compile", "updec_vm_info", "updec_vm_listing", "updec_vm_free", "updec_batch_create",
-            "updec_batch_tuning", "updec_batch_add", "updec_batch_pending", "updec_batch_launch", "updec_batch_free"]
+            "updec_batch_tuning", "updec_batch_add", "updec_batch_pending", "updec_batch_launch", "updec_batch_free",
+            "updec_parse_compact", "updec_format_stride", "updec_format_compact"]
 
 
 def lib():
@@ -359,6 +363,28 @@ def div_int64(sums, stride, counts, out, n, mode=MODE_AUTO, stream=None, sums_of
 def segsum_compact(col, precision, scale, rows, associated, groups, ngroups, out, mode=MODE_AUTO, stream=None):
     _check(_lib.updec_segsum_compact(_ptr(col), precision, scale, rows, _ptr(associated), _ptr(groups), ngroups,
                                      _ptr(out), mode, _stream(stream)))
+
+
+FORMAT_HEX, FORMAT_TEXT = 0, 1
+
+
+def parse_compact(chars, offsets, rows, precision, scale, out, stream=None):
+    """Decimal(P, S, text) -> compact rows for a text column on the device:
+    row r is chars[offsets[r] .. offsets[r+1])"""
+    _check(_lib.updec_parse_compact(_ptr(chars), _ptr(offsets), rows, precision, scale, _ptr(out), _stream(stream)))
+
+
+def format_stride(precision, scale, form):
+    """bytes per output slot of format_compact"""
+    n = ctypes.c_int64(0)
+    _check(_lib.updec_format_stride(precision, scale, form, ctypes.byref(n)))
+    return n.value
+
+
+def format_compact(col, rows, precision, scale, form, out, stride, stream=None):
+    """GetDecimal hex (FORMAT_HEX) or decimal text (FORMAT_TEXT) of every
+    compact row, into stride-byte NUL-padded slots"""
+    _check(_lib.updec_format_compact(_ptr(col), rows, precision, scale, form, _ptr(out), stride, _stream(stream)))
 
 
 def column_maxbits(col, precision, rows, stream=None):

@@ -32,7 +32,7 @@ def build(verbose=False):
     embed_device_header()
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     objs = []
-    for src in ("updec_api.hip", "vm.hip", "planner.cpp"):
+    for src in ("updec_api.hip", "vm.hip", "codec.hip", "planner.cpp"):
         obj = os.path.join(CSRC, src.rsplit(".", 1)[0] + ".o")
         cmd = [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
