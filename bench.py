@@ -4,15 +4,17 @@
 Headline workload: grouped aggregation of the TPC-H Q1 shape, 60,000,000
 rows (seed 2): filter ship <= 2435, group by (flag1, flag2), SUM(qty),
 SUM(price), SUM(price*(1-disc)), SUM(price*(1-disc)*(1+tax)), AVG(qty),
-AVG(price), AVG(disc), COUNT(*).  One step = one full pass (group
-discovery, fused filter/expression/SUM kernel, final combine, the three
-AVG divisions) over the device-resident columns.
+AVG(price), AVG(disc), COUNT(*).  One step = one full pass (the fused
+filter/key-hash/expression/SUM kernel, the device merge in key order, the
+group-count readback, the three AVG divisions) over the device-resident
+columns.
 
 Also reported: the plain-method workload (1,000,000 rows, SUM(p*(1-d)) and
 SUM(q*p)), the precision sweep ((a+b)*c - a/d over 10,000,000 rows at
-P = 18, 38, 76, 150, 300) and the high-precision series (sin(x) by a
-25-term Horner series over 1,000,000 DECIMAL(300,100) rows, then SUM), each
-with its kernel time and bytes.
+P = 18, 38, 76, 150, 300), the high-precision series (sin(x) by a 25-term
+Horner series over 1,000,000 DECIMAL(300,100) rows, then SUM), the batched
+serving loop, and the compact codec (text ingest and output formatting of
+10,000,000 DECIMAL(15,2) rows), each with its kernel time and bytes.
 
 Prints one JSON line.
 """
@@ -204,6 +206,65 @@ def series(U, W, steps, cpu_rows, threads):
     return res
 
 
+def codec(U, W, steps, cpu_rows):
+    """the compact codec at the edges over 10,000,000 rows of the Q1 price
+    column, DECIMAL(15,2): format as decimal text and as GetDecimal hex,
+    and ingest the packed text column back (one lane per row each)"""
+    import numpy as np
+    rows = 10_000_000
+    P, S = 15, 2
+    spec = W.CONFIG3["cols"][1]
+    B = U.compact_bytes(P)
+    col = U.gen_compact_device(W.gen_desc(U, W.CONFIG3["seed"], 1, spec), rows)
+    out = {}
+    ev0, ev1 = U.Event(), U.Event()
+    slots = None
+    for name, form in (("format_text", U.FORMAT_TEXT), ("format_hex", U.FORMAT_HEX)):
+        stride = U.format_stride(P, S, form)
+        buf = U.DeviceBuffer(rows * stride)
+        U.format_compact(col, rows, P, S, form, buf, stride)
+        ev0.record()
+        for _ in range(steps):
+            U.format_compact(col, rows, P, S, form, buf, stride)
+        ev1.record()
+        ms = ev0.elapsed_ms(ev1) / steps
+        alg = rows * (B + stride)
+        out[name] = {"rows": rows, "stride": stride, "kernel_ms": ms, "mrows_per_s": rows / ms / 1e3,
+                     "alg_bytes": alg, "achieved_gbps": alg / ms / 1e6}
+        if form == U.FORMAT_TEXT:
+            slots = buf.numpy(count=rows * stride).reshape(rows, stride)
+        del buf
+    # packed text column (no padding): the text rows of the formatted slots
+    lens = (slots != 0).sum(axis=1).astype(np.int64)
+    chars = slots[slots != 0]
+    offs = np.zeros(rows + 1, dtype=np.int64)
+    offs[1:] = np.cumsum(lens)
+    dch = U.DeviceBuffer.from_numpy(chars)
+    doff = U.DeviceBuffer.from_numpy(offs)
+    back = U.DeviceBuffer(rows * B)
+    U.parse_compact(dch, doff, rows, P, S, back)
+    U.device_sync()
+    same = bool(np.array_equal(back.numpy(), col.numpy(count=rows * B)))
+    ev0.record()
+    for _ in range(steps):
+        U.parse_compact(dch, doff, rows, P, S, back)
+    ev1.record()
+    ms = ev0.elapsed_ms(ev1) / steps
+    alg = int(chars.size) + 8 * (rows + 1) + rows * B
+    out["parse_text"] = {"rows": rows, "text_bytes": int(chars.size), "kernel_ms": ms, "mrows_per_s": rows / ms / 1e3,
+                         "alg_bytes": alg, "achieved_gbps": alg / ms / 1e6, "round_trip_identical": same}
+    if cpu_rows > 0:
+        from oracle import oracle as O
+        n = min(cpu_rows, rows)
+        texts = [bytes(chars[offs[i]:offs[i + 1]]) for i in range(n)]
+        t0 = time.perf_counter()
+        O.parse_compact(texts, P, S)
+        cpu_s = time.perf_counter() - t0
+        out["parse_text"]["cpu"] = {"rows": n, "seconds": cpu_s, "mrows_per_s": n / cpu_s / 1e6, "threads": 1,
+                                    "kind": "port"}
+    return out
+
+
 def serving_bench(U, W, nq, rates, budget_gb, jit_queries=12):
     """the batched serving workload: nq seeded queries over resident columns,
     saturated (all queued at once) and at Poisson arrival rates, batched
@@ -328,9 +389,10 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=2_000_000, help="rows of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="headline only")
-    ap.add_argument("--extra", default="copy,plain,sweep,series,serving", help="comma list of extra workloads")
+    ap.add_argument("--extra", default="copy,plain,sweep,series,serving,codec", help="comma list of extra workloads")
     ap.add_argument("--sweep-rows", type=int, default=0)
     ap.add_argument("--series-cpu-rows", type=int, default=50_000, help="rows of the series CPU sample")
+    ap.add_argument("--codec-cpu-rows", type=int, default=1_000_000, help="rows of the text-ingest CPU sample")
     ap.add_argument("--serve-queries", type=int, default=2000, help="queries per serving run (workload: 10,000)")
     ap.add_argument("--serve-rates", default="200,400,800")
     ap.add_argument("--serve-budget-gb", type=int, default=24, help="device memory for resident columns")
@@ -361,6 +423,9 @@ def main():
     if "series" in want:
         extra["series_1M"] = series(U, W, max(2, args.steps // 5), args.series_cpu_rows, threads)
         log("series:", extra["series_1M"])
+    if "codec" in want:
+        extra["codec_10M"] = codec(U, W, max(3, args.steps // 4), args.codec_cpu_rows)
+        log("codec:", extra["codec_10M"])
     if "serving" in want:
         rates = [int(r) for r in args.serve_rates.split(",")]
         extra["serving"] = serving_bench(U, W, args.serve_queries, rates, args.serve_budget_gb)

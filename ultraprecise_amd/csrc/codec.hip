@@ -153,6 +153,58 @@ template <int W, int WO> __device__ Mag<WO> parse_cast(const char* s, u32 len, c
 // rows per block of the ingest and format kernels
 constexpr int kCodecBlock = 64;
 
+// Rows of at most 16 bytes (every DECIMAL(15,2) text, for one) take one
+// pass over registers: the dwords covering the row are loaded (only those
+// holding a byte of it, so nothing past the text is touched beyond its last
+// dword), realigned to the row's first byte, and the same character rules
+// run over the 16 byte positions with static indices.  Sixteen characters
+// keep the accumulated value within an int64 (|v| < 208 * 10^16 / 9).
+struct ShortRow {
+    i64 v;
+    u32 frac, neg, plain;
+};
+
+__device__ ShortRow scan_short(const char* s, u32 len)
+{
+    const uintptr_t a = reinterpret_cast<uintptr_t>(s);
+    const u32* p = reinterpret_cast<const u32*>(a & ~(uintptr_t)3);
+    const u32 sh = (u32)(a & 3u);
+    const u32 nd = (sh + len + 3u) >> 2;
+    u32 w[6];
+#pragma unroll
+    for (int k = 0; k < 5; k++) w[k] = (u32)k < nd ? p[k] : 0u;
+    w[5] = 0u;
+    u32 t[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) t[j] = __builtin_amdgcn_alignbyte(w[j + 1], w[j], sh);
+    ShortRow r{0, 0u, 0u, 1u};
+    bool dot = false, force = false, stop = false;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const u32 c = (t[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+        if ((u32)i >= len || stop) continue;
+        if (i == 0 && c == (u32)'-') {
+            r.neg = 1u;
+            continue;
+        }
+        if (c == 0u) {  // the row ends at its first NUL, also right after a '.'
+            stop = true;
+            continue;
+        }
+        if (!force && c == (u32)'.') {
+            dot = true;
+            force = true;  // the next character is taken as a digit whatever it is
+            continue;
+        }
+        force = false;
+        if (dot) r.frac++;
+        const i32 d = (i32)(signed char)(char)c - '0';
+        r.plain &= ((u32)d <= 9u) ? 1u : 0u;
+        r.v = r.v * 10 + d;
+    }
+    return r;
+}
+
 template <int WO>
 __global__ void __launch_bounds__(kCodecBlock) k_parse_compact(const char* __restrict__ chars,
                                                                const int64_t* __restrict__ offsets, int64_t rows,
@@ -166,8 +218,31 @@ __global__ void __launch_bounds__(kCodecBlock) k_parse_compact(const char* __res
         const int64_t o0 = offsets[row], o1 = offsets[row + 1];
         const char* s = chars + o0;
         const u32 len = o1 > o0 ? (u32)(o1 - o0) : 0u;
-        const RowScan sc = scan_row(s, len);
         Mag<WO> m;
+        u32 neg;
+        bool done = false;
+        if (len <= 16u) {
+            const ShortRow sr = scan_short(s, len);
+            neg = sr.neg;
+            if (S >= sr.frac || sr.plain) {
+                // v * 10^(S - frac) modulo 2^(32 WO), or floor(v / 10^(frac - S)) for v >= 0
+                u64 u = (u64)sr.v;
+                if (S < sr.frac) {
+                    const u32 n = sr.frac - S;  // <= 16
+                    u64 d = 1;
+                    for (u32 k = 0; k < n; k++) d *= 10u;
+                    u /= d;
+                }
+                const u32 ext = (S >= sr.frac && sr.v < 0) ? 0xffffffffu : 0u;
+#pragma unroll
+                for (int i = 0; i < WO; i++) m.w[i] = i == 0 ? (u32)u : (i == 1 ? (u32)(u >> 32) : ext);
+                if (S > sr.frac) m = mul_p10_rt<WO>(m, S - sr.frac);
+                done = true;
+            }
+        }
+        if (!done) {
+        const RowScan sc = scan_row(s, len);
+        neg = sc.neg;
         const bool down = S < sc.frac;
         if (!down) {
             m = parse_cast<WO, WO>(s, len, sc, S);
@@ -180,11 +255,12 @@ __global__ void __launch_bounds__(kCodecBlock) k_parse_compact(const char* __res
         } else {
             m = parse_cast<32, WO>(s, len, sc, S);
         }
+        }
         u8* dst = t8 + (u32)threadIdx.x * B;
 #pragma unroll
         for (int b = 0; b < 4 * WO; b++)
             if ((u32)b < B) dst[b] = (u8)(m.w[b >> 2] >> (8 * (b & 3)));
-        dst[B - 1] |= (u8)(sc.neg << 7);
+        dst[B - 1] |= (u8)(neg << 7);
     }
     __syncthreads();
     const int64_t nrows = rows - base < kCodecBlock ? rows - base : kCodecBlock;
