@@ -118,8 +118,10 @@ void updec_expr_free(updec_expr_t e);
  * AggregateColumnData(SUM/COUNT) (src/CudaAcc/AriesSqlOperator_group.cu:360-424)
  * for the case the planner can fuse: SUM(expr_k) for k < nexprs over rows
  * passing an optional int32 filter, grouped by up to two 1-byte key columns.
- * Group discovery runs first; groups are numbered in ascending key order
- * (the order SortColumns gives, AriesSqlOperator_group.cu:14-54).
+ * Groups are numbered in ascending key order (the order SortColumns gives,
+ * AriesSqlOperator_group.cu:14-54): one pass with per-block key tables
+ * merged on the device, or, when a block meets more than 16 keys, a key
+ * discovery pass and a host-numbered key table first.
  * Output: images[g * nexprs + k] is the 133-byte nullable_type<Decimal>
  * image of SUM(expr_k) in group g; counts[g] = COUNT(*) of group g;
  * keys_out[g] = key0 | key1 << 8 of group g (host memory). */
