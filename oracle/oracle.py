@@ -58,6 +58,7 @@ def _load():
         "od_sub": (None, [D, D, ctypes.c_int]),
         "od_mul": (None, [D, D, ctypes.c_int]),
         "od_div": (None, [D, D, ctypes.c_int]),
+        "od_mod": (None, [D, D, ctypes.c_int]),
         "od_cmp": (ctypes.c_int, [D, D, ctypes.c_int]),
         "od_is_zero": (ctypes.c_int, [D]),
         "od_bitlen": (ctypes.c_int, [D]),

@@ -496,6 +496,88 @@ static void from_nullable_image(od_dec* d, const uint8_t* in)
     memcpy(d->v, in + 5, 32 * sizeof(uint32_t));
 }
 
+/* ModCalc (decimal.cu:2621-2716), operator %= of two decimals:
+ *   sign = xor, frac = 0, prec = b.prec; 0 % x -> +0.  The scales are
+ *   dropped: A = |a| and B = |b| are taken as integers.  Faithful paths:
+ *   (i)  bitlen(A) <= 64 and bitlen(B) <= 64: ModByInt64 (:2682-2693), A % B in uint64;
+ *   (ii) B < 2^32: ModByInt (:2695-2714), one-limb long division;
+ *   (iii) DivInt (:2125-2191): bisection for the quotient between
+ *        2^max(0, la-lb-1) and 2^(la-lb+1) with the midpoint
+ *        floor((lo+hi)/2) (HalfIntDecimal :2093-2119), products and
+ *        comparisons in the 32-limb arithmetic, then the magnitude of the
+ *        signed residual A - q*B (CopyValue keeps no sign).  For
+ *        la < lb - 1 the reference's upper bound is a shift by a negative
+ *        count, or a store below its limb array: no defined result; here
+ *        the remainder A.
+ *   x % 0: the reference scans below the divisor's limbs: no defined
+ *   result; here A is kept, as x / 0 keeps it.
+ * Extended mode: A % B exactly (x % 0 -> A), same sign and scale rules. */
+void od_mod(od_dec* a, const od_dec* b, int mode)
+{
+    int nw = NW(mode);
+    od_dec A = *a;
+    a->sign = b->sign ^ a->sign;
+    a->prec = b->prec;
+    a->frac = 0;
+    if (od_is_zero(&A)) { a->sign = 0; a->prec = 1; return; }
+    if (od_is_zero(b)) return;
+    int la = bitlen_w(A.v, nw), lb = bitlen_w(b->v, nw);
+    int na = top_word(A.v, nw) + 1, nb = top_word(b->v, nw) + 1;
+    uint32_t R[OD_MAXW];
+    memset(R, 0, sizeof R);
+    if (mode == OD_FAITHFUL && la <= 64 && lb <= 64) {
+        uint64_t dvs = (uint64_t)A.v[0] | ((uint64_t)A.v[1] << 32);
+        uint64_t dvt = (uint64_t)b->v[0] | ((uint64_t)b->v[1] << 32);
+        uint64_t res = dvs % dvt;
+        R[0] = (uint32_t)res; R[1] = (uint32_t)(res >> 32);
+    } else if (mode == OD_FAITHFUL && nb == 1) {
+        uint32_t rem = 0;
+        for (int i = nw - 1; i >= 0; i--) {
+            if (A.v[i] || rem) {
+                uint64_t t = (uint64_t)A.v[i] + (uint64_t)rem * 4294967296ull;
+                rem = (uint32_t)(t % b->v[0]);
+            }
+        }
+        R[0] = rem;
+    } else if (mode == OD_FAITHFUL && la + 1 >= lb) {
+        od_dec ds = A, dt = *b, qmin, qmax, qmid, t;
+        ds.sign = dt.sign = 0; ds.frac = dt.frac = 0;
+        int pmax = la - lb + 1, pmin = la - lb - 1;
+        if (pmin < 0) pmin = 0;
+        od_zero(&qmin); od_zero(&qmax);
+        qmax.v[pmax / 32] = 1u << (pmax % 32);
+        qmin.v[pmin / 32] = 1u << (pmin % 32);
+        while (cmp_w(qmin.v, qmax.v, nw) < 0) {
+            od_zero(&qmid);
+            add_w(qmid.v, qmin.v, qmax.v, nw);   /* HalfIntDecimal: (qmin + qmax) / 2, 32 limbs */
+            uint32_t carry = 0;
+            for (int i = nw - 1; i >= 0; i--) {
+                uint64_t x = ((uint64_t)carry << 32) | qmid.v[i];
+                qmid.v[i] = (uint32_t)(x >> 1);
+                carry = (uint32_t)(x & 1u);
+            }
+            if (cmp_w(qmid.v, qmin.v, nw) == 0) break;
+            t = qmid;
+            od_mul(&t, &dt, OD_FAITHFUL);
+            int c = cmp_w(t.v, ds.v, nw);
+            if (c == 0) { qmin = qmid; break; }
+            if (c < 0) qmin = qmid; else qmax = qmid;
+        }
+        t = qmin;
+        od_mul(&t, &dt, OD_FAITHFUL);
+        if (cmp_w(ds.v, t.v, nw) >= 0) sub_w(R, ds.v, t.v, nw);
+        else sub_w(R, t.v, ds.v, nw);
+    } else if (na < nb) {
+        memcpy(R, A.v, sizeof(uint32_t) * (size_t)na);
+    } else {
+        uint32_t Q[OD_MAXW + 1];
+        memset(Q, 0, sizeof Q);
+        knuth_d(Q, R, A.v, na, b->v, nb);
+    }
+    memset(a->v, 0, sizeof a->v);
+    memcpy(a->v, R, sizeof(uint32_t) * (size_t)nw);
+}
+
 /* GetDecimal (decimal.cu:362-397): '+' or '-', then 8 upper-case hex
  * digits per limb from the top nonzero limb of the reference's 32 down to
  * limb 0 (none for a zero magnitude), then "(" three decimal digits of frac
@@ -592,7 +674,7 @@ int od_to_string(const od_dec* d, char* buf, int cap)
 
 /* ---------------- plan ---------------- */
 
-enum { N_COL, N_CONST, N_NEG, N_ADD, N_SUB, N_MUL, N_DIV, N_CAST };
+enum { N_COL, N_CONST, N_NEG, N_ADD, N_SUB, N_MUL, N_DIV, N_CAST, N_MOD };
 
 typedef struct pnode {
     int kind, P, S, col, a, b;
@@ -667,7 +749,7 @@ static const char* parse_node(od_plan* p, const char* s, int* out)
         p->n[id].kind = N_CAST; p->n[id].a = c; p->n[id].P = P; p->n[id].S = S;
     } else {
         int kind = !strcmp(op, "ADD") ? N_ADD : !strcmp(op, "SUB") ? N_SUB : !strcmp(op, "MUL") ? N_MUL
-                 : !strcmp(op, "DIV") ? N_DIV : -1;
+                 : !strcmp(op, "DIV") ? N_DIV : !strcmp(op, "MOD") ? N_MOD : -1;
         if (kind < 0) return NULL;
         int P = (int)strtol(s, (char**)&s, 10);
         int S = (int)strtol(s, (char**)&s, 10);
@@ -727,6 +809,7 @@ static void eval_node(const evctx* c, int id, int64_t row, od_dec* out)
     case N_SUB: od_sub(out, &rb, c->mode); break;
     case N_MUL: od_mul(out, &rb, c->mode); break;
     case N_DIV: od_div(out, &rb, c->mode); break;
+    case N_MOD: od_mod(out, &rb, c->mode); break;
     }
 }
 

@@ -52,6 +52,7 @@ void od_add(od_dec* a, const od_dec* b, int mode);                   /* decimal.
 void od_sub(od_dec* a, const od_dec* b, int mode);                   /* decimal.cu:1575-1613 */
 void od_mul(od_dec* a, const od_dec* b, int mode);                   /* decimal.cu:1325-1351 */
 void od_div(od_dec* a, const od_dec* b, int mode);                   /* decimal.cu:2201-2333 */
+void od_mod(od_dec* a, const od_dec* b, int mode);                   /* decimal.cu:2621-2716 */
 int  od_cmp(const od_dec* a, const od_dec* b, int mode);             /* decimal.cu:638-745 */
 int  od_is_zero(const od_dec* d);
 int  od_bitlen(const od_dec* d);

@@ -168,3 +168,21 @@ def get_decimal_hex(v):
     top = max((i for i in range(32) if limbs[i]), default=-1)
     body = "".join("%08X" % limbs[i] for i in range(top, -1, -1))
     return ("-" if sign else "+") + body + "(" + "%d%d%d" % (s // 100, s // 10 % 10, s % 10) + ")"
+
+
+def mod(a, b, faithful=True):
+    """ModCalc (decimal.cu:2621-2716) in closed form: scales dropped, sign
+    xor, 0 % x = +0, x % 0 keeps |x|; faithful: when the reference bisects
+    (not both <= 64 bits, |b| >= 2^32) and bitlen(|a|) is bitlen(|b|) or one
+    less, the result is ||a| - |b||"""
+    sa, A = a[0], a[1]
+    sb, B = b[0], b[1]
+    if A == 0:
+        return 0, 0, 0
+    sign = sa ^ sb
+    if B == 0:
+        return sign, A, 0
+    la, lb = A.bit_length(), B.bit_length()
+    if faithful and not (la <= 64 and lb <= 64) and B >= 1 << 32 and lb - 1 <= la <= lb:
+        return sign, abs(A - B), 0
+    return sign, A % B, 0

@@ -66,12 +66,14 @@ def test_parse_errors(U):
         U.Expr("x+1", C15)
     with pytest.raises(U.UpdecError):
         U.Expr("q+", C15)
-    with pytest.raises(U.UpdecError, match="MOD"):
-        U.Expr("q%p", C15)
+    with pytest.raises(U.UpdecError, match="cast"):
+        U.Expr("cast(q, 123456789012345678901234, 2)", C15)
+    # MOD compiles on the JIT path (it has no interpreter instruction)
+    assert U.Expr("q%p", C15).plan.endswith("(MOD 15 0 c0:15:2 c1:15:2)")
 
 
 def test_rewritten_plan_parses_in_oracle(U, O):
-    for text in ["p*(1-d)", "q*p", "(q+p)*(d-q)/p", "-q*p+d", "q-(p-d)"]:
+    for text in ["p*(1-d)", "q*p", "(q+p)*(d-q)/p", "-q*p+d", "q-(p-d)", "q*p % d + 1"]:
         pl = O.Plan(U.Expr(text, C15).plan)
         assert pl.P > 0
 

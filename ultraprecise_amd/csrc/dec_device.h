@@ -749,6 +749,55 @@ UPD_DEV Dec<WO> ddiv(const Dec<WA>& a, const Dec<WB>& b, u32* err = nullptr)
     return r;
 }
 
+// ModCalc (decimal.cu:2621-2716), the % of two decimals: both operands are
+// taken as integers (their scales are dropped), the result has scale 0 and
+// sign a.sign xor b.sign, and 0 % x = +0.  By bit lengths la = bitlen(|a|),
+// lb = bitlen(|b|):
+//   la <= 64 and lb <= 64 (ModByInt64), or |b| < 2^32 (ModByInt): |a| mod |b|;
+//   otherwise DivInt's bisection (decimal.cu:2125-2191), whose quotient
+//   search starts at [2^(la-lb-1), 2^(la-lb+1)]:
+//     la > lb: |a| mod |b|;
+//     la = lb or la = lb - 1: the search stops at quotient 1 and the
+//       remainder |a| - |b| is kept without its sign: ||a| - |b|| (FAITH);
+//     la <= lb - 2: the reference shifts by a negative count or stores
+//       below its limb array (no defined result); |a| mod |b| = |a| here.
+// x % 0: the reference scans below the divisor's limbs (no defined result);
+// |a| is kept, as x / 0 keeps it.  Without FAITH: |a| mod |b| throughout.
+template <int WO, int WA, int WB, int FAITH> UPD_DEV Dec<WO> dmod(const Dec<WA>& a, const Dec<WB>& b)
+{
+    Dec<WO> r;
+    if (is_zero(a.m)) {
+        r.m = mzero<WO>();
+        r.neg = 0u;
+        return r;
+    }
+    r.neg = a.neg ^ b.neg;
+    if (is_zero(b.m)) {
+        r.m = resize<WO>(a.m);
+        return r;
+    }
+    Mag<WA + 1> Q;
+    Mag<WB> R;
+    divmod<WA, WB>(a.m, b.m, Q, R);
+    r.m = resize<WO>(R);
+    if constexpr (FAITH) {
+        const u32 la = bitlen(a.m), lb = bitlen(b.m);
+        bool small_b = true;
+#pragma unroll
+        for (int i = 1; i < WB; i++) small_b = small_b && b.m.w[i] == 0u;
+        const bool bisect = !(la <= 64u && lb <= 64u) && !small_b;
+        if (bisect && la <= lb && la + 1u >= lb) {
+            constexpr int WM = WA > WB ? WA : WB;
+            const Mag<WM> x = resize<WM>(a.m), y = resize<WM>(b.m);
+            Mag<WM> d;
+            if (cmp<WM>(x, y) >= 0) sub_to<WM>(d, x, y);
+            else sub_to<WM>(d, y, x);
+            r.m = resize<WO>(d);
+        }
+    }
+    return r;
+}
+
 // ---------------------------------------------------------------- compact codec
 
 // Decode one compact DECIMAL row (decimal.cu:168-176): B little-endian bytes

@@ -325,8 +325,19 @@ struct Parser {
                 e->kind = NK::Dec;
                 e->dval = HostDec::from_string(lit, mode);
             } else {
-                e->kind = NK::Int;
-                e->ival = std::stoll(lit);
+                // an integer literal that int64 cannot hold is a DECIMAL
+                // literal of its digits (MySQL's rule for integer literals
+                // past BIGINT)
+                size_t nz = lit.find_first_not_of('0');
+                const std::string digits = nz == std::string::npos ? "0" : lit.substr(nz);
+                if (digits.size() < 19 || (digits.size() == 19 && digits <= "9223372036854775807")) {
+                    e->kind = NK::Int;
+                    e->ival = 0;
+                    for (char d : digits) e->ival = e->ival * 10 + (d - '0');
+                } else {
+                    e->kind = NK::Dec;
+                    e->dval = HostDec::from_string(digits, mode);
+                }
             }
             return e;
         }
@@ -345,6 +356,7 @@ struct Parser {
                     size_t b = lx.i;
                     while (lx.i < lx.s.size() && isdigit((unsigned char)lx.s[lx.i])) lx.i++;
                     if (b == lx.i) return false;
+                    if (lx.i - b > 6) return false;   // P and S have at most a few digits
                     *v = std::stol(lx.s.substr(b, lx.i - b));
                     return true;
                 };
@@ -882,6 +894,11 @@ static int op_bits(const PNode& n, const PNode& A, const PNode* B, int ba, int b
     }
     case PK::Mul: return ba + bb;
     case PK::Div: return std::max(64, ba + pow10_bits(B->S + 4) + 1);
+    // every path of the reference's % leaves a value below |b| when b != 0
+    // (a remainder, or ||a| - |b|| with bitlen(a) <= bitlen(b)); x % 0 keeps |a|
+    case PK::Mod:
+        if (B->kind == PK::Const && B->k.bitlen() > 0) return std::max(1, bb);
+        return std::max(1, std::max(ba, bb));
     case PK::Cast:
         if (n.S > A.S) return ba + pow10_bits(n.S - A.S);
         if (n.S < A.S) return std::max(1, ba - (pow10_bits(A.S - n.S) - 1));
@@ -958,7 +975,10 @@ bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode,
                 n.kind = PK::Div;
                 n.S = A.S + 4;
                 break;
-            case Op::Mod: *err = "MOD is not implemented on the HIP path"; return -1;
+            case Op::Mod:
+                n.kind = PK::Mod;
+                n.S = 0;
+                break;
             }
             if (n.S != e.S) { *err = "internal: runtime scale differs from node type"; return -1; }
         }
@@ -1009,7 +1029,11 @@ std::string Plan::text() const
         case PK::Cast: snprintf(buf, sizeof buf, "(CAST %d %d ", x.P, x.S); return buf + rec(x.a) + ")";
         default: break;
         }
-        const char* op = x.kind == PK::Add ? "ADD" : x.kind == PK::Sub ? "SUB" : x.kind == PK::Mul ? "MUL" : "DIV";
+        const char* op = x.kind == PK::Add   ? "ADD"
+                         : x.kind == PK::Sub ? "SUB"
+                         : x.kind == PK::Mul ? "MUL"
+                         : x.kind == PK::Mod ? "MOD"
+                                             : "DIV";
         snprintf(buf, sizeof buf, "(%s %d %d ", op, x.P, x.S);
         return buf + rec(x.a) + " " + rec(x.b) + ")";
     };
@@ -1029,6 +1053,9 @@ bool plan_is_faithful_envelope(const Plan& p, const std::vector<ColumnType>& col
         }
         // operator<< reads one row of the 38-row 10^n table (decimal.cu:2028-2058)
         if (x.kind == PK::Cast && x.S - p.n[x.a].S > 38) return false;
+        // the reference's MOD bisection multiplies quotient candidates by the
+        // divisor modulo 2^1024: exact only while the dividend is below 2^1023
+        if (x.kind == PK::Mod && p.n[x.a].bits > 1023) return false;
         if (x.kind == PK::Div) {
             int s = p.n[x.b].S + 4;
             if (s > 38) return false;
@@ -1042,8 +1069,8 @@ bool plan_is_faithful_envelope(const Plan& p, const std::vector<ColumnType>& col
     return true;
 }
 
-bool plan_expression(const std::string& text, const std::vector<std::string>& names,
-                     const std::vector<ColumnType>& types, int mode, Plan* out, std::string* err)
+static bool plan_expression_impl(const std::string& text, const std::vector<std::string>& names,
+                                 const std::vector<ColumnType>& types, int mode, Plan* out, std::string* err)
 {
     for (int attempt = 0; attempt < 2; attempt++) {
         Mode m;
@@ -1066,6 +1093,18 @@ bool plan_expression(const std::string& text, const std::vector<std::string>& na
         return true;
     }
     return false;
+}
+
+bool plan_expression(const std::string& text, const std::vector<std::string>& names,
+                     const std::vector<ColumnType>& types, int mode, Plan* out, std::string* err)
+{
+    // nothing thrown below may reach the C ABI
+    try {
+        return plan_expression_impl(text, names, types, mode, out, err);
+    } catch (const std::exception& ex) {
+        *err = std::string("planner: ") + ex.what();
+        return false;
+    }
 }
 
 // ------------------------------------------------------------------ code generation
@@ -1140,6 +1179,10 @@ std::string emit_plan(Emit& E, const Plan& p, const std::vector<ColumnType>& col
             E.os << ind << "const Dec<" << x.W << "> " << v << " = ddiv<" << x.W << ", " << p.n[x.a].W << ", " << p.n[x.b].W
                  << ", " << (p.n[x.b].S + 4) << ", " << (faith ? 1 : 0) << ">(" << var[x.a] << ", " << var[x.b] << ");\n";
             break;
+        case PK::Mod:
+            E.os << ind << "const Dec<" << x.W << "> " << v << " = dmod<" << x.W << ", " << p.n[x.a].W << ", " << p.n[x.b].W
+                 << ", " << (faith ? 1 : 0) << ">(" << var[x.a] << ", " << var[x.b] << ");\n";
+            break;
         case PK::Cast: {
             const PNode& A = p.n[x.a];
             E.os << ind << "const Dec<" << x.W << "> " << v << " = ";
@@ -1172,6 +1215,7 @@ int plan_cost(const Plan& p)
         switch (x.kind) {
         case PK::Mul: cost += wa * wb; break;
         case PK::Div: cost += 4 * (wa + words_for(pow10_bits(p.n[x.b].S + 4)) + 1) * wb; break;
+        case PK::Mod: cost += 4 * (wa + 1) * wb; break;
         case PK::Add:
         case PK::Sub: cost += x.W; break;
         case PK::Cast: cost += wa * (1 + std::abs(x.S - p.n[x.a].S) / 9); break;
@@ -1973,6 +2017,7 @@ struct VmBuilder {
         }
         case PK::Mul: ok = emit(updec_vm::OP_MUL, dst, 0, a, b); break;
         case PK::Div: ok = emit(updec_vm::OP_DIV, dst, p.n[x.b].S + 4, a, b); break;
+        case PK::Mod: err = "MOD runs on the JIT path only (no interpreter instruction)"; return false;
         default: err = "internal: unexpected plan node"; return false;
         }
         *res = {updec_vm::K_REG, (uint8_t)dst, 0};

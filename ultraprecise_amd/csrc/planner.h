@@ -80,7 +80,7 @@ void infer_frontend_types(Expr& e, Mode mode);
 ExprP align_optimize(const ExprP& e, Mode mode);
 
 // Typed evaluation plan
-enum class PK { Col, Const, Neg, Add, Sub, Mul, Div, Cast };
+enum class PK { Col, Const, Neg, Add, Sub, Mul, Div, Cast, Mod };
 struct PNode {
     PK kind;
     int P = 0, S = 0;
@@ -103,7 +103,7 @@ int compact_bytes(int P);   // __PREC_BYTE_ARRAY (decimal.cu:92), closed form pa
 
 // Build a plan from a rewritten tree.  Column types give the operand
 // layout.  Returns false with *err set when the tree uses something the
-// HIP path does not implement (MOD).
+// HIP path does not implement.
 bool build_plan(const ExprP& rewritten, const std::vector<ColumnType>& cols, Mode requested, Plan* out,
                 std::string* err);
 
