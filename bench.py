@@ -13,8 +13,9 @@ Also reported: the plain-method workload (1,000,000 rows, SUM(p*(1-d)) and
 SUM(q*p)), the precision sweep ((a+b)*c - a/d over 10,000,000 rows at
 P = 18, 38, 76, 150, 300), the high-precision series (sin(x) by a 25-term
 Horner series over 1,000,000 DECIMAL(300,100) rows, then SUM), the batched
-serving loop, and the compact codec (text ingest and output formatting of
-10,000,000 DECIMAL(15,2) rows), each with its kernel time and bytes.
+serving loop, the compact codec (text ingest and output formatting of
+10,000,000 DECIMAL(15,2) rows) and the RSA workload (col1*col1%N*col1%N over
+10,000,000 rows for four key lengths), each with its kernel time and bytes.
 
 Prints one JSON line.
 """
@@ -265,6 +266,51 @@ def codec(U, W, steps, cpu_rows):
     return out
 
 
+RSA_KEYS = [
+    (17, "514679589732349351"),
+    (35, "331972734823750661745273085756461571"),
+    (71, "273420351015258185841522860354335753665897876941082396543621156533322691"),
+    (143, "1686878444203618142189652405784181065581460450325437359510561733117064525110066314054980898011446"
+          "29119144418134762919565200448008667563331468801"),
+]
+
+
+def rsa(U, W, steps, cpu_rows, threads):
+    """the RSA workload: col1 * col1 % N * col1 % N over 10,000,000 rows of
+    DECIMAL(P,0) integers with P digits, for the four key lengths"""
+    rows = 10_000_000
+    out = {}
+    for P, N in RSA_KEYS:
+        col = U.gen_compact_device(W.digits_desc(U, 8, 0, P, neg_permille=0), rows)
+        e = U.Expr(f"col1 * col1 % {N} * col1 % {N}", [("col1", P, 0)])
+        obuf = U.DeviceBuffer(rows * e.out_bytes + 16)
+        t0 = time.time()
+        e.calc([col], rows, obuf)
+        U.device_sync()
+        first_s = time.time() - t0
+        ev0, ev1 = U.Event(), U.Event()
+        ev0.record()
+        for _ in range(steps):
+            e.calc([col], rows, obuf)
+        ev1.record()
+        ms = ev0.elapsed_ms(ev1) / steps
+        alg = rows * (U.compact_bytes(P) + e.out_bytes)
+        r = {"rows": rows, "kernel_ms": ms, "mrows_per_s": rows / ms / 1e3, "alg_bytes": alg,
+             "achieved_gbps": alg / ms / 1e6, "out_type": [e.precision, e.scale], "faithful": e.faithful,
+             "first_call_s": first_s}
+        if cpu_rows > 0:
+            from oracle import oracle as O
+            n = min(cpu_rows, rows)
+            host = col.numpy(count=n * U.compact_bytes(P))
+            t0 = time.perf_counter()
+            O.plan_eval(O.Plan(e.plan), [host], [P], [0], n, e.out_bytes, threads=threads)
+            cpu_s = time.perf_counter() - t0
+            r["cpu"] = {"rows": n, "seconds": cpu_s, "mrows_per_s": n / cpu_s / 1e6, "threads": threads, "kind": "port"}
+        out[f"LEN_{4 * (1 << RSA_KEYS.index((P, N)))}"] = r
+        del col, obuf
+    return out
+
+
 def serving_bench(U, W, nq, rates, budget_gb, jit_queries=12):
     """the batched serving workload: nq seeded queries over resident columns,
     saturated (all queued at once) and at Poisson arrival rates, batched
@@ -389,10 +435,11 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=2_000_000, help="rows of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="headline only")
-    ap.add_argument("--extra", default="copy,plain,sweep,series,serving,codec", help="comma list of extra workloads")
+    ap.add_argument("--extra", default="copy,plain,sweep,series,serving,codec,rsa", help="comma list of extra workloads")
     ap.add_argument("--sweep-rows", type=int, default=0)
     ap.add_argument("--series-cpu-rows", type=int, default=50_000, help="rows of the series CPU sample")
     ap.add_argument("--codec-cpu-rows", type=int, default=1_000_000, help="rows of the text-ingest CPU sample")
+    ap.add_argument("--rsa-cpu-rows", type=int, default=200_000, help="rows of the RSA CPU sample (per key length)")
     ap.add_argument("--serve-queries", type=int, default=2000, help="queries per serving run (workload: 10,000)")
     ap.add_argument("--serve-rates", default="200,400,800")
     ap.add_argument("--serve-budget-gb", type=int, default=24, help="device memory for resident columns")
@@ -423,6 +470,9 @@ def main():
     if "series" in want:
         extra["series_1M"] = series(U, W, max(2, args.steps // 5), args.series_cpu_rows, threads)
         log("series:", extra["series_1M"])
+    if "rsa" in want:
+        extra["rsa_10M"] = rsa(U, W, max(3, args.steps // 4), args.rsa_cpu_rows, threads)
+        log("rsa:", extra["rsa_10M"])
     if "codec" in want:
         extra["codec_10M"] = codec(U, W, max(3, args.steps // 4), args.codec_cpu_rows)
         log("codec:", extra["codec_10M"])
