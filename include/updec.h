@@ -112,6 +112,26 @@ int updec_expr_source(updec_expr_t e, char* buf, size_t cap, size_t* needed);
 int updec_expr_calc(updec_expr_t e, const uint8_t* const* d_cols, int64_t rows, uint8_t* d_out, updec_stream_t stream);
 void updec_expr_free(updec_expr_t e);
 
+/* Block-column inputs: AriesColumnDataIterator (src/AriesColumnDataIterator.hxx:8-41,
+ * AriesColumnDataIterator.cu:1-117).  A column is nblocks device blocks of
+ * compact rows; block k holds rows [d_starts[k], d_starts[k+1]) (the last one
+ * up to `rows`), d_starts[0] = 0, ascending.  Row i of a launch reads
+ * position d_indices[i] of every column when d_indices is given (int32; -1 =
+ * NULL, as a join's outer rows), else position i.  Both pointer arrays are
+ * device memory. */
+typedef struct updec_column_blocks {
+    const uint8_t* const* d_blocks;
+    const int64_t* d_starts;
+    int32_t nblocks;
+    int64_t rows;
+} updec_column_blocks;
+/* updec_expr_calc over block columns and indices: output row i is the
+ * expression at position d_indices[i] (or i).  A NULL position, or one at or
+ * past a column's rows, gives a zero row and d_valid[i] = 0 (d_valid: one
+ * byte per row, 1 = value; may be NULL).  d_out 4-byte aligned. */
+int updec_expr_calc_blocks(updec_expr_t e, const updec_column_blocks* cols, const int32_t* d_indices, int64_t rows,
+                           uint8_t* d_out, uint8_t* d_valid, updec_stream_t stream);
+
 /* ---------------- fused aggregation ----------------
  * Replaces AEExprAggFunctionNode::RunKernelFunction
  * (src/AriesEngine/AriesExprCalcNode.cpp:1623-1700) with
@@ -148,6 +168,13 @@ int updec_agg_last_kernel_ms(updec_agg_t a, float* ms);
  * one-pass form with ngroups = 0 (no device needed) */
 int updec_agg_source(updec_agg_t a, int32_t ngroups, char* buf, size_t cap, size_t* needed);
 void updec_agg_free(updec_agg_t a);
+/* updec_agg_run over block columns and indices, for aggregations without
+ * filter and keys (one group): SUM of each expression over the positions
+ * d_indices[0..rows) (or 0..rows); NULL positions (-1) and positions past a
+ * column add to no SUM and to no COUNT.  d_images: nexprs 133-byte images;
+ * d_counts: one int64. */
+int updec_agg_run_blocks(updec_agg_t a, const updec_column_blocks* cols, const int32_t* d_indices, int64_t rows,
+                         uint8_t* d_images, int64_t* d_counts, updec_stream_t stream);
 
 /* AVG = SUM / COUNT per group: DivisionInt64 (src/CudaAcc/AriesSqlOperator_helpers.cu:717-764),
  * division_data_int64 (src/CudaAcc/AriesEngineAlgorithm.cu:9476-9535), do_div

@@ -33,6 +33,37 @@ class AggDesc(ctypes.Structure):
                 ("nkeys", ctypes.c_int32), ("mode", ctypes.c_int32)]
 
 
+class ColumnBlocks(ctypes.Structure):
+    _fields_ = [("d_blocks", ctypes.c_void_p), ("d_starts", ctypes.c_void_p), ("nblocks", ctypes.c_int32),
+                ("rows", ctypes.c_int64)]
+
+
+class BlockColumn:
+    """A compact column held as several device blocks (the reference's
+    AriesColumnDataIterator data blocks): `blocks` are DeviceBuffers or
+    views, `counts` their row counts.  Keeps the device arrays of block
+    pointers and start rows alive."""
+
+    def __init__(self, blocks, counts):
+        import numpy as np
+        if len(blocks) != len(counts) or not blocks:
+            raise ValueError("one row count per block")
+        self.blocks = list(blocks)
+        starts = np.zeros(len(counts), dtype=np.int64)
+        starts[1:] = np.cumsum(counts)[:-1]
+        self.rows = int(np.sum(counts))
+        self._ptrs = DeviceBuffer.from_numpy(np.array([b.ptr for b in blocks], dtype=np.uint64))
+        self._starts = DeviceBuffer.from_numpy(starts)
+        self.desc = ColumnBlocks(self._ptrs.ptr, self._starts.ptr, len(blocks), self.rows)
+
+
+def _blocks_array(columns):
+    arr = (ColumnBlocks * max(1, len(columns)))()
+    for i, c in enumerate(columns):
+        arr[i] = c.desc
+    return arr
+
+
 class GenDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("precision", ctypes.c_int32), ("digits", ctypes.c_int32),
                 ("neg_permille", ctypes.c_int32), ("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64),
@@ -104,6 +135,8 @@ def _load():
         "updec_batch_launch": (ctypes.c_int, [P, ctypes.c_int, P]),
         "updec_batch_free": (None, [P]),
         "updec_parse_compact": (ctypes.c_int, [P, P, I64, I32, I32, P, P]),
+        "updec_expr_calc_blocks": (ctypes.c_int, [P, ctypes.POINTER(ColumnBlocks), P, I64, P, P, P]),
+        "updec_agg_run_blocks": (ctypes.c_int, [P, ctypes.POINTER(ColumnBlocks), P, I64, P, P, P]),
         "updec_format_stride": (ctypes.c_int, [I32, I32, I32, ctypes.POINTER(I64)]),
         "updec_format_compact": (ctypes.c_int, [P, I64, I32, I32, I32, P, I64, P]),
     }
@@ -128,7 +161,8 @@ EXPORTED = ["updec_last_error", "updec_version", "updec_init", "updec_compact_by
             "updec_module_free", "updec_check_compile", "updec_gen_compact", "updec_gen_int32", "updec_gen_byte",
             "updec_vm_compile", "updec_vm_info", "updec_vm_listing", "updec_vm_free", "updec_batch_create",
             "updec_batch_tuning", "updec_batch_add", "updec_batch_pending", "updec_batch_launch", "updec_batch_free",
-            "updec_parse_compact", "updec_format_stride", "updec_format_compact"]
+            "updec_parse_compact", "updec_format_stride", "updec_format_compact", "updec_expr_calc_blocks",
+            "updec_agg_run_blocks"]
 
 
 def lib():
@@ -296,6 +330,13 @@ class Expr:
     def source(self):
         return _text(_lib.updec_expr_source, self._h)
 
+    def calc_blocks(self, columns, rows, out, indices=None, valid=None, stream=None):
+        """columns: BlockColumns in descriptor order; output row i is the
+        expression at position indices[i] (int32 DeviceBuffer, -1 = NULL)
+        or i; valid (optional DeviceBuffer, one byte per row) gets 1 / 0"""
+        _check(_lib.updec_expr_calc_blocks(self._h, _blocks_array(columns), _ptr(indices), rows, _ptr(out), _ptr(valid),
+                                           _stream(stream)))
+
     def calc(self, columns, rows, out, stream=None):
         """columns: DeviceBuffers in descriptor order; out: DeviceBuffer of
         at least rows * out_bytes bytes, 4-byte aligned."""
@@ -330,6 +371,12 @@ class Agg:
 
     def source(self, ngroups=1):
         return _text(_lib.updec_agg_source, self._h, ctypes.c_int32(ngroups))
+
+    def run_blocks(self, columns, rows, images, counts, indices=None, stream=None):
+        """SUM of each expression over BlockColumns, at positions indices[0..rows)
+        (or 0..rows); NULL positions are skipped.  Ungrouped, unfiltered."""
+        _check(_lib.updec_agg_run_blocks(self._h, _blocks_array(columns), _ptr(indices), rows, _ptr(images),
+                                         _ptr(counts), _stream(stream)))
 
     def run(self, columns, rows, images, counts, filter_col=None, key0=None, key1=None, max_groups=255, stream=None):
         """Returns (ngroups, keys).  images: DeviceBuffer of at least

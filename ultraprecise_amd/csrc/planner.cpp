@@ -1134,8 +1134,8 @@ std::string emit_plan(Emit& E, const Plan& p, const std::vector<ColumnType>& col
             int B = compact_bytes(cols[c].P);
             if (loaded[c].empty()) {
                 loaded[c] = "c" + std::to_string(c);
-                E.os << ind << "const Dec<" << (B + 3) / 4 << "> " << loaded[c] << " = load_compact<" << B
-                     << ">(a.col[" << c << "], row, a.colbytes[" << c << "]);\n";
+                E.os << ind << "const Dec<" << (B + 3) / 4 << "> " << loaded[c] << " = col_load<" << B << ">(a, " << c
+                     << ", row);\n";
             }
             E.os << ind << "const Dec<" << x.W << "> " << v << " = dresize<" << x.W << ">(" << loaded[c] << ");\n";
             break;
@@ -1274,7 +1274,29 @@ const char* kArgsStruct =
     "#define UPD_MAXCOLS 16\n"
     "struct ColArgs { const u8* col[UPD_MAXCOLS]; usize colbytes[UPD_MAXCOLS]; usize rows; u8* out; };\n"
     "struct AggArgs { const u8* col[UPD_MAXCOLS]; usize colbytes[UPD_MAXCOLS]; usize rows;\n"
-    "                 const i32* fcol; const u8* key0; const u8* key1; const u8* lut; u32* partials; };\n";
+    "                 const i32* fcol; const u8* key0; const u8* key1; const u8* lut; u32* partials; };\n"
+    // block-column inputs (AriesColumnDataIterator, AriesColumnDataIterator.cu:1-117):
+    // column c is nblocks[c] blocks of compact rows, block k starting at row
+    // starts[c][k]; rows of the launch are taken through idx (row i reads
+    // position idx[i], -1 = NULL) when idx is set
+    "struct IterArgs { const u8* const* blocks[UPD_MAXCOLS]; const i64* starts[UPD_MAXCOLS];\n"
+    "                  i32 nblocks[UPD_MAXCOLS]; usize colrows[UPD_MAXCOLS]; const i32* idx; usize rows;\n"
+    "                  usize rowlimit; u8* out; u8* valid; u32* partials; };\n"
+    "template <int B> __device__ __forceinline__ Dec<(B + 3) / 4> col_load(const ColArgs& a, int c, usize row)\n"
+    "{ return load_compact<B>(a.col[c], row, a.colbytes[c]); }\n"
+    "template <int B> __device__ __forceinline__ Dec<(B + 3) / 4> col_load(const AggArgs& a, int c, usize row)\n"
+    "{ return load_compact<B>(a.col[c], row, a.colbytes[c]); }\n"
+    // row < rowlimit is checked by the caller; the block is the last one
+    // starting at or before the row (the iterator's bounds_upper search)
+    "template <int B> __device__ __forceinline__ Dec<(B + 3) / 4> col_load(const IterArgs& a, int c, usize row)\n"
+    "{\n"
+    "  const i64* st = a.starts[c];\n"
+    "  int lo = 0, hi = a.nblocks[c];\n"
+    "  while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if ((usize)st[mid] <= row) lo = mid; else hi = mid; }\n"
+    "  const usize first = (usize)st[lo];\n"
+    "  const usize end = lo + 1 < a.nblocks[c] ? (usize)st[lo + 1] : a.colrows[c];\n"
+    "  return load_compact<B>(a.blocks[c][lo], row - first, (end - first) * (usize)B);\n"
+    "}\n";
 
 // Columns of narrow rows (B <= 32) are read through a per-wave LDS stage
 // with 16-byte loads (stage_wave / load_staged), up to kStageWordsPerWave
@@ -1348,7 +1370,7 @@ static void emit_row_preload(Emit& E, const AggSpec& s, const std::vector<Column
         E.os << ind << "const Dec<" << (B + 3) / 4 << "> " << loaded[c] << " = ";
         if (it != sp.col.end())
             E.os << "wfull ? load_staged<" << B << ">(wstg + " << sp.off[it - sp.col.begin()] << "u, lane) : ";
-        E.os << "load_compact<" << B << ">(a.col[" << c << "], row, a.colbytes[" << c << "]);\n";
+        E.os << "col_load<" << B << ">(a, " << c << ", row);\n";
     }
 }
 
@@ -1371,8 +1393,18 @@ static int quad_words(const std::vector<int>& incols, const std::vector<ColumnTy
 // quad, columns loaded as quads and decoded per row.  Returns the body's
 // indentation.
 static std::string emit_row_loop_open(Emit& E, const AggSpec& s, const std::vector<ColumnType>& cols, int rps,
-                                      const StagePlan& sp, bool keys, std::vector<std::string>& loaded)
+                                      const StagePlan& sp, bool keys, std::vector<std::string>& loaded,
+                                      bool blocks = false)
 {
+    if (blocks) {
+        // block-column inputs: row i reads position idx[i] (or i); NULL
+        // positions (-1) and positions past the columns add nothing
+        E.os << "  for (usize i_ = lblk * T + tid; i_ < a.rows; i_ += (usize)gridDim.x * T) {\n";
+        E.os << "    const i64 pos_ = a.idx ? (i64)a.idx[i_] : (i64)i_;\n";
+        E.os << "    if (pos_ < 0 || (usize)pos_ >= a.rowlimit) continue;\n";
+        E.os << "    const usize row = (usize)pos_;\n";
+        return "    ";
+    }
     if (rps == 1) {
         E.os << "  for (usize row = lblk * T + tid; row < a.rows; row += (usize)gridDim.x * T) {\n";
         if (s.filter_col >= 0) E.os << "    const i32 fv = a.fcol[row];\n";
@@ -1429,7 +1461,8 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
     // [i*OB, (i+1)*OB) of the block's span); other widths are assembled in an
     // LDS tile and copied out in dwords.
     const int VB = OB % 16 == 0 ? 16 : (OB == 8 || OB == 4) ? OB : 0;
-    E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name << "(ColArgs a)\n{\n";
+    E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name << "(" << (o.blocks ? "IterArgs" : "ColArgs")
+         << " a)\n{\n";
     E.os << "  __shared__ u32 tile[(" << T << " * " << OB << " + 3) / 4];\n";
     E.os << "  u8* t8 = reinterpret_cast<u8*>(tile);\n";
     if (VB) E.os << "  const bool direct = ((usize)a.out & " << VB - 1 << "u) == 0;\n";
@@ -1438,8 +1471,24 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
     E.os << "    const usize row = base + threadIdx.x;\n";
     E.os << "    if (row < a.rows) {\n";
     std::vector<std::string> loaded(cols.size());
-    std::string rv = emit_plan_any(E, p, cols, loaded, "      ", "", plan_fn_name(0));
-    E.os << "      const Mag<" << (OB + 3) / 4 << "> enc = encode_compact<" << OB << ", " << p.n[p.root].W << ">(" << rv << ");\n";
+    if (o.blocks) {
+        // output row `row` evaluates input position idx[row] (or row); a
+        // NULL position (-1), or one past the columns, gives a zero row
+        // with valid = 0
+        E.os << "      const i64 pos_ = a.idx ? (i64)a.idx[row] : (i64)row;\n";
+        E.os << "      const bool ok_ = pos_ >= 0 && (usize)pos_ < a.rowlimit;\n";
+        E.os << "      if (a.valid) a.valid[row] = ok_ ? 1u : 0u;\n";
+        E.os << "      Mag<" << (OB + 3) / 4 << "> enc = mzero<" << (OB + 3) / 4 << ">();\n";
+        E.os << "      if (ok_) {\n";
+        E.os << "        const usize row = (usize)pos_;\n";
+        std::string rv = emit_plan_any(E, p, cols, loaded, "        ", "", plan_fn_name(0));
+        E.os << "        enc = encode_compact<" << OB << ", " << p.n[p.root].W << ">(" << rv << ");\n";
+        E.os << "      }\n";
+    } else {
+        std::string rv = emit_plan_any(E, p, cols, loaded, "      ", "", plan_fn_name(0));
+        E.os << "      const Mag<" << (OB + 3) / 4 << "> enc = encode_compact<" << OB << ", " << p.n[p.root].W << ">(" << rv
+             << ");\n";
+    }
     if (VB) {
         E.os << "      if (direct) {\n";
         if (VB == 4) {
@@ -1478,7 +1527,7 @@ int agg_acc_words(const AggSpec& s) { return words_for(agg_value_bits(s) + 40 + 
 
 int agg_rows_per_step(const AggSpec& s, const std::vector<ColumnType>& cols, const GenOptions& o)
 {
-    if (!o.row_quads) return 1;
+    if (!o.row_quads || o.blocks) return 1;
     const std::vector<int> inl = inline_columns(s);
     const int w = quad_words(inl, cols);
     const int extra = (s.filter_col >= 0 ? 4 : 0) + s.nkeys;
@@ -1808,9 +1857,11 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
     const int vbits = agg_value_bits(s);
     const int AW1 = words_for(vbits + kAggRowsPerLaneLog2 + 1);
     E.os << "#define AW1 " << AW1 << "\n";
-    E.os << "extern \"C\" __global__ void __launch_bounds__(" << c.T << ") " << name << "_main(AggArgs a)\n{\n";
+    E.os << "extern \"C\" __global__ void __launch_bounds__(" << c.T << ") " << name << "_main("
+         << (o.blocks ? "IterArgs" : "AggArgs") << " a)\n{\n";
     E.os << "  const u32 tid = threadIdx.x;\n  const u32 wave = tid >> 6, lane = tid & 63u;\n";
-    const StagePlan sp = (reg && agg_rows_per_step(s, cols, o) == 1) ? stage_plan(s, cols) : StagePlan();
+    if (o.blocks && (!reg || s.filter_col >= 0 || s.nkeys > 0)) return std::string();
+    const StagePlan sp = (reg && !o.blocks && agg_rows_per_step(s, cols, o) == 1) ? stage_plan(s, cols) : StagePlan();
     emit_stage_decl(E, sp, c.T / 64);
     if (!reg) {
         E.os << "  __shared__ u64 aslot[NCOPY * NS * NP * AWR];\n  __shared__ u32 acnt[NCOPY * NS];\n";
@@ -1825,7 +1876,7 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
     E.os << "  const usize lblk = xcd_block();\n";
     std::vector<std::string> loaded(cols.size());
     const int rps = agg_rows_per_step(s, cols, o);
-    const std::string bi = emit_row_loop_open(E, s, cols, rps, sp, !reg, loaded);
+    const std::string bi = emit_row_loop_open(E, s, cols, rps, sp, !reg, loaded, o.blocks);
     if (s.filter_col >= 0) E.os << bi << "if (!(fv " << fops[s.filter_op] << " (i64)" << s.filter_val << "LL)) continue;\n";
     if (reg) {
         for (int i = 0; i < NP; i++) {

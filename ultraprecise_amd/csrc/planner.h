@@ -127,6 +127,9 @@ struct GenOptions {
     // aggregation row loops take a lane's four consecutive rows per step
     // ("row quads") when the columns they load are narrow enough
     bool row_quads = true;
+    // block-column inputs (IterArgs: blocks, block start rows, optional row
+    // indices with -1 for NULL) instead of one contiguous array per column
+    bool blocks = false;
 };
 
 // Row quads (aggregation row loops): a lane owns rows [4q, 4q + 4) and

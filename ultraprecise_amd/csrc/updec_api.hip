@@ -227,6 +227,45 @@ struct AggArgs {
     const u8* lut;
     u32* partials;
 };
+// block-column kernel arguments (the generated kernels' IterArgs)
+struct IterArgs {
+    const u8* const* blocks[UPD_MAXCOLS];
+    const i64* starts[UPD_MAXCOLS];
+    i32 nblocks[UPD_MAXCOLS];
+    usize colrows[UPD_MAXCOLS];
+    const i32* idx;
+    usize rows;
+    usize rowlimit;
+    u8* out;
+    u8* valid;
+    u32* partials;
+};
+static_assert(sizeof(IterArgs) == 496, "IterArgs layout");
+
+// validate block-column descriptors and fill the kernel arguments
+static int fill_iter(IterArgs* a, const std::vector<ColumnType>& types, const updec_column_blocks* cols,
+                     const int32_t* d_indices, int64_t rows)
+{
+    *a = IterArgs{};
+    usize limit = ~(usize)0;
+    for (size_t i = 0; i < types.size(); i++) {
+        const updec_column_blocks& c = cols[i];
+        if (c.nblocks < 1 || !c.d_blocks || !c.d_starts || c.rows < 0)
+            return api_fail(UPDEC_EINVAL, "column " + std::to_string(i) + ": no blocks");
+        a->blocks[i] = c.d_blocks;
+        a->starts[i] = reinterpret_cast<const i64*>(c.d_starts);
+        a->nblocks[i] = c.nblocks;
+        a->colrows[i] = (usize)c.rows;
+        limit = std::min(limit, (usize)c.rows);
+    }
+    if (types.empty()) limit = 0;
+    // without indices row i reads position i: every column must hold the rows
+    if (!d_indices && (usize)rows > limit) return api_fail(UPDEC_EINVAL, "rows exceed the columns' rows");
+    a->idx = d_indices;
+    a->rows = (usize)rows;
+    a->rowlimit = limit;
+    return UPDEC_OK;
+}
 
 __global__ void __launch_bounds__(256) k_gen_compact(updec_gen::Desc d, uint64_t key, int64_t row0, int64_t rows,
                                                      u8* out, int nbytes)
@@ -367,11 +406,16 @@ struct updec_expr_s {
     std::string name, src;
     ModuleP mod;
     hipFunction_t fn = nullptr;
+    // the block-column variant, generated and compiled at first use
+    ModuleP bmod;
+    hipFunction_t bfn = nullptr;
 };
 
 struct updec_agg_s {
     AggSpec spec;
     std::vector<ColumnType> types;
+    ModuleP bmod;   // the block-column variant of the one-group form
+    int bblock = 256;
     int nkeys = 0, has_filter = 0;
     std::string last_src;
     struct GenMod {
@@ -634,6 +678,33 @@ int updec_expr_calc(updec_expr_t e, const uint8_t* const* d_cols, int64_t rows, 
     return UPDEC_OK;
 }
 
+int updec_expr_calc_blocks(updec_expr_t e, const updec_column_blocks* cols, const int32_t* d_indices, int64_t rows,
+                           uint8_t* d_out, uint8_t* d_valid, updec_stream_t stream)
+{
+    if (!e || rows < 0 || (rows && (!d_out || (!cols && !e->types.empty())))) return fail(UPDEC_EINVAL, "bad argument");
+    if (reinterpret_cast<uintptr_t>(d_out) & 3u) return fail(UPDEC_EINVAL, "output must be 4-byte aligned");
+    IterArgs a;
+    int rc = fill_iter(&a, e->types, cols, d_indices, rows);
+    if (rc != UPDEC_OK) return rc;
+    if (rows == 0) return UPDEC_OK;
+    std::string err;
+    if (!e->bmod) {
+        GenOptions o;
+        o.blocks = true;
+        e->bmod = ModuleCache::get().compile(gen_calc_kernel(e->plan, e->types, "calc_blocks", o), &err);
+        if (!e->bmod) return fail(UPDEC_ECOMPILE, err);
+        e->bfn = e->bmod->fn("calc_blocks", &err);
+        if (!e->bfn) return fail(UPDEC_ECOMPILE, err);
+    }
+    a.out = d_out;
+    a.valid = d_valid;
+    int64_t tiles = (rows + 255) / 256;
+    unsigned grid = (unsigned)std::min<int64_t>(tiles, (int64_t)device_cus() * 32);
+    void* params[] = {&a};
+    HIPCHK(hipModuleLaunchKernel(e->bfn, grid, 1, 1, 256, 1, 1, 0, S(stream), params, nullptr));
+    return UPDEC_OK;
+}
+
 void updec_expr_free(updec_expr_t e) { delete e; }
 
 int updec_agg_compile(const updec_agg_desc* d, updec_agg_t* out)
@@ -891,6 +962,53 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
     uint64_t* cnt = reinterpret_cast<uint64_t*>(d_counts);
     void* pf[] = {&cp, &nb, &img, &cnt};
     HIPCHK(hipModuleLaunchKernel(ffin, (unsigned)(NP * ngroups + ngroups), 1, 1, 256, 1, 1, 0, st, pf, nullptr));
+    return UPDEC_OK;
+}
+
+int updec_agg_run_blocks(updec_agg_t a, const updec_column_blocks* cols, const int32_t* d_indices, int64_t rows,
+                         uint8_t* d_images, int64_t* d_counts, updec_stream_t stream)
+{
+    if (!a || rows < 0 || !d_images || !d_counts || (!cols && !a->types.empty())) return fail(UPDEC_EINVAL, "bad argument");
+    if (a->has_filter || a->nkeys > 0)
+        return fail(UPDEC_EUNSUPPORTED, "block-column inputs take ungrouped, unfiltered aggregations");
+    IterArgs args;
+    int rc = fill_iter(&args, a->types, cols, d_indices, rows);
+    if (rc != UPDEC_OK) return rc;
+    std::string err;
+    if (!a->bmod) {
+        AggSpec s = a->spec;
+        s.ngroups = 1;
+        GenOptions o;
+        o.blocks = true;
+        int aw = 0;
+        std::string src = gen_agg_kernels(s, a->types, "agg_blocks", o, &aw, &a->bblock);
+        if (src.empty()) return fail(UPDEC_EUNSUPPORTED, "no block-column form for this aggregation");
+        a->bmod = ModuleCache::get().compile(src, &err);
+        if (!a->bmod) return fail(UPDEC_ECOMPILE, err);
+    }
+    hipFunction_t fmain = a->bmod->fn("agg_blocks_main", &err);
+    hipFunction_t ffin = fmain ? a->bmod->fn("agg_blocks_final", &err) : nullptr;
+    if (!fmain || !ffin) return fail(UPDEC_ECOMPILE, err);
+    hipStream_t st = S(stream);
+    const int T = a->bblock;
+    const int NP = (int)a->spec.plans.size();
+    const size_t PW = (size_t)NP * agg_acc_words(a->spec) + 2;
+    const int64_t grid = agg_grid(fmain, rows, T, 1);
+    if ((size_t)grid * PW > a->partial_words) {
+        if (a->d_partials) HIPCHK(hipFree(a->d_partials));
+        a->d_partials = nullptr;
+        HIPCHK(hipMalloc((void**)&a->d_partials, (size_t)grid * PW * sizeof(u32)));
+        a->partial_words = (size_t)grid * PW;
+    }
+    args.partials = a->d_partials;
+    void* pm[] = {&args};
+    HIPCHK(hipModuleLaunchKernel(fmain, (uint32_t)grid, 1, 1, T, 1, 1, 0, st, pm, nullptr));
+    const u32* cp = a->d_partials;
+    u32 nb = (u32)grid;
+    uint8_t* img = d_images;
+    uint64_t* cnt = reinterpret_cast<uint64_t*>(d_counts);
+    void* pf[] = {&cp, &nb, &img, &cnt};
+    HIPCHK(hipModuleLaunchKernel(ffin, (unsigned)(NP + 1), 1, 1, 256, 1, 1, 0, st, pf, nullptr));
     return UPDEC_OK;
 }
 
