@@ -294,29 +294,35 @@ template <int NX, int NB> UPD_DEV void divmod(const Mag<NX>& X, const Mag<NB>& B
                 rhat += fix ? d1 : 0u;
             }
         }
-        // un[j .. j+NB] -= qhat * vn
-        u64 kk = 0;
-        i64 t;
+        // un[j .. j+NB] -= qhat * vn.  A zero digit changes nothing: the top
+        // digits of a dividend shorter than its static width are all zero,
+        // and skipping them (a wave-uniform branch when every lane's digit is
+        // zero) removes most of the work of a wide division
+        if (qhat != 0u) {
+            u64 kk = 0;
+            i64 t;
 #pragma unroll
-        for (int i = 0; i < NB; i++) {
-            u64 p = (u64)qhat * vn.w[i];
-            t = (i64)un.w[i + j] - (i64)kk - (i64)(p & 0xffffffffull);
-            un.w[i + j] = (u32)t;
-            kk = (p >> 32) - (u64)(t >> 32);
-        }
-        t = (i64)un.w[j + NB] - (i64)kk;
-        un.w[j + NB] = (u32)t;
-        // rare add-back when the estimate was one too large
-        const u32 neg = (t < 0) ? 0xffffffffu : 0u;
-        qhat += neg;  // qhat - 1 when negative
-        u64 c = 0;
+            for (int i = 0; i < NB; i++) {
+                u64 p = (u64)qhat * vn.w[i];
+                t = (i64)un.w[i + j] - (i64)kk - (i64)(p & 0xffffffffull);
+                un.w[i + j] = (u32)t;
+                kk = (p >> 32) - (u64)(t >> 32);
+            }
+            t = (i64)un.w[j + NB] - (i64)kk;
+            un.w[j + NB] = (u32)t;
+            // rare add-back when the estimate was one too large
+            if (t < 0) {
+                qhat -= 1u;
+                u64 c = 0;
 #pragma unroll
-        for (int i = 0; i < NB; i++) {
-            u64 s = (u64)un.w[i + j] + (vn.w[i] & neg) + c;
-            un.w[i + j] = (u32)s;
-            c = s >> 32;
+                for (int i = 0; i < NB; i++) {
+                    u64 s = (u64)un.w[i + j] + vn.w[i] + c;
+                    un.w[i + j] = (u32)s;
+                    c = s >> 32;
+                }
+                un.w[j + NB] += (u32)c;
+            }
         }
-        un.w[j + NB] += (u32)c & neg;
         Q.w[j] = qhat;
     }
     R = shr_var<NB>(resize<NB>(un), k);
