@@ -1225,12 +1225,15 @@ int plan_cost(const Plan& p)
     return cost;
 }
 
-bool plan_out_of_line(const Plan& p)
+static int plan_ops(const Plan& p)
 {
     int ops = 0;
     for (const PNode& x : p.n) ops += (x.kind != PK::Col && x.kind != PK::Const) ? 1 : 0;
-    return ops > kOutOfLineOps || plan_cost(p) > kOutOfLineCost;
+    return ops;
 }
+
+bool plan_out_of_line(const Plan& p) { return plan_ops(p) > kOutOfLineOps || plan_cost(p) > kOutOfLineCost; }
+
 
 // definition of the out-of-line evaluator `fname` (any args struct whose
 // first members are col[] and colbytes[])
@@ -1448,6 +1451,13 @@ static void emit_row_loop_close(Emit& E, int rps) { E.os << (rps == 4 ? "    }\n
 
 }  // namespace
 
+// A costly plan with few operations (wide limbs, not a long series) runs in
+// the calc kernel inline, one row per thread and no row loop: the loop
+// passes that made inline bodies slow to compile have no loop to work on
+// (2-3 s for a P = 300 division), and the evaluator's arguments and result
+// no longer pass through scratch (P = 150 sweep: 2.34 -> 1.87 ms).
+bool calc_one_row(const Plan& p) { return plan_out_of_line(p) && plan_ops(p) <= kOutOfLineOps; }
+
 std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, const std::string& name, const GenOptions& o)
 {
     Emit E;
@@ -1455,7 +1465,7 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
     const int OB = p.outB;
     E.os << "#include \"dec_device.h\"\nusing namespace updec;\n" << kArgsStruct;
     E.os << "// plan: " << p.text() << "\n";
-    emit_plan_functions(E, {p}, cols);
+    if (!calc_one_row(p)) emit_plan_functions(E, {p}, cols);
     // Rows whose image is 4, 8 or a multiple of 16 bytes go straight to HBM
     // as dword / dwordx2 / dwordx4 stores (coalesced: lane i writes bytes
     // [i*OB, (i+1)*OB) of the block's span); other widths are assembled in an
@@ -1467,10 +1477,18 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
     E.os << "  u8* t8 = reinterpret_cast<u8*>(tile);\n";
     if (VB) E.os << "  const bool direct = ((usize)a.out & " << VB - 1 << "u) == 0;\n";
     else E.os << "  const bool direct = false;\n";
-    E.os << "  for (usize base = xcd_block() * " << T << "; base < a.rows; base += (usize)gridDim.x * " << T << ") {\n";
+    const bool one_row = calc_one_row(p);
+    if (one_row)   // one block per T rows; `continue` leaves the do-while
+        E.os << "  do {\n    const usize base = (usize)blockIdx.x * " << T << ";\n    if (base >= a.rows) break;\n";
+    else
+        E.os << "  for (usize base = xcd_block() * " << T << "; base < a.rows; base += (usize)gridDim.x * " << T << ") {\n";
     E.os << "    const usize row = base + threadIdx.x;\n";
     E.os << "    if (row < a.rows) {\n";
     std::vector<std::string> loaded(cols.size());
+    // emit_plan_any calls the out-of-line evaluator; one_row inlines instead
+    auto body = [&](const std::string& ind) {
+        return one_row ? emit_plan(E, p, cols, loaded, ind, "") : emit_plan_any(E, p, cols, loaded, ind, "", plan_fn_name(0));
+    };
     if (o.blocks) {
         // output row `row` evaluates input position idx[row] (or row); a
         // NULL position (-1), or one past the columns, gives a zero row
@@ -1481,11 +1499,11 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
         E.os << "      Mag<" << (OB + 3) / 4 << "> enc = mzero<" << (OB + 3) / 4 << ">();\n";
         E.os << "      if (ok_) {\n";
         E.os << "        const usize row = (usize)pos_;\n";
-        std::string rv = emit_plan_any(E, p, cols, loaded, "        ", "", plan_fn_name(0));
+        std::string rv = body("        ");
         E.os << "        enc = encode_compact<" << OB << ", " << p.n[p.root].W << ">(" << rv << ");\n";
         E.os << "      }\n";
     } else {
-        std::string rv = emit_plan_any(E, p, cols, loaded, "      ", "", plan_fn_name(0));
+        std::string rv = body("      ");
         E.os << "      const Mag<" << (OB + 3) / 4 << "> enc = encode_compact<" << OB << ", " << p.n[p.root].W << ">(" << rv
              << ");\n";
     }
@@ -1512,7 +1530,7 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
     E.os << "    u8* dst = a.out + base * " << OB << ";\n";
     E.os << "    for (usize i = threadIdx.x; i < n / 4; i += " << T << ") reinterpret_cast<u32*>(dst)[i] = tile[i];\n";
     E.os << "    for (usize i = (n / 4) * 4 + threadIdx.x; i < n; i += " << T << ") dst[i] = t8[i];\n";
-    E.os << "    __syncthreads();\n  }\n}\n";
+    E.os << "    __syncthreads();\n  }" << (one_row ? " while (0);" : "") << "\n}\n";
     return E.os.str();
 }
 

@@ -146,6 +146,10 @@ constexpr int kQuadMaxWords = 40;
 // rows; the generated accumulator widths rely on it.
 constexpr int kAggRowsPerLaneLog2 = 14;
 
+// true when the calc kernel of this plan takes one row per thread with no
+// row loop (the launch needs ceil(rows / block) blocks)
+bool calc_one_row(const Plan& p);
+
 // calc kernel: one compact output row per input row (the reference's
 // calc_expr_<id>, AriesExprCalcNode.cpp:1285-1318)
 std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, const std::string& name,

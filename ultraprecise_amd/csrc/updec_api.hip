@@ -672,7 +672,8 @@ int updec_expr_calc(updec_expr_t e, const uint8_t* const* d_cols, int64_t rows, 
     a.rows = (usize)rows;
     a.out = d_out;
     int64_t tiles = (rows + 255) / 256;
-    unsigned grid = (unsigned)std::min<int64_t>(tiles, (int64_t)device_cus() * 32);
+    if (tiles > INT32_MAX) return fail(UPDEC_EINVAL, "too many rows for one launch");
+    unsigned grid = calc_one_row(e->plan) ? (unsigned)tiles : (unsigned)std::min<int64_t>(tiles, (int64_t)device_cus() * 32);
     void* params[] = {&a};
     HIPCHK(hipModuleLaunchKernel(e->fn, grid, 1, 1, 256, 1, 1, 0, S(stream), params, nullptr));
     return UPDEC_OK;
@@ -699,7 +700,8 @@ int updec_expr_calc_blocks(updec_expr_t e, const updec_column_blocks* cols, cons
     a.out = d_out;
     a.valid = d_valid;
     int64_t tiles = (rows + 255) / 256;
-    unsigned grid = (unsigned)std::min<int64_t>(tiles, (int64_t)device_cus() * 32);
+    if (tiles > INT32_MAX) return fail(UPDEC_EINVAL, "too many rows for one launch");
+    unsigned grid = calc_one_row(e->plan) ? (unsigned)tiles : (unsigned)std::min<int64_t>(tiles, (int64_t)device_cus() * 32);
     void* params[] = {&a};
     HIPCHK(hipModuleLaunchKernel(e->bfn, grid, 1, 1, 256, 1, 1, 0, S(stream), params, nullptr));
     return UPDEC_OK;
