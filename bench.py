@@ -445,8 +445,8 @@ def main():
     ap.add_argument("--serve-budget-gb", type=int, default=24, help="device memory for resident columns")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(the sharded form is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: the path runs on one "
+              "GPU here and has no multi-GPU form", file=sys.stderr)
         sys.exit(2)
     import ultraprecise_amd as U
     from ultraprecise_amd import workloads as W

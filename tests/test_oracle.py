@@ -104,7 +104,7 @@ def test_compact_bytes_table_holds_every_value(O):
         assert O.lib.od_compact_bytes(P) <= need + 1
     for P in (300, 301, 450, 602):
         assert O.lib.od_compact_bytes(P) == ((10 ** P - 1).bit_length() + 1 + 7) // 8
-    # values the survey lists for the configurations
+    # the widths of the workloads' precisions (decimal.cu:92 table)
     assert [O.lib.od_compact_bytes(P) for P in (15, 18, 38, 76, 150)] == [7, 8, 16, 32, 63]
 
 

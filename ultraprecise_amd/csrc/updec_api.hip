@@ -1,5 +1,5 @@
 // updec_api.hip -- the C ABI (include/updec.h): hipRTC module cache,
-// expression / aggregation drivers, and the static kernels (generator,
+// expression / aggregation launchers, and the static kernels (generator,
 // statistics, group discovery, AVG division).
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
