@@ -49,17 +49,16 @@ def headline(U, W, steps, warmup, cpu_rows, threads):
     bits = [U.column_maxbits(t, c[1], rows) for t, c in zip(dev, C["cols"])]
     cols = [(c[0], c[1], c[2], b) for c, b in zip(C["cols"], bits)]
     t0 = time.time()
-    agg = U.Agg(C["exprs"], cols, filter=C["filter"], nkeys=2)
+    # AVG(qty), AVG(price), AVG(disc) are divided in the aggregation's final
+    # kernel; their images follow the SUM images
+    agg = U.Agg(C["exprs"], cols, filter=C["filter"], nkeys=2, avg=(0, 1, 4))
     ne = len(C["exprs"])
     maxg = 16
-    images = U.DeviceBuffer(maxg * ne * 133).zero()
+    images = U.DeviceBuffer(maxg * (ne + 3) * 133).zero()
     counts = U.DeviceBuffer(maxg * 8).zero()
-    avgs = U.DeviceBuffer(3 * maxg * 133).zero()
 
     def step():
         ng, keys = agg.run(dev, rows, images, counts, filter_col=ship, key0=f1, key1=f2, max_groups=maxg)
-        for j, k in enumerate((0, 1, 4)):  # AVG(qty), AVG(price), AVG(disc)
-            U.div_int64(images, ne * 133, counts, avgs, ng, sums_offset=k * 133, out_offset=j * maxg * 133)
         return ng
 
     ng = step()

@@ -143,8 +143,9 @@ int updec_expr_calc_blocks(updec_expr_t e, const updec_column_blocks* cols, cons
  * merged on the device, or, when a block meets more than 16 keys, a key
  * discovery pass and a host-numbered key table first.
  * Output: images[g * nexprs + k] is the 133-byte nullable_type<Decimal>
- * image of SUM(expr_k) in group g; counts[g] = COUNT(*) of group g;
- * keys_out[g] = key0 | key1 << 8 of group g (host memory). */
+ * image of SUM(expr_k) in group g (AVG images follow, see avg_mask);
+ * counts[g] = COUNT(*) of group g; keys_out[g] = key0 | key1 << 8 of group
+ * g (host memory). */
 typedef struct updec_agg_desc {
     const updec_column_desc* cols;
     int32_t ncols;
@@ -155,6 +156,12 @@ typedef struct updec_agg_desc {
     int64_t filter_value;
     int32_t nkeys; /* 0, 1 or 2 one-byte key columns */
     int32_t mode;
+    /* bit k set: also AVG(expr_k) = SUM(expr_k) / COUNT(*) per group by the
+     * DivisionInt64 rule (as updec_div_int64 with this mode), written after
+     * the SUM images: d_images then holds max_groups * (nexprs + navg)
+     * images, AVG j of group g at index max_groups * nexprs + g * navg + j
+     * (navg = bits set, j counting them from bit 0) */
+    uint32_t avg_mask;
 } updec_agg_desc;
 
 int updec_agg_compile(const updec_agg_desc* desc, updec_agg_t* out);
@@ -171,8 +178,8 @@ void updec_agg_free(updec_agg_t a);
 /* updec_agg_run over block columns and indices, for aggregations without
  * filter and keys (one group): SUM of each expression over the positions
  * d_indices[0..rows) (or 0..rows); NULL positions (-1) and positions past a
- * column add to no SUM and to no COUNT.  d_images: nexprs 133-byte images;
- * d_counts: one int64. */
+ * column add to no SUM and to no COUNT.  d_images: nexprs 133-byte images,
+ * then one AVG image per avg_mask bit; d_counts: one int64. */
 int updec_agg_run_blocks(updec_agg_t a, const updec_column_blocks* cols, const int32_t* d_indices, int64_t rows,
                          uint8_t* d_images, int64_t* d_counts, updec_stream_t stream);
 

@@ -45,15 +45,19 @@ def test_config3_grouped_bit_exact(U, O, gpu, rows):
                 for i in range(0, rows, max(1, rows // 2000))]
         assert max(m.bit_length() for m in mags) <= b
     cols = [(c[0], c[1], c[2], b) for c, b in zip(C["cols"], bits)]
-    agg = U.Agg(C["exprs"], cols, filter=C["filter"], nkeys=2)
+    avg = [0, 1, 4]
+    agg = U.Agg(C["exprs"], cols, filter=C["filter"], nkeys=2, avg=avg)
     maxg = 16
-    images = U.DeviceBuffer(maxg * len(C["exprs"]) * 133).zero()
+    NE = len(C["exprs"])
+    images = U.DeviceBuffer(maxg * (NE + len(avg)) * 133).zero()
     counts = U.DeviceBuffer(maxg * 8).zero()
     ng, keys = agg.run(dev, rows, images, counts, filter_col=ship, key0=f1, key1=f2, max_groups=maxg)
     hkeys, gid = host_groups(hf1, hf2, hship, C["filter"][1])
     assert ng == len(hkeys)
     assert keys == [int((k >> 8) | ((k & 0xff) << 8)) for k in hkeys]
-    got = images.numpy().reshape(maxg, len(C["exprs"]), 133)
+    allimg = images.numpy()
+    got = allimg[:maxg * NE * 133].reshape(maxg, NE, 133)
+    fused = allimg[maxg * NE * 133:].reshape(maxg, len(avg), 133)
     gotc = counts.numpy("int64")[:ng]
     P = [c[1] for c in cols]
     S = [c[2] for c in cols]
@@ -63,9 +67,10 @@ def test_config3_grouped_bit_exact(U, O, gpu, rows):
         assert np.array_equal(gotc, wc)
         for g in range(ng):
             assert bytes(got[g, k]) == bytes(want[g]), f"SUM({text}) group {g}"
-    # AVG(qty), AVG(price), AVG(disc) through the division kernel
+    # AVG(qty), AVG(price), AVG(disc): fused into the aggregation, and
+    # through the separate division kernel
     avg2 = U.DeviceBuffer(maxg * 133).zero()
-    for k in (0, 1, 4):
+    for j, k in enumerate(avg):
         # contiguous copy of column k's sums
         sums = U.DeviceBuffer.from_numpy(np.ascontiguousarray(got[:ng, k]).reshape(-1))
         U.div_int64(sums, 133, counts, avg2, ng)
@@ -73,6 +78,7 @@ def test_config3_grouped_bit_exact(U, O, gpu, rows):
         for g in range(ng):
             want = O.avg_from_sum(bytes(got[g, k]), int(gotc[g]))
             assert bytes(ga2[g]) == want, f"AVG expr {k} group {g}"
+            assert bytes(fused[g, j]) == want, f"fused AVG expr {k} group {g}"
 
 
 @pytest.mark.parametrize("nf1,nf2,nkeys", [(40, 3, 2), (3, 2, 2), (7, 1, 1), (60, 1, 1)])
@@ -92,9 +98,9 @@ def test_grouping_many_keys_and_fallback(U, O, gpu, nf1, nf2, nkeys):
     hf1, hf2 = f1.numpy(count=rows), f2.numpy(count=rows)
     cols = [(c[0], c[1], c[2]) for c in spec]
     exprs = ["qty", "price*qty"]
-    agg = U.Agg(exprs, cols, nkeys=nkeys)
+    agg = U.Agg(exprs, cols, nkeys=nkeys, avg=[1])
     maxg = 255
-    images = U.DeviceBuffer(maxg * len(exprs) * 133).zero()
+    images = U.DeviceBuffer(maxg * (len(exprs) + 1) * 133).zero()
     counts = U.DeviceBuffer(maxg * 8).zero()
     ng, keys = agg.run(dev, rows, images, counts, key0=f1, key1=f2 if nkeys == 2 else None, max_groups=maxg)
     key = (hf1.astype(np.int32) << 8) | hf2 if nkeys == 2 else hf1.astype(np.int32)
@@ -103,13 +109,17 @@ def test_grouping_many_keys_and_fallback(U, O, gpu, nf1, nf2, nkeys):
     lut = np.full(65536, -1, dtype=np.int32)
     lut[hkeys] = np.arange(len(hkeys), dtype=np.int32)
     gid = lut[key].astype(np.int32)
-    got = images.numpy().reshape(maxg, len(exprs), 133)
+    allimg = images.numpy()
+    got = allimg[:maxg * len(exprs) * 133].reshape(maxg, len(exprs), 133)
+    fused = allimg[maxg * len(exprs) * 133:].reshape(maxg, 133)
     gotc = counts.numpy("int64")[:ng]
     for k, text in enumerate(exprs):
         e = U.Expr(text, cols)
         want, wc = O.plan_group_sum(O.Plan(e.plan), host, [c[1] for c in cols], [c[2] for c in cols], rows, gid, ng)
         assert np.array_equal(gotc, wc)
         assert np.array_equal(got[:ng, k], want), text
+    for g in range(ng):
+        assert bytes(fused[g]) == O.avg_from_sum(bytes(got[g, 1]), int(gotc[g])), f"fused AVG group {g}"
 
 
 @pytest.mark.parametrize("mode", ["faithful", "extended"])
@@ -149,6 +159,35 @@ def test_div_int64_edges(U, O, gpu, mode):
     got = out.numpy().reshape(n, 133)
     for i in range(n):
         assert bytes(got[i]) == O.avg_from_sum(imgs[i], cnts[i], mode=om), i
+
+
+@pytest.mark.parametrize("mode", ["faithful", "extended"])
+@pytest.mark.parametrize("nkeys", [0, 1])
+def test_fused_avg_modes(U, O, gpu, mode, nkeys):
+    """fused AVG of a wide sum (DECIMAL(38,10) products) in both modes,
+    ungrouped and grouped, against the division of the SUM image"""
+    rows = 50_000
+    seed = 23
+    spec = [("a", 38, 10), ("b", 20, 5)]
+    dev = [U.gen_compact_device(U.gen_desc(U.GEN_DIGITS, P, seed, i, digits=P - 2, neg_permille=400), rows)
+           for i, (n, P, S) in enumerate(spec)]
+    f1 = U.gen_byte_device(seed, 7, bytes([3, 9, 12]), rows)
+    exprs = ["a", "a*b", "b"]
+    um, om = (U.MODE_FAITHFUL, O.FAITHFUL) if mode == "faithful" else (U.MODE_EXTENDED, O.EXTENDED)
+    agg = U.Agg(exprs, spec, nkeys=nkeys, mode=um, avg=[0, 1])
+    maxg = 4
+    images = U.DeviceBuffer(maxg * 5 * 133).zero()
+    counts = U.DeviceBuffer(maxg * 8).zero()
+    ng, keys = agg.run(dev, rows, images, counts, key0=f1 if nkeys else None, max_groups=maxg)
+    assert ng == (3 if nkeys else 1)
+    allimg = images.numpy()
+    got = allimg[:maxg * 3 * 133].reshape(maxg, 3, 133)
+    fused = allimg[maxg * 3 * 133:maxg * 5 * 133].reshape(maxg, 2, 133)
+    gotc = counts.numpy("int64")[:ng]
+    assert int(gotc.sum()) == rows
+    for g in range(ng):
+        for j, k in enumerate([0, 1]):
+            assert bytes(fused[g, j]) == O.avg_from_sum(bytes(got[g, k]), int(gotc[g]), mode=om), (g, k)
 
 
 @pytest.mark.parametrize("P,S", [(15, 2), (38, 10), (76, 20)])

@@ -92,8 +92,8 @@ def test_agg_blocks_bit_exact(U, O, gpu, text, specs):
     pos = rng.integers(0, rows, size=n).astype(np.int64)
     pos[rng.random(n) < 0.1] = -1
     idx = U.DeviceBuffer.from_numpy(pos.astype(np.int32))
-    agg = U.Agg([text], cols)
-    images = U.DeviceBuffer(133).zero()
+    agg = U.Agg([text], cols, avg=[0])
+    images = U.DeviceBuffer(2 * 133).zero()
     counts = U.DeviceBuffer(8).zero()
     agg.run_blocks(bcols, n, images, counts, indices=idx)
     keep = pos[pos >= 0]
@@ -102,7 +102,10 @@ def test_agg_blocks_bit_exact(U, O, gpu, text, specs):
     want, cnt = O.plan_sum(O.Plan(e.plan), ghost, [P for P, S in specs], [S for P, S in specs], len(keep),
                            mode=O.FAITHFUL if e.faithful else O.EXTENDED)
     assert int(counts.numpy("int64")[0]) == len(keep) == cnt
-    assert bytes(images.numpy()) == want
+    img = bytes(images.numpy())
+    assert img[:133] == want
+    # MODE_AUTO divides by the faithful rule, as div_int64 does
+    assert img[133:] == O.avg_from_sum(want, cnt, mode=O.FAITHFUL)
 
 
 def test_blocks_argument_errors(U, gpu):

@@ -30,7 +30,7 @@ class AggDesc(ctypes.Structure):
     _fields_ = [("cols", ctypes.POINTER(ColumnDesc)), ("ncols", ctypes.c_int32),
                 ("exprs", ctypes.POINTER(ctypes.c_char_p)), ("nexprs", ctypes.c_int32),
                 ("has_filter", ctypes.c_int32), ("filter_op", ctypes.c_int32), ("filter_value", ctypes.c_int64),
-                ("nkeys", ctypes.c_int32), ("mode", ctypes.c_int32)]
+                ("nkeys", ctypes.c_int32), ("mode", ctypes.c_int32), ("avg_mask", ctypes.c_uint32)]
 
 
 class ColumnBlocks(ctypes.Structure):
@@ -354,14 +354,21 @@ FILTER_OPS = {"<=": 0, "<": 1, ">=": 2, ">": 3, "==": 4, "!=": 5}
 
 class Agg:
     """Fused SUM(expr_k) / COUNT(*) with an optional int32 filter and up to
-    two one-byte group keys."""
+    two one-byte group keys.  avg: indices k of the expressions whose
+    AVG = SUM / COUNT(*) is also produced, after the SUM images."""
 
-    def __init__(self, exprs, cols, filter=None, nkeys=0, mode=MODE_AUTO):
+    def __init__(self, exprs, cols, filter=None, nkeys=0, mode=MODE_AUTO, avg=()):
         self.cols = [tuple(c) for c in cols]
         self.exprs = list(exprs)
+        self.avg = sorted(set(int(k) for k in avg))
+        mask = 0
+        for k in self.avg:
+            if not 0 <= k < min(32, len(self.exprs)):
+                raise ValueError("avg names an expression past the list")
+            mask |= 1 << k
         arr, keep = _col_array(self.cols)
         ex = (ctypes.c_char_p * len(self.exprs))(*[e.encode() for e in self.exprs])
-        d = AggDesc(arr, len(self.cols), ex, len(self.exprs), 0, 0, 0, nkeys, mode)
+        d = AggDesc(arr, len(self.cols), ex, len(self.exprs), 0, 0, 0, nkeys, mode, mask)
         if filter is not None:
             d.has_filter, d.filter_op, d.filter_value = 1, FILTER_OPS[filter[0]], int(filter[1])
         h = ctypes.c_void_p()
@@ -380,7 +387,9 @@ class Agg:
 
     def run(self, columns, rows, images, counts, filter_col=None, key0=None, key1=None, max_groups=255, stream=None):
         """Returns (ngroups, keys).  images: DeviceBuffer of at least
-        max_groups * nexprs * 133 bytes; counts: DeviceBuffer of max_groups int64."""
+        max_groups * (nexprs + len(avg)) * 133 bytes, SUM image of expression k
+        in group g at g * nexprs + k, AVG j at max_groups * nexprs + g * len(avg)
+        + j; counts: DeviceBuffer of max_groups int64."""
         ptrs = (ctypes.c_void_p * max(1, len(columns)))(*[c.ptr for c in columns])
         ng = ctypes.c_int32(0)
         keys = (ctypes.c_uint16 * max(1, max_groups))()

@@ -1064,6 +1064,46 @@ template <int AW> UPD_DEV void write_sum_image(u8* out, const Mag<AW>& acc, u32 
     }
 }
 
+// AVG = SUM / COUNT of one group (kernel_util.hxx:427-438, do_div):
+// Decimal(sum image) /= Decimal((int64)count) by the division rule, scale
+// S + 4; s and o are 133-byte nullable_type<Decimal> images.  Sums that fit
+// 8 limbs divide at that width (the quotient does not depend on the
+// operand widths).
+template <int FAITH> UPD_DEV void div_image_int64(const u8* s, i64 cnt, u8* o)
+{
+    Dec<32> a;
+    a.neg = s[1];
+    const u32 prec = s[2], frac = s[3], error = s[4];
+#pragma unroll
+    for (int w = 0; w < 32; w++)
+        a.m.w[w] = (u32)s[5 + 4 * w] | ((u32)s[6 + 4 * w] << 8) | ((u32)s[7 + 4 * w] << 16) | ((u32)s[8 + 4 * w] << 24);
+    Dec<2> b;
+    const u64 uc = cnt < 0 ? (u64)(-(cnt + 1)) + 1u : (u64)cnt;
+    b.neg = cnt < 0 ? 1u : 0u;
+    b.m.w[0] = (u32)uc;
+    b.m.w[1] = (u32)(uc >> 32);
+    u32 err = error;
+    bool narrow = true;
+#pragma unroll
+    for (int w = 8; w < 32; w++) narrow = narrow && a.m.w[w] == 0u;
+    Dec<32> q;
+    if (narrow) q = dresize<32>(ddiv<10, 8, 2, 4, FAITH>(dresize<8>(a), b, &err));
+    else q = ddiv<32, 32, 2, 4, FAITH>(a, b, &err);
+    const u8 flag = s[0];
+    o[0] = flag;
+    o[1] = (u8)q.neg;
+    o[2] = is_zero(a.m) ? (u8)(frac + 4u + 1u) : (u8)prec;
+    o[3] = (u8)(frac + 4u);
+    o[4] = (u8)err;
+#pragma unroll
+    for (int w = 0; w < 32; w++) {
+        o[5 + 4 * w] = (u8)q.m.w[w];
+        o[6 + 4 * w] = (u8)(q.m.w[w] >> 8);
+        o[7 + 4 * w] = (u8)(q.m.w[w] >> 16);
+        o[8 + 4 * w] = (u8)(q.m.w[w] >> 24);
+    }
+}
+
 } // namespace updec
 
 #endif

@@ -1794,7 +1794,7 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     // final: meta[0] = ngroups, meta[1] = 1 when ngroups > maxg, meta[2..] = keys
     E.os << "UPD_DEV usize rec_at(u32 i, u32 nblocks) { const u32 sl = i / nblocks; return (usize)sl * EW * nblocks + (i - sl * nblocks); }\n";
     E.os << "extern \"C\" __global__ void __launch_bounds__(256) " << name
-         << "_final(const u32* __restrict__ rec, u32 nblocks, const u32* __restrict__ used, u32 maxg, u8* __restrict__ images, u64* __restrict__ counts, u32* __restrict__ meta)\n{\n";
+         << "_final(const u32* __restrict__ rec, u32 nblocks, const u32* __restrict__ used, u32 maxg, u8* __restrict__ images, u64* __restrict__ counts, u32* __restrict__ meta, u8* __restrict__ avgs)\n{\n";
     E.os << "  const u32 tid = threadIdx.x;\n";
     E.os << "  __shared__ u32 bm[2048];\n  __shared__ u32 pre[2049];\n  __shared__ u32 part[4];\n";
     E.os << "  __shared__ u32 red[4 * AW2 + 8];\n  __shared__ u32 gkey;\n";
@@ -1832,14 +1832,36 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     E.os << "  for (u32 w = tid; w < 2048; w += 256) { u32 x = bm[w]; u32 r = pre[w];\n";
     E.os << "    while (x) { const u32 b = __ffs(x) - 1u; x &= x - 1u; if (r == g) gkey = w * 32u + b + 1u; r++; } }\n";
     E.os << "  __syncthreads();\n  const u32 want = gkey;\n";
+    // a SUM block of an AVG expression also counts its group, then divides
+    const bool avg = s.avg_mask != 0;
+    int navg = 0;
+    for (int k = 0; k < NP; k++) navg += (s.avg_mask >> k) & 1u;
     E.os << "  if (pp < NP) {\n    Mag<AW2> sm = mzero<AW2>();\n";
+    if (avg) E.os << "    const bool avg = ((" << s.avg_mask << "u >> pp) & 1u) != 0u;\n    u64 c = 0;\n";
     E.os << "    for (u32 i0 = tid; i0 < nrec; i0 += 256u * 16u) {\n      u32 kk[16];\n      usize at[16];\n";
     E.os << "#pragma unroll\n      for (int j = 0; j < 16; j++) { const u32 i = i0 + 256u * j; at[j] = i < nrec ? rec_at(i, nblocks) : 0; kk[j] = i < nrec ? rec[at[j]] : 0u; }\n";
     E.os << "#pragma unroll\n      for (int j = 0; j < 16; j++) {\n        if (kk[j] != want) continue;\n";
     E.os << "        Mag<AW2> o;\n#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = rec[at[j] + (usize)(1 + pp * AW2 + w) * nblocks];\n";
-    E.os << "        add_to<AW2>(sm, sm, o);\n      }\n    }\n";
+    E.os << "        add_to<AW2>(sm, sm, o);\n";
+    if (avg)
+        E.os << "        if (avg) c += (u64)rec[at[j] + (usize)(1 + NP * AW2) * nblocks] | ((u64)rec[at[j] + (usize)(2 + NP * AW2) * nblocks] << 32);\n";
+    E.os << "      }\n    }\n";
+    if (avg) {
+        E.os << "    __shared__ u64 cred[4];\n";
+        E.os << "#pragma unroll\n    for (int off = 32; off >= 1; off >>= 1) c += (u64)__shfl_xor((long long)c, off, 64);\n";
+        E.os << "    if ((tid & 63u) == 0) cred[tid >> 6] = c;\n";
+    }
     E.os << kBlockReduceAW2;
     emit_agg_final_image(E, s, "images + ((usize)g * NP + pp) * 133u");
+    if (avg) {
+        E.os << "      if (avg) {\n        u32 aj = 0;\n";
+        for (int k = 0, j = 0; k < NP; k++)
+            if ((s.avg_mask >> k) & 1u) E.os << "        if (pp == " << k << ") aj = " << j++ << "u;\n";
+        E.os << "        div_image_int64<" << (s.avg_faithful ? 1 : 0)
+             << ">(images + ((usize)g * NP + pp) * 133u, (i64)(cred[0] + cred[1] + cred[2] + cred[3]), avgs + ((usize)g * "
+             << navg << " + aj) * 133u);\n";
+        E.os << "      }\n";
+    }
     E.os << "    }\n  } else {\n    u64 c = 0;\n";
     E.os << "    for (u32 i0 = tid; i0 < nrec; i0 += 256u * 16u) {\n      u32 kk[16];\n      usize at[16];\n";
     E.os << "#pragma unroll\n      for (int j = 0; j < 16; j++) { const u32 i = i0 + 256u * j; at[j] = i < nrec ? rec_at(i, nblocks) : 0; kk[j] = i < nrec ? rec[at[j]] : 0u; }\n";

@@ -171,6 +171,10 @@ struct AggSpec {
     // per-block tables by key.  Needs nkeys > 0 and 64-bit accumulators.
     bool onepass = false;
     int maxslots = 16;
+    // AVG(expr_k) = SUM / COUNT for every bit k set, written by the final
+    // kernel of the one-pass form (DivisionInt64 rule, faithful or exact)
+    uint32_t avg_mask = 0;
+    bool avg_faithful = true;
 };
 
 // true when the one-pass grouped form applies to this aggregation

@@ -357,45 +357,12 @@ __global__ void __launch_bounds__(256) k_key_presence(const u8* key0, const u8* 
 
 // AVG: Decimal(sum) /= Decimal((int64)count) per group (kernel_util.hxx:427-438)
 template <int FAITH>
-__global__ void __launch_bounds__(64) k_div_int64(const u8* sums, int64_t stride, const int64_t* counts, u8* out, int64_t n)
+__global__ void __launch_bounds__(64) k_div_int64(const u8* sums, int64_t stride, const int64_t* counts, u8* out, int64_t n,
+                                                  int64_t out_stride)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const u8* s = sums + i * stride;
-    Dec<32> a;
-    a.neg = s[1];
-    const u32 prec = s[2], frac = s[3], error = s[4];
-#pragma unroll
-    for (int w = 0; w < 32; w++)
-        a.m.w[w] = (u32)s[5 + 4 * w] | ((u32)s[6 + 4 * w] << 8) | ((u32)s[7 + 4 * w] << 16) | ((u32)s[8 + 4 * w] << 24);
-    const int64_t cnt = counts[i];
-    Dec<2> b;
-    const u64 uc = cnt < 0 ? (u64)(-(cnt + 1)) + 1u : (u64)cnt;
-    b.neg = cnt < 0 ? 1u : 0u;
-    b.m.w[0] = (u32)uc;
-    b.m.w[1] = (u32)(uc >> 32);
-    u32 err = error;
-    // sums of up to 8 limbs (every sum of narrow columns) divide at that
-    // width: the quotient's value does not depend on the operand widths
-    bool narrow = true;
-#pragma unroll
-    for (int w = 8; w < 32; w++) narrow = narrow && a.m.w[w] == 0u;
-    Dec<32> q;
-    if (narrow) q = dresize<32>(ddiv<10, 8, 2, 4, FAITH>(dresize<8>(a), b, &err));
-    else q = ddiv<32, 32, 2, 4, FAITH>(a, b, &err);
-    u8* o = out + i * 133;
-    o[0] = s[0];
-    o[1] = (u8)q.neg;
-    o[2] = is_zero(a.m) ? (u8)(frac + 4u + 1u) : (u8)prec;
-    o[3] = (u8)(frac + 4u);
-    o[4] = (u8)err;
-#pragma unroll
-    for (int w = 0; w < 32; w++) {
-        o[5 + 4 * w] = (u8)q.m.w[w];
-        o[6 + 4 * w] = (u8)(q.m.w[w] >> 8);
-        o[7 + 4 * w] = (u8)(q.m.w[w] >> 16);
-        o[8 + 4 * w] = (u8)(q.m.w[w] >> 24);
-    }
+    div_image_int64<FAITH>(sums + i * stride, counts[i], out + i * out_stride);
 }
 
 // ------------------------------------------------------------------ objects
@@ -729,6 +696,9 @@ int updec_agg_compile(const updec_agg_desc* d, updec_agg_t* out)
     a->spec.filter_val = d->filter_value;
     a->nkeys = d->nkeys;
     a->spec.nkeys = d->nkeys;
+    if (d->nexprs < 32 && (d->avg_mask >> d->nexprs)) return fail(UPDEC_EINVAL, "avg_mask names an expression past nexprs");
+    a->spec.avg_mask = d->avg_mask;
+    a->spec.avg_faithful = d->mode != UPDEC_MODE_EXTENDED;
     a->onepass = agg_onepass_ok(a->spec, GenOptions());
     *out = a.release();
     return UPDEC_OK;
@@ -786,6 +756,34 @@ static void fill_args(updec_agg_t a, AggArgs* args, const uint8_t* const* d_cols
 }
 
 // rps: rows a lane takes per loop step (1, or 4 for row quads)
+// AVG images of the two-pass forms: one k_div_int64 launch per AVG
+// expression over the groups' SUM images and counts
+static int launch_avgs(updec_agg_t a, uint8_t* d_images, const int64_t* d_counts, int ngroups, int max_groups,
+                       hipStream_t st)
+{
+    const uint32_t mask = a->spec.avg_mask;
+    if (!mask || ngroups <= 0) return UPDEC_OK;
+    const int NP = (int)a->spec.plans.size();
+    int navg = 0;
+    for (int k = 0; k < NP; k++) navg += (mask >> k) & 1u;
+    uint8_t* avgs = d_images + (size_t)max_groups * NP * 133;
+    const unsigned blocks = (unsigned)((ngroups + 63) / 64);
+    for (int k = 0, j = 0; k < NP; k++) {
+        if (!((mask >> k) & 1u)) continue;
+        const u8* sums = d_images + (size_t)k * 133;
+        u8* out = avgs + (size_t)j * 133;
+        if (a->spec.avg_faithful)
+            hipLaunchKernelGGL(k_div_int64<1>, dim3(blocks), dim3(64), 0, st, sums, (int64_t)NP * 133, d_counts, out,
+                               (int64_t)ngroups, (int64_t)navg * 133);
+        else
+            hipLaunchKernelGGL(k_div_int64<0>, dim3(blocks), dim3(64), 0, st, sums, (int64_t)NP * 133, d_counts, out,
+                               (int64_t)ngroups, (int64_t)navg * 133);
+        HIPCHK(hipGetLastError());
+        j++;
+    }
+    return UPDEC_OK;
+}
+
 static int64_t agg_grid(hipFunction_t f, int64_t rows, int T, int rps)
 {
     const int64_t steps = (rows + rps - 1) / rps;
@@ -857,7 +855,8 @@ static int agg_run_onepass(updec_agg_t a, const uint8_t* const* d_cols, const in
     uint64_t* cnt = reinterpret_cast<uint64_t*>(d_counts);
     const u32* used = a->d_meta + 1;
     u32* meta = a->d_meta + 2;
-    void* pf[] = {&rec, &nb, &used, &maxg, &img, &cnt, &meta};
+    uint8_t* avgs = a->spec.avg_mask ? d_images + (size_t)max_groups * a->spec.plans.size() * 133 : nullptr;
+    void* pf[] = {&rec, &nb, &used, &maxg, &img, &cnt, &meta, &avgs};
     const int NP = (int)a->spec.plans.size();
     HIPCHK(hipModuleLaunchKernel(ffin, (unsigned)(max_groups * (NP + 1)), 1, 1, 256, 1, 1, 0, st, pf, nullptr));
     HIPCHK(hipMemcpyAsync(a->h_meta, a->d_meta, (4 + (size_t)max_groups) * sizeof(u32), hipMemcpyDeviceToHost, st));
@@ -964,7 +963,7 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
     uint64_t* cnt = reinterpret_cast<uint64_t*>(d_counts);
     void* pf[] = {&cp, &nb, &img, &cnt};
     HIPCHK(hipModuleLaunchKernel(ffin, (unsigned)(NP * ngroups + ngroups), 1, 1, 256, 1, 1, 0, st, pf, nullptr));
-    return UPDEC_OK;
+    return launch_avgs(a, d_images, d_counts, ngroups, max_groups, st);
 }
 
 int updec_agg_run_blocks(updec_agg_t a, const updec_column_blocks* cols, const int32_t* d_indices, int64_t rows,
@@ -1011,7 +1010,7 @@ int updec_agg_run_blocks(updec_agg_t a, const updec_column_blocks* cols, const i
     uint64_t* cnt = reinterpret_cast<uint64_t*>(d_counts);
     void* pf[] = {&cp, &nb, &img, &cnt};
     HIPCHK(hipModuleLaunchKernel(ffin, (unsigned)(NP + 1), 1, 1, 256, 1, 1, 0, st, pf, nullptr));
-    return UPDEC_OK;
+    return launch_avgs(a, d_images, d_counts, 1, 1, st);
 }
 
 int updec_agg_last_kernel_ms(updec_agg_t a, float* ms)
@@ -1045,9 +1044,11 @@ int updec_div_int64(const uint8_t* d_sums, int64_t stride_bytes, const int64_t* 
     if (n == 0) return UPDEC_OK;
     unsigned blocks = (unsigned)((n + 63) / 64);
     if (mode == UPDEC_MODE_EXTENDED)
-        hipLaunchKernelGGL(k_div_int64<0>, dim3(blocks), dim3(64), 0, S(stream), d_sums, stride_bytes, d_counts, d_out, n);
+        hipLaunchKernelGGL(k_div_int64<0>, dim3(blocks), dim3(64), 0, S(stream), d_sums, stride_bytes, d_counts, d_out, n,
+                           (int64_t)133);
     else
-        hipLaunchKernelGGL(k_div_int64<1>, dim3(blocks), dim3(64), 0, S(stream), d_sums, stride_bytes, d_counts, d_out, n);
+        hipLaunchKernelGGL(k_div_int64<1>, dim3(blocks), dim3(64), 0, S(stream), d_sums, stride_bytes, d_counts, d_out, n,
+                           (int64_t)133);
     HIPCHK(hipGetLastError());
     return UPDEC_OK;
 }
