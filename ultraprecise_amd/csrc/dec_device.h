@@ -1037,26 +1037,35 @@ template <int AW> UPD_DEV void wave_reduce(Mag<AW>& acc)
 // Convert a two's complement sum into the 133-byte nullable_type<Decimal>
 // image {flag, sign, prec, frac, error, v[32]} (aries_types.hxx:298-310,
 // decimal.hxx:130-141).  Zero is +0 (a +0 seed never yields -0).
-template <int AW> UPD_DEV void write_sum_image(u8* out, const Mag<AW>& acc, u32 prec, u32 frac, u32 error)
+// sign-magnitude value of a two's complement accumulator
+template <int AW> UPD_DEV Dec<AW> sum_value(const Mag<AW>& acc)
 {
+    Dec<AW> d;
     const bool neg = (acc.w[AW - 1] >> 31) != 0;
-    Mag<AW> m;
     const u32 fm = neg ? 0xffffffffu : 0u;
     u64 c = neg ? 1u : 0u;
 #pragma unroll
     for (int i = 0; i < AW; i++) {
         u64 s = (u64)(acc.w[i] ^ fm) + c;
-        m.w[i] = (u32)s;
+        d.m.w[i] = (u32)s;
         c = s >> 32;
     }
+    d.neg = neg ? 1u : 0u;
+    return d;
+}
+
+// 133-byte nullable_type<Decimal> image of a sign-magnitude value of AW
+// limbs (limbs past AW are zero)
+template <int AW> UPD_DEV void write_dec_image(u8* out, const Dec<AW>& d, u32 prec, u32 frac, u32 error)
+{
     out[0] = 1;
-    out[1] = neg ? 1 : 0;
+    out[1] = (u8)d.neg;
     out[2] = (u8)prec;
     out[3] = (u8)frac;
     out[4] = (u8)error;
 #pragma unroll
     for (int i = 0; i < 32; i++) {
-        u32 x = (i < AW) ? m.w[i] : 0u;
+        u32 x = (i < AW) ? d.m.w[i] : 0u;
         out[5 + 4 * i + 0] = (u8)x;
         out[5 + 4 * i + 1] = (u8)(x >> 8);
         out[5 + 4 * i + 2] = (u8)(x >> 16);
@@ -1064,32 +1073,36 @@ template <int AW> UPD_DEV void write_sum_image(u8* out, const Mag<AW>& acc, u32 
     }
 }
 
-// AVG = SUM / COUNT of one group (kernel_util.hxx:427-438, do_div):
-// Decimal(sum image) /= Decimal((int64)count) by the division rule, scale
-// S + 4; s and o are 133-byte nullable_type<Decimal> images.  Sums that fit
-// 8 limbs divide at that width (the quotient does not depend on the
-// operand widths).
-template <int FAITH> UPD_DEV void div_image_int64(const u8* s, i64 cnt, u8* o)
+template <int AW> UPD_DEV void write_sum_image(u8* out, const Mag<AW>& acc, u32 prec, u32 frac, u32 error)
 {
-    Dec<32> a;
-    a.neg = s[1];
-    const u32 prec = s[2], frac = s[3], error = s[4];
-#pragma unroll
-    for (int w = 0; w < 32; w++)
-        a.m.w[w] = (u32)s[5 + 4 * w] | ((u32)s[6 + 4 * w] << 8) | ((u32)s[7 + 4 * w] << 16) | ((u32)s[8 + 4 * w] << 24);
+    write_dec_image<AW>(out, sum_value<AW>(acc), prec, frac, error);
+}
+
+// AVG = SUM / COUNT of one group (kernel_util.hxx:427-438, do_div):
+// Decimal(sum) /= Decimal((int64)count) by the division rule, scale S + 4,
+// written as a 133-byte nullable_type<Decimal> image with the given flag
+// byte.  a is the sum's sign-magnitude value (W limbs of the 32).  Sums that
+// fit 8 limbs divide at that width (the quotient does not depend on the
+// operand widths).
+template <int FAITH, int W>
+UPD_DEV void div_dec_int64(const Dec<W>& a, u32 prec, u32 frac, u32 error, u8 flag, i64 cnt, u8* o)
+{
     Dec<2> b;
     const u64 uc = cnt < 0 ? (u64)(-(cnt + 1)) + 1u : (u64)cnt;
     b.neg = cnt < 0 ? 1u : 0u;
     b.m.w[0] = (u32)uc;
     b.m.w[1] = (u32)(uc >> 32);
     u32 err = error;
-    bool narrow = true;
-#pragma unroll
-    for (int w = 8; w < 32; w++) narrow = narrow && a.m.w[w] == 0u;
     Dec<32> q;
-    if (narrow) q = dresize<32>(ddiv<10, 8, 2, 4, FAITH>(dresize<8>(a), b, &err));
-    else q = ddiv<32, 32, 2, 4, FAITH>(a, b, &err);
-    const u8 flag = s[0];
+    if constexpr (W <= 8) {
+        q = dresize<32>(ddiv<10, 8, 2, 4, FAITH>(dresize<8>(a), b, &err));
+    } else {
+        bool narrow = true;
+#pragma unroll
+        for (int w = 8; w < W; w++) narrow = narrow && a.m.w[w] == 0u;
+        if (narrow) q = dresize<32>(ddiv<10, 8, 2, 4, FAITH>(dresize<8>(a), b, &err));
+        else q = ddiv<32, 32, 2, 4, FAITH>(dresize<32>(a), b, &err);
+    }
     o[0] = flag;
     o[1] = (u8)q.neg;
     o[2] = is_zero(a.m) ? (u8)(frac + 4u + 1u) : (u8)prec;
@@ -1102,6 +1115,17 @@ template <int FAITH> UPD_DEV void div_image_int64(const u8* s, i64 cnt, u8* o)
         o[7 + 4 * w] = (u8)(q.m.w[w] >> 16);
         o[8 + 4 * w] = (u8)(q.m.w[w] >> 24);
     }
+}
+
+// the same from a 133-byte SUM image s
+template <int FAITH> UPD_DEV void div_image_int64(const u8* s, i64 cnt, u8* o)
+{
+    Dec<32> a;
+    a.neg = s[1];
+#pragma unroll
+    for (int w = 0; w < 32; w++)
+        a.m.w[w] = (u32)s[5 + 4 * w] | ((u32)s[6 + 4 * w] << 8) | ((u32)s[7 + 4 * w] << 16) | ((u32)s[8 + 4 * w] << 24);
+    div_dec_int64<FAITH, 32>(a, s[2], s[3], s[4], s[0], cnt, o);
 }
 
 } // namespace updec

@@ -1673,21 +1673,32 @@ static void emit_agg_row(Emit& E, const AggSpec& s, const std::vector<ColumnType
     }
 }
 
-static void emit_agg_final_image(Emit& E, const AggSpec& s, const std::string& target)
+static void emit_agg_final_prec(Emit& E, const AggSpec& s)
 {
     E.os << "      u32 prec = 0, frac = 0;\n";
     for (int i = 0; i < (int)s.plans.size(); i++)
         E.os << "      if (pp == " << i << ") { prec = " << (s.plans[i].outP & 0xff) << "u; frac = " << (s.plans[i].outS & 0xff)
              << "u; }\n";
+}
+
+static void emit_agg_final_image(Emit& E, const AggSpec& s, const std::string& target)
+{
+    emit_agg_final_prec(E, s);
     E.os << "      write_sum_image<AW2>(" << target << ", t, prec, frac, 0u);\n";
 }
 
-static const char* kBlockReduceAW2 =
-    "    wave_reduce<AW2>(sm);\n"
-    "    if ((tid & 63u) == 0) {\n#pragma unroll\n      for (int w = 0; w < AW2; w++) red[(tid >> 6) * AW2 + w] = sm.w[w];\n    }\n"
-    "    __syncthreads();\n"
-    "    if (tid == 0) {\n      Mag<AW2> t = mzero<AW2>();\n      for (int wv = 0; wv < 4; wv++) {\n        Mag<AW2> o;\n"
-    "#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = red[wv * AW2 + w];\n        add_to<AW2>(t, t, o);\n      }\n";
+// block sum of the AW2-word accumulators sm of `waves` waves into t (in
+// thread 0, which the text leaves inside an open `if (tid == 0) {`)
+static std::string block_reduce_aw2(int waves)
+{
+    std::ostringstream os;
+    os << "    wave_reduce<AW2>(sm);\n"
+       << "    if ((tid & 63u) == 0) {\n#pragma unroll\n      for (int w = 0; w < AW2; w++) red[(tid >> 6) * AW2 + w] = sm.w[w];\n    }\n"
+       << "    __syncthreads();\n"
+       << "    if (tid == 0) {\n      Mag<AW2> t = mzero<AW2>();\n      for (int wv = 0; wv < " << waves << "; wv++) {\n        Mag<AW2> o;\n"
+       << "#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = red[wv * AW2 + w];\n        add_to<AW2>(t, t, o);\n      }\n";
+    return os.str();
+}
 
 static const char* kBlockReduceCount =
     "#pragma unroll\n    for (int off = 32; off >= 1; off >>= 1) c += (u64)__shfl_xor((long long)c, off, 64);\n"
@@ -1780,6 +1791,9 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     E.os << "  const u32 ns = nslot < MAXS ? nslot : MAXS;\n";
     // overflow[1]: the most slots any block used (the final kernel scans only those)
     E.os << "  if (tid == 0 && ns) atomicMax(overflow + 1, ns);\n";
+    // overflow[2..2050): bitmap of the keys seen by any block (bit k for
+    // key k), from which the final kernel orders the groups
+    E.os << "  if (tid < ns) { const u32 k = skey[tid] - 1u; atomicOr(overflow + 2 + (k >> 5), 1u << (k & 31u)); }\n";
     E.os << "  for (u32 k = tid; k < MAXS * (NP + 1); k += T) {\n";
     E.os << "    const u32 sl = k / (NP + 1), pp = k % (NP + 1);\n";
     E.os << "    u32* e = a.partials + (usize)sl * EW * nb + blockIdx.x;\n";
@@ -1791,85 +1805,98 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     E.os << "      if (sl < ns) for (u32 cc = 0; cc < NCOPY; cc++) cnt += acnt[cc * NS + sl];\n";
     E.os << "      e[(usize)(1 + NP * AW2) * nb] = (u32)cnt; e[(usize)(2 + NP * AW2) * nb] = (u32)(cnt >> 32);\n    }\n  }\n}\n";
 
-    // final: meta[0] = ngroups, meta[1] = 1 when ngroups > maxg, meta[2..] = keys
+    // final: stat[0] = 1 when a block's key table overflowed, stat[1] = the
+    // most slots any block used, stat[2..2050) = the key bitmap; out (host
+    // memory): meta[0] = ngroups, meta[1] = 1 when ngroups > maxg, meta[2] =
+    // stat[0], meta[4..] = keys in ascending order.  Block (g, p) sums
+    // expression p (p = NP: COUNT) over the records of group g.  Block 0
+    // also clears `next`, the state the next run accumulates into.
+    const int FT = kAggFinalThreads, FW = FT / 64, BW = 2048 / FT;
+    const bool avg = s.avg_mask != 0;
+    int navg = 0;
+    for (int k = 0; k < NP; k++) navg += (s.avg_mask >> k) & 1u;
     E.os << "UPD_DEV usize rec_at(u32 i, u32 nblocks) { const u32 sl = i / nblocks; return (usize)sl * EW * nblocks + (i - sl * nblocks); }\n";
-    E.os << "extern \"C\" __global__ void __launch_bounds__(256) " << name
-         << "_final(const u32* __restrict__ rec, u32 nblocks, const u32* __restrict__ used, u32 maxg, u8* __restrict__ images, u64* __restrict__ counts, u32* __restrict__ meta, u8* __restrict__ avgs)\n{\n";
+    E.os << "extern \"C\" __global__ void __launch_bounds__(" << FT << ") " << name
+         << "_final(const u32* __restrict__ rec, u32 nblocks, const u32* __restrict__ stat, u32* __restrict__ next, u32 maxg, u8* __restrict__ images, u64* __restrict__ counts, u32* __restrict__ meta, u8* __restrict__ avgs)\n{\n";
     E.os << "  const u32 tid = threadIdx.x;\n";
-    E.os << "  __shared__ u32 bm[2048];\n  __shared__ u32 pre[2049];\n  __shared__ u32 part[4];\n";
-    E.os << "  __shared__ u32 red[4 * AW2 + 8];\n  __shared__ u32 gkey;\n";
-    E.os << "  for (u32 i = tid; i < 2048; i += 256) bm[i] = 0u;\n  __syncthreads();\n";
-    // the scans load 16 record keys per thread before using any (the
-    // records sit in other XCDs' caches; one load at a time would leave
-    // each block waiting out every load's latency in turn).  Record i of
-    // the MAXS * nblocks is slot i / nblocks of block i % nblocks.
-    E.os << "  const u32 nrec = nblocks * used[0];\n";
-    E.os << "  for (u32 i0 = tid; i0 < nrec; i0 += 256u * 16u) {\n    u32 kk[16];\n";
-    E.os << "#pragma unroll\n    for (int j = 0; j < 16; j++) { const u32 i = i0 + 256u * j; kk[j] = i < nrec ? rec[rec_at(i, nblocks)] : 0u; }\n";
-    E.os << "#pragma unroll\n    for (int j = 0; j < 16; j++) {\n      const u32 k1 = kk[j];\n";
-    E.os << "      if (k1 && !(bm[(k1 - 1u) >> 5] & (1u << ((k1 - 1u) & 31u)))) atomicOr(&bm[(k1 - 1u) >> 5], 1u << ((k1 - 1u) & 31u));\n";
-    E.os << "    }\n  }\n";
-    E.os << "  __syncthreads();\n";
-    E.os << "  // prefix popcounts: thread t owns words [8t, 8t+8)\n";
-    E.os << "  u32 loc = 0;\n#pragma unroll\n  for (int j = 0; j < 8; j++) loc += __popc(bm[tid * 8 + j]);\n";
-    // exclusive prefix of loc over the block: scan in each wave, then the
-    // wave totals
+    E.os << "  __shared__ u32 bm[2048];\n  __shared__ u32 pre[2049];\n  __shared__ u32 part[" << FW << "];\n";
+    E.os << "  __shared__ u32 red[" << FW << " * AW2 + " << 2 * FW << "];\n  __shared__ u32 gkey;\n";
+    if (avg) E.os << "  __shared__ u64 cred[" << FW << "];\n";
+    // prefix popcounts of the bitmap: thread t owns words [BW t, BW t + BW);
+    // scan in each wave, then over the wave totals
+    E.os << "  u32 loc = 0;\n";
+    E.os << "#pragma unroll\n  for (int j = 0; j < " << BW << "; j++) { const u32 x = stat[2 + tid * " << BW << " + j]; bm[tid * " << BW
+         << " + j] = x; loc += __popc(x); }\n";
     E.os << "  u32 inc = loc;\n";
     E.os << "#pragma unroll\n  for (int off = 1; off < 64; off <<= 1) { const u32 y = __shfl_up(inc, off, 64); if ((tid & 63u) >= (u32)off) inc += y; }\n";
     E.os << "  if ((tid & 63u) == 63u) part[tid >> 6] = inc;\n  __syncthreads();\n";
     E.os << "  u32 wbase = 0;\n  for (u32 wv = 0; wv < (tid >> 6); wv++) wbase += part[wv];\n";
-    E.os << "  if (tid == 255u) pre[2048] = wbase + inc;\n";
-    E.os << "  { u32 acc = wbase + inc - loc;\n#pragma unroll\n    for (int j = 0; j < 8; j++) { pre[tid * 8 + j] = acc; acc += __popc(bm[tid * 8 + j]); } }\n";
+    E.os << "  if (tid == " << FT - 1 << "u) pre[2048] = wbase + inc;\n";
+    E.os << "  { u32 acc = wbase + inc - loc;\n#pragma unroll\n    for (int j = 0; j < " << BW << "; j++) { pre[tid * " << BW
+         << " + j] = acc; acc += __popc(bm[tid * " << BW << " + j]); } }\n";
     E.os << "  __syncthreads();\n";
     E.os << "  const u32 ngroups = pre[2048];\n";
     E.os << "  if (blockIdx.x == 0) {\n";
-    E.os << "    if (tid == 0) { meta[0] = ngroups; meta[1] = ngroups > maxg ? 1u : 0u; }\n";
-    E.os << "    for (u32 w = tid; w < 2048; w += 256) { u32 x = bm[w]; u32 r = pre[w];\n";
-    E.os << "      while (x) { const u32 b = __ffs(x) - 1u; x &= x - 1u; if (r < maxg) meta[2 + r] = w * 32u + b; r++; } }\n";
+    E.os << "    if (tid == 0) { meta[0] = ngroups; meta[1] = ngroups > maxg ? 1u : 0u; meta[2] = stat[0]; }\n";
+    E.os << "    for (u32 i = tid; i < " << kAggOnepassStatWords << "u; i += " << FT << ") next[i] = 0u;\n";
+    E.os << "    for (u32 w = tid; w < 2048; w += " << FT << ") { u32 x = bm[w]; u32 r = pre[w];\n";
+    E.os << "      while (x) { const u32 b = __ffs(x) - 1u; x &= x - 1u; if (r < maxg) meta[4 + r] = w * 32u + b; r++; } }\n";
     E.os << "  }\n";
     E.os << "  const u32 g = blockIdx.x / (NP + 1), pp = blockIdx.x % (NP + 1);\n";
     E.os << "  if (g >= ngroups || g >= maxg) return;\n";
-    E.os << "  for (u32 w = tid; w < 2048; w += 256) { u32 x = bm[w]; u32 r = pre[w];\n";
+    E.os << "  for (u32 w = tid; w < 2048; w += " << FT << ") { u32 x = bm[w]; u32 r = pre[w];\n";
     E.os << "    while (x) { const u32 b = __ffs(x) - 1u; x &= x - 1u; if (r == g) gkey = w * 32u + b + 1u; r++; } }\n";
     E.os << "  __syncthreads();\n  const u32 want = gkey;\n";
-    // a SUM block of an AVG expression also counts its group, then divides
-    const bool avg = s.avg_mask != 0;
-    int navg = 0;
-    for (int k = 0; k < NP; k++) navg += (s.avg_mask >> k) & 1u;
+    // Record i of the used slots * nblocks is slot i / nblocks of block
+    // i % nblocks.  Every record of a batch is loaded whole, matching or not,
+    // so that the batch's loads are all in flight at once (the records sit
+    // in other XCDs' caches); the key test selects after.
+    E.os << "  const u32 nrec = nblocks * stat[1];\n";
+    const int RW = 1 + AW2 + (avg ? 2 : 0);
+    const int D = std::max(2, std::min(16, 96 / RW));
     E.os << "  if (pp < NP) {\n    Mag<AW2> sm = mzero<AW2>();\n";
     if (avg) E.os << "    const bool avg = ((" << s.avg_mask << "u >> pp) & 1u) != 0u;\n    u64 c = 0;\n";
-    E.os << "    for (u32 i0 = tid; i0 < nrec; i0 += 256u * 16u) {\n      u32 kk[16];\n      usize at[16];\n";
-    E.os << "#pragma unroll\n      for (int j = 0; j < 16; j++) { const u32 i = i0 + 256u * j; at[j] = i < nrec ? rec_at(i, nblocks) : 0; kk[j] = i < nrec ? rec[at[j]] : 0u; }\n";
-    E.os << "#pragma unroll\n      for (int j = 0; j < 16; j++) {\n        if (kk[j] != want) continue;\n";
-    E.os << "        Mag<AW2> o;\n#pragma unroll\n        for (int w = 0; w < AW2; w++) o.w[w] = rec[at[j] + (usize)(1 + pp * AW2 + w) * nblocks];\n";
-    E.os << "        add_to<AW2>(sm, sm, o);\n";
+    E.os << "    for (u32 i0 = tid; i0 < nrec; i0 += " << FT << "u * " << D << "u) {\n      u32 kk[" << D << "];\n      Mag<AW2> o[" << D << "];\n";
+    if (avg) E.os << "      u64 oc[" << D << "];\n";
+    E.os << "#pragma unroll\n      for (int j = 0; j < " << D << "; j++) {\n        const u32 i = i0 + " << FT << "u * j;\n";
+    E.os << "        const usize at = i < nrec ? rec_at(i, nblocks) : 0;\n        kk[j] = i < nrec ? rec[at] : 0u;\n";
+    E.os << "#pragma unroll\n        for (int w = 0; w < AW2; w++) o[j].w[w] = i < nrec ? rec[at + (usize)(1 + pp * AW2 + w) * nblocks] : 0u;\n";
     if (avg)
-        E.os << "        if (avg) c += (u64)rec[at[j] + (usize)(1 + NP * AW2) * nblocks] | ((u64)rec[at[j] + (usize)(2 + NP * AW2) * nblocks] << 32);\n";
+        E.os << "        oc[j] = (avg && i < nrec) ? ((u64)rec[at + (usize)(1 + NP * AW2) * nblocks] | ((u64)rec[at + (usize)(2 + NP * AW2) * nblocks] << 32)) : 0ull;\n";
+    E.os << "      }\n";
+    E.os << "#pragma unroll\n      for (int j = 0; j < " << D << "; j++) {\n        if (kk[j] != want) continue;\n";
+    E.os << "        add_to<AW2>(sm, sm, o[j]);\n";
+    if (avg) E.os << "        c += oc[j];\n";
     E.os << "      }\n    }\n";
     if (avg) {
-        E.os << "    __shared__ u64 cred[4];\n";
         E.os << "#pragma unroll\n    for (int off = 32; off >= 1; off >>= 1) c += (u64)__shfl_xor((long long)c, off, 64);\n";
         E.os << "    if ((tid & 63u) == 0) cred[tid >> 6] = c;\n";
     }
-    E.os << kBlockReduceAW2;
-    emit_agg_final_image(E, s, "images + ((usize)g * NP + pp) * 133u");
+    E.os << block_reduce_aw2(FW);
+    emit_agg_final_prec(E, s);
+    E.os << "      const Dec<AW2> v = sum_value<AW2>(t);\n";
+    E.os << "      write_dec_image<AW2>(images + ((usize)g * NP + pp) * 133u, v, prec, frac, 0u);\n";
+    // a SUM block of an AVG expression also counted its group: it divides
     if (avg) {
         E.os << "      if (avg) {\n        u32 aj = 0;\n";
         for (int k = 0, j = 0; k < NP; k++)
             if ((s.avg_mask >> k) & 1u) E.os << "        if (pp == " << k << ") aj = " << j++ << "u;\n";
-        E.os << "        div_image_int64<" << (s.avg_faithful ? 1 : 0)
-             << ">(images + ((usize)g * NP + pp) * 133u, (i64)(cred[0] + cred[1] + cred[2] + cred[3]), avgs + ((usize)g * "
+        E.os << "        u64 n = 0;\n        for (int wv = 0; wv < " << FW << "; wv++) n += cred[wv];\n";
+        E.os << "        div_dec_int64<" << (s.avg_faithful ? 1 : 0) << ", AW2>(v, prec, frac, 0u, (u8)1, (i64)n, avgs + ((usize)g * "
              << navg << " + aj) * 133u);\n";
         E.os << "      }\n";
     }
     E.os << "    }\n  } else {\n    u64 c = 0;\n";
-    E.os << "    for (u32 i0 = tid; i0 < nrec; i0 += 256u * 16u) {\n      u32 kk[16];\n      usize at[16];\n";
-    E.os << "#pragma unroll\n      for (int j = 0; j < 16; j++) { const u32 i = i0 + 256u * j; at[j] = i < nrec ? rec_at(i, nblocks) : 0; kk[j] = i < nrec ? rec[at[j]] : 0u; }\n";
-    E.os << "#pragma unroll\n      for (int j = 0; j < 16; j++)\n";
-    E.os << "        if (kk[j] == want) c += (u64)rec[at[j] + (usize)(1 + NP * AW2) * nblocks] | ((u64)rec[at[j] + (usize)(2 + NP * AW2) * nblocks] << 32);\n";
+    E.os << "    for (u32 i0 = tid; i0 < nrec; i0 += " << FT << "u * 16u) {\n      u32 kk[16];\n      u64 oc[16];\n";
+    E.os << "#pragma unroll\n      for (int j = 0; j < 16; j++) {\n        const u32 i = i0 + " << FT << "u * j;\n";
+    E.os << "        const usize at = i < nrec ? rec_at(i, nblocks) : 0;\n        kk[j] = i < nrec ? rec[at] : 0u;\n";
+    E.os << "        oc[j] = i < nrec ? ((u64)rec[at + (usize)(1 + NP * AW2) * nblocks] | ((u64)rec[at + (usize)(2 + NP * AW2) * nblocks] << 32)) : 0ull;\n";
+    E.os << "      }\n";
+    E.os << "#pragma unroll\n      for (int j = 0; j < 16; j++)\n        if (kk[j] == want) c += oc[j];\n";
     E.os << "    }\n";
     E.os << kBlockReduceCount;
-    E.os << "    if (tid == 0) { u64 t = 0; for (int wv = 0; wv < 4; wv++) t += (u64)red[2 * wv] | ((u64)red[2 * wv + 1] << 32); counts[g] = t; }\n  }\n}\n";
+    E.os << "    if (tid == 0) { u64 t = 0; for (int wv = 0; wv < " << FW
+         << "; wv++) t += (u64)red[2 * wv] | ((u64)red[2 * wv + 1] << 32); counts[g] = t; }\n  }\n}\n";
     return E.os.str();
 }
 
@@ -1976,7 +2003,7 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
     E.os << "  __shared__ u32 red[4 * AW2 + 8];\n";
     E.os << "  if (k < NP * NG) {\n    const u32 pp = k % NP;\n    Mag<AW2> sm = mzero<AW2>();\n";
     E.os << "    for (u32 b = tid; b < nblocks; b += 256) {\n      Mag<AW2> o;\n#pragma unroll\n      for (int w = 0; w < AW2; w++) o.w[w] = partials[(usize)b * PW + k * AW2 + w];\n      add_to<AW2>(sm, sm, o);\n    }\n";
-    E.os << kBlockReduceAW2;
+    E.os << block_reduce_aw2(4);
     emit_agg_final_image(E, s, "images + (usize)k * 133u");
     E.os << "    }\n  } else {\n    const u32 gg = k - NP * NG;\n    u64 c = 0;\n";
     E.os << "    for (u32 b = tid; b < nblocks; b += 256) c += (u64)partials[(usize)b * PW + NP * NG * AW2 + 2 * gg] | ((u64)partials[(usize)b * PW + NP * NG * AW2 + 2 * gg + 1] << 32);\n";

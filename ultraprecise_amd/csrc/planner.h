@@ -142,6 +142,13 @@ struct GenOptions {
 constexpr int kQuadMaxBytes = 16;
 constexpr int kQuadMaxWords = 40;
 
+// one-pass aggregation: threads per block of the final kernel, and the
+// words of state the main kernel accumulates before it (overflow flag,
+// most slots used, 2048-word key bitmap, 2 words padding the cleared span
+// to a multiple of 16 bytes), which the host clears per run
+constexpr int kAggFinalThreads = 512;
+constexpr int kAggOnepassStatWords = 2 + 2048 + 2;
+
 // The fused aggregation launch keeps every lane at <= 2^kAggRowsPerLaneLog2
 // rows; the generated accumulator widths rely on it.
 constexpr int kAggRowsPerLaneLog2 = 14;

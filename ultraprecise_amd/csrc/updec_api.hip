@@ -3,7 +3,6 @@
 // statistics, group discovery, AVG division).
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
-#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <atomic>
@@ -400,8 +399,19 @@ struct updec_agg_s {
     ModuleP op_mod;
     std::string op_src;
     int op_block = 256;
-    u32* d_meta = nullptr;  // [0] overflow flag, [1] slots used, [2] ngroups, [3] too many, [4..] keys
-    u32* h_meta = nullptr;  // pinned host copy
+    // one-pass state, two regions of kAggOnepassStatWords used by turns:
+    // [0] a block's key table overflowed, [1] most slots a block used,
+    // [2..2050) bitmap of the keys seen.  A run's final kernel clears the
+    // other region for the next run; stat_dirty asks the host to clear both
+    // (first run, or a run that stopped between its two launches).
+    u32* d_meta = nullptr;
+    int stat_phase = 0;
+    bool stat_dirty = true;
+    // the final kernel's output, written by it into pinned host memory:
+    // [0] ngroups, [1] too many groups, [2] copy of the overflow flag,
+    // [4..] keys
+    u32* h_meta = nullptr;
+    u32* h_meta_dev = nullptr;  // its device address
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     ~updec_agg_s()
@@ -832,41 +842,51 @@ static int agg_run_onepass(updec_agg_t a, const uint8_t* const* d_cols, const in
         a->partial_words = need;
     }
     if (!a->d_meta) {
-        HIPCHK(hipMalloc((void**)&a->d_meta, (4 + 256) * sizeof(u32)));
-        HIPCHK(hipHostMalloc((void**)&a->h_meta, (4 + 256) * sizeof(u32)));
+        HIPCHK(hipMalloc((void**)&a->d_meta, 2 * kAggOnepassStatWords * sizeof(u32)));
+        a->stat_dirty = true;
     }
-    HIPCHK(hipMemsetAsync(a->d_meta, 0, 2 * sizeof(u32), st));
+    if (!a->h_meta) {
+        HIPCHK(hipHostMalloc((void**)&a->h_meta, (4 + 256) * sizeof(u32), hipHostMallocMapped | hipHostMallocCoherent));
+        HIPCHK(hipHostGetDevicePointer((void**)&a->h_meta_dev, a->h_meta, 0));
+    }
+    if (a->stat_dirty) HIPCHK(hipMemsetAsync(a->d_meta, 0, 2 * kAggOnepassStatWords * sizeof(u32), st));
+    a->stat_dirty = true;
+    u32* stat = a->d_meta + (size_t)a->stat_phase * kAggOnepassStatWords;
+    u32* next = a->d_meta + (size_t)(1 - a->stat_phase) * kAggOnepassStatWords;
     AggArgs args;
     fill_args(a, &args, d_cols, d_filter, d_key0, d_key1, rows);
-    u32* ovf = a->d_meta;
+    u32* ovf = stat;
     if (!a->ev0) {
         HIPCHK(hipEventCreate(&a->ev0));
         HIPCHK(hipEventCreate(&a->ev1));
     }
-    // the events are attached to the dispatch itself: they time the kernel,
-    // not the host's enqueue latency in front of it
+    // events recorded around the launch (events attached to the dispatch
+    // with hipExtModuleLaunchKernel cost about 16 us more host time per
+    // launch); the span also takes in the few microseconds between the first
+    // marker and the dispatch
     void* pm[] = {&args, &ovf};
-    HIPCHK(hipExtModuleLaunchKernel(fmain, (uint32_t)grid * (uint32_t)T, 1, 1, T, 1, 1, 0, st, pm, nullptr, a->ev0,
-                                    a->ev1, 0));
+    HIPCHK(hipEventRecord(a->ev0, st));
+    HIPCHK(hipModuleLaunchKernel(fmain, (uint32_t)grid, 1, 1, T, 1, 1, 0, st, pm, nullptr));
+    HIPCHK(hipEventRecord(a->ev1, st));
     a->timed = true;
     const u32* rec = a->d_partials;
     u32 nb = (u32)grid, maxg = (u32)max_groups;
     uint8_t* img = d_images;
     uint64_t* cnt = reinterpret_cast<uint64_t*>(d_counts);
-    const u32* used = a->d_meta + 1;
-    u32* meta = a->d_meta + 2;
+    u32* meta = a->h_meta_dev;
     uint8_t* avgs = a->spec.avg_mask ? d_images + (size_t)max_groups * a->spec.plans.size() * 133 : nullptr;
-    void* pf[] = {&rec, &nb, &used, &maxg, &img, &cnt, &meta, &avgs};
+    void* pf[] = {&rec, &nb, &stat, &next, &maxg, &img, &cnt, &meta, &avgs};
     const int NP = (int)a->spec.plans.size();
-    HIPCHK(hipModuleLaunchKernel(ffin, (unsigned)(max_groups * (NP + 1)), 1, 1, 256, 1, 1, 0, st, pf, nullptr));
-    HIPCHK(hipMemcpyAsync(a->h_meta, a->d_meta, (4 + (size_t)max_groups) * sizeof(u32), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipModuleLaunchKernel(ffin, (unsigned)(max_groups * (NP + 1)), 1, 1, kAggFinalThreads, 1, 1, 0, st, pf, nullptr));
+    a->stat_dirty = false;
+    a->stat_phase = 1 - a->stat_phase;
     HIPCHK(hipStreamSynchronize(st));
-    if (a->h_meta[0]) {
+    if (a->h_meta[2]) {
         *overflowed = true;
         return UPDEC_OK;
     }
-    if (a->h_meta[3]) return fail(UPDEC_EINVAL, "more groups than max_groups");
-    const int ng = (int)a->h_meta[2];
+    if (a->h_meta[1]) return fail(UPDEC_EINVAL, "more groups than max_groups");
+    const int ng = (int)a->h_meta[0];
     *ngroups_out = ng;
     if (keys_out)
         for (int g = 0; g < ng; g++) {
@@ -954,8 +974,9 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
         HIPCHK(hipEventCreate(&a->ev1));
     }
     void* pm[] = {&args};
-    HIPCHK(hipExtModuleLaunchKernel(fmain, (uint32_t)grid * (uint32_t)T, 1, 1, T, 1, 1, 0, st, pm, nullptr, a->ev0,
-                                    a->ev1, 0));
+    HIPCHK(hipEventRecord(a->ev0, st));
+    HIPCHK(hipModuleLaunchKernel(fmain, (uint32_t)grid, 1, 1, T, 1, 1, 0, st, pm, nullptr));
+    HIPCHK(hipEventRecord(a->ev1, st));
     a->timed = true;
     const u32* cp = a->d_partials;
     u32 nb = (u32)grid;
