@@ -33,7 +33,7 @@ def host_groups(hf1, hf2, hship, limit):
     return keys, gid
 
 
-@pytest.mark.parametrize("rows", [1000, 1_000_000])
+@pytest.mark.parametrize("rows", [1000, 262_144, 1_000_000])  # 262,144: whole chunks of 1,024 rows only
 def test_config3_grouped_bit_exact(U, O, gpu, rows):
     C = W.CONFIG3
     dev, host, (f1, f2, ship), (hf1, hf2, hship) = config3_columns(U, rows, C["seed"])

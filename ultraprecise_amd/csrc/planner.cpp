@@ -1729,19 +1729,22 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     E.os << "// accumulation: one pass, LDS key hash + 64-bit atomic slots\n";
     E.os << "#define MAXS " << MAXS << "\n#define HT 64\n#define EW " << EW << "\n";
     // key -> slot: open addressing over HT entries {key+1 << 8 | slot};
-    // slot 0xff marks an entry whose slot number is being assigned.  All
-    // table accesses are volatile so a waiting lane re-reads it.
-    E.os << "UPD_DEV u32 slot_of(u32 key, volatile u32* htab, u32* nslot, u32* skey) {\n"
+    // slot 0xff marks an entry whose slot number is being assigned.  Table
+    // reads and writes are relaxed workgroup-scope atomics, so a waiting lane
+    // re-reads the entry; unlike volatile accesses they do not make the wave
+    // wait for its outstanding global loads.
+    E.os << "UPD_DEV u32 ht_get(u32* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }\n";
+    E.os << "UPD_DEV u32 slot_of(u32 key, u32* htab, u32* nslot, u32* skey) {\n"
             "  u32 h = (key * 2654435761u) >> 26;\n"
             "  for (int probe = 0; probe < 4 * HT; probe++) {\n"
-            "    u32 e = htab[h];\n"
+            "    u32 e = ht_get(&htab[h]);\n"
             "    if (e == 0u) {\n"
-            "      const u32 old = atomicCAS((u32*)&htab[h], 0u, ((key + 1u) << 8) | 0xffu);\n"
+            "      const u32 old = atomicCAS(&htab[h], 0u, ((key + 1u) << 8) | 0xffu);\n"
             "      if (old == 0u) {\n"
             "        u32 sl = atomicAdd(nslot, 1u);\n"
             "        if (sl >= MAXS) sl = 0xfeu;\n"
             "        else skey[sl] = key + 1u;\n"
-            "        htab[h] = ((key + 1u) << 8) | sl;\n"
+            "        __hip_atomic_store(&htab[h], ((key + 1u) << 8) | sl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);\n"
             "        return sl;\n"
             "      }\n"
             "      e = old;\n"
@@ -1757,8 +1760,12 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     E.os << "extern \"C\" __global__ void __launch_bounds__(" << c.T << ") " << name
          << "_main(AggArgs a, u32* __restrict__ overflow)\n{\n";
     E.os << "  const u32 tid = threadIdx.x;\n  const u32 wave = tid >> 6, lane = tid & 63u;\n";
-    // the one-pass form accumulates with LDS atomics; staging through LDS
-    // as well measured slower there (0.52 vs 0.45 ms on the Q1 shape)
+    // the one-pass form accumulates with LDS atomics and reads its row
+    // quads straight from memory.  Staging whole chunks through LDS (16-byte
+    // coalesced loads one chunk ahead in registers) reads the Q1 columns
+    // alone in 0.29 ms against 0.33 ms for the quads, but with this
+    // kernel's work it ran 0.49-0.50 ms against 0.38 ms (two blocks per CU,
+    // two barriers per chunk); loading the next quad ahead changed nothing.
     const StagePlan sp;
     emit_stage_decl(E, sp, c.T / 64);
     E.os << "  __shared__ u32 htab[HT];\n  __shared__ u32 nslot;\n  __shared__ u32 skey[MAXS];\n";
