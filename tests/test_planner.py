@@ -1,5 +1,6 @@
 """CPU: the rewrite (AlignOptimize), result typing, classification, code
 generation and the C ABI surface -- no device needed."""
+import ctypes
 import os
 import re
 
@@ -98,6 +99,29 @@ def test_agg_kernels_compile(U):
     src = U.Agg(W.CONFIG1["exprs"], C15).source(1)
     assert "registers" in src
     U.check_compile(src)
+
+
+def test_fused_avg_kernels(U):
+    """AVG expressions: the one-pass final kernel divides by the faithful or
+    the extended rule as the mode says; masks past the expressions fail"""
+    from ultraprecise_amd import workloads as W
+    cols = [(c[0], c[1], c[2], b) for c, b in zip(W.CONFIG3["cols"], [13, 24, 4, 4])]
+    for mode, faith in ((U.MODE_AUTO, 1), (U.MODE_FAITHFUL, 1), (U.MODE_EXTENDED, 0)):
+        src = U.Agg(W.CONFIG3["exprs"], cols, filter=W.CONFIG3["filter"], nkeys=2, mode=mode, avg=(0, 1, 4)).source(0)
+        assert f"div_dec_int64<{faith}, AW2>" in src
+        assert "u8* __restrict__ avgs" in src
+        U.check_compile(src)
+    src = U.Agg(W.CONFIG3["exprs"], cols, filter=W.CONFIG3["filter"], nkeys=2).source(0)
+    assert "div_dec_int64" not in src
+    with pytest.raises(ValueError):
+        U.Agg(["a"], [("a", 15, 2)], avg=[1])
+    # the C entry point checks the mask itself
+    arr, keep = U._col_array([("a", 15, 2)])
+    ex = (ctypes.c_char_p * 1)(b"a")
+    d = U.AggDesc(arr, 1, ex, 1, 0, 0, 0, 0, U.MODE_AUTO, 2)
+    h = ctypes.c_void_p()
+    with pytest.raises(U.UpdecError, match="avg_mask"):
+        U._check(U.lib().updec_agg_compile(ctypes.byref(d), ctypes.byref(h)))
 
 
 def test_c_abi_exports_every_declared_symbol(U):
