@@ -145,7 +145,10 @@ int updec_expr_calc_blocks(updec_expr_t e, const updec_column_blocks* cols, cons
  * Output: images[g * nexprs + k] is the 133-byte nullable_type<Decimal>
  * image of SUM(expr_k) in group g (AVG images follow, see avg_mask);
  * counts[g] = COUNT(*) of group g; keys_out[g] = key0 | key1 << 8 of group
- * g (host memory). */
+ * g (host memory).  An aggregation object keeps device state between runs
+ * (the one-pass form's key tables are cleared by the previous run), so it
+ * serves one run at a time: callers that run one aggregation from several
+ * threads at once compile one object per thread. */
 typedef struct updec_agg_desc {
     const updec_column_desc* cols;
     int32_t ncols;
