@@ -122,6 +122,52 @@ def test_grouping_many_keys_and_fallback(U, O, gpu, nf1, nf2, nkeys):
         assert bytes(fused[g]) == O.avg_from_sum(bytes(got[g, 1]), int(gotc[g])), f"fused AVG group {g}"
 
 
+def test_repeated_runs_reuse_state(U, O, gpu):
+    """one aggregation object run again and again on inputs with different
+    key sets, a block-table overflow (two-pass fallback) and a too-small
+    max_groups in between: each run's one-pass state is cleared by the
+    previous run's final kernel, so every result must match the oracle"""
+    spec = W.CONFIG3["cols"][:2]
+    cols = [(c[0], c[1], c[2]) for c in spec]
+    exprs = ["qty", "price*qty"]
+    agg = U.Agg(exprs, cols, nkeys=1, avg=[0])
+    maxg = 64
+    images = U.DeviceBuffer(maxg * 3 * 133)
+    counts = U.DeviceBuffer(maxg * 8)
+    cases = [(100_000, bytes([5, 9])), (70_000, bytes([3])), (150_000, bytes(range(1, 41))),
+             (90_000, bytes([2, 7, 200])), (30_000, bytes(range(100, 110))), (110_000, bytes([9, 5, 11]))]
+    for i, (rows, letters) in enumerate(cases):
+        seed = 40 + i
+        dev = [U.gen_compact_device(W.gen_desc(U, seed, j, c), rows) for j, c in enumerate(spec)]
+        host = [t.numpy(count=rows * U.compact_bytes(c[1])) for t, c in zip(dev, spec)]
+        f1 = U.gen_byte_device(seed, 1, letters, rows)
+        hf1 = f1.numpy(count=rows).astype(np.int32)
+        images.zero()
+        counts.zero()
+        mg = 8 if i == 4 else maxg  # case 4: 10 groups, no table overflow
+        if i == 4:
+            with pytest.raises(U.UpdecError, match="max_groups"):
+                agg.run(dev, rows, images, counts, key0=f1, max_groups=mg)
+            continue
+        ng, keys = agg.run(dev, rows, images, counts, key0=f1, max_groups=mg)
+        hkeys = np.unique(hf1)
+        assert ng == len(hkeys) and keys == [int(k) for k in hkeys], i
+        lut = np.full(256, -1, dtype=np.int32)
+        lut[hkeys] = np.arange(len(hkeys), dtype=np.int32)
+        gid = lut[hf1].astype(np.int32)
+        allimg = images.numpy()
+        got = allimg[:mg * 2 * 133].reshape(mg, 2, 133)
+        fused = allimg[mg * 2 * 133:mg * 3 * 133].reshape(mg, 133)
+        gotc = counts.numpy("int64")[:ng]
+        for k, text in enumerate(exprs):
+            e = U.Expr(text, cols)
+            want, wc = O.plan_group_sum(O.Plan(e.plan), host, [c[1] for c in cols], [c[2] for c in cols], rows, gid, ng)
+            assert np.array_equal(gotc, wc), i
+            assert np.array_equal(got[:ng, k], want), (i, text)
+        for g in range(ng):
+            assert bytes(fused[g]) == O.avg_from_sum(bytes(got[g, 0]), int(gotc[g])), (i, g)
+
+
 @pytest.mark.parametrize("mode", ["faithful", "extended"])
 def test_div_int64_edges(U, O, gpu, mode):
     """AVG division on crafted sums: zero, negative, path (i) wrap region,
