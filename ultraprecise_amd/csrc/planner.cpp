@@ -1465,6 +1465,21 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
     const int OB = p.outB;
     E.os << "#include \"dec_device.h\"\nusing namespace updec;\n" << kArgsStruct;
     E.os << "// plan: " << p.text() << "\n";
+    if (OB == 0) {
+        // a result of precision 0 has no bytes (B(0) = 0, decimal.cu:92):
+        // nothing is evaluated or stored; the block form still reports
+        // which rows are valid
+        E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name << "("
+             << (o.blocks ? "IterArgs" : "ColArgs") << " a)\n{\n";
+        if (o.blocks) {
+            E.os << "  for (usize row = (usize)blockIdx.x * " << T << " + threadIdx.x; row < a.rows; row += (usize)gridDim.x * "
+                 << T << ") {\n";
+            E.os << "    const i64 pos_ = a.idx ? (i64)a.idx[row] : (i64)row;\n";
+            E.os << "    if (a.valid) a.valid[row] = (pos_ >= 0 && (usize)pos_ < a.rowlimit) ? 1u : 0u;\n  }\n";
+        }
+        E.os << "}\n";
+        return E.os.str();
+    }
     if (!calc_one_row(p)) emit_plan_functions(E, {p}, cols);
     // Rows whose image is 4, 8 or a multiple of 16 bytes go straight to HBM
     // as dword / dwordx2 / dwordx4 stores (coalesced: lane i writes bytes
