@@ -79,3 +79,40 @@ def test_random_expression_bit_exact(U, O, gpu, k):
     bad = np.nonzero((got != want).reshape(ROWS, e.out_bytes).any(axis=1))[0]
     assert bad.size == 0, f"{text} ({'faithful' if e.faithful else 'extended'}): {bad.size} rows differ, first {bad[:5]}"
     assert not full[ROWS * e.out_bytes:].any()
+
+
+@pytest.mark.parametrize("k", range(0, NCASES, 3))
+def test_random_expression_grouped_sum_avg(U, O, gpu, k):
+    """SUM and AVG of the same random expressions, grouped by a 1-byte key"""
+    specs, text = case(k)
+    cols = [(n, P, S) for n, (P, S) in zip("abc", specs)]
+    print(f"case {k}: {text} over {specs}", flush=True)
+    try:
+        e = U.Expr(text, cols)
+        agg = U.Agg([text], cols, nkeys=1, avg=[0])
+    except U.UpdecError as err:
+        assert any(w in str(err) for w in ("constant", "integer operand", "wide", "limb", "digits", "LDS")), str(err)
+        pytest.skip(f"rejected: {err}")
+    dev, host = columns(U, specs, 500 + k)
+    key = U.gen_byte_device(500 + k, 9, bytes([4, 8, 15, 16, 23]), ROWS)
+    hkey = key.numpy(count=ROWS).astype(np.int32)
+    maxg = 8
+    images = U.DeviceBuffer(maxg * 2 * 133).zero()
+    counts = U.DeviceBuffer(maxg * 8).zero()
+    ng, keys = agg.run(dev, ROWS, images, counts, key0=key, max_groups=maxg)
+    hkeys = np.unique(hkey)
+    assert ng == len(hkeys) and keys == [int(x) for x in hkeys]
+    lut = np.full(256, -1, dtype=np.int32)
+    lut[hkeys] = np.arange(ng, dtype=np.int32)
+    mode = O.FAITHFUL if e.faithful else O.EXTENDED
+    want, wc = O.plan_group_sum(O.Plan(e.plan), host, [P for P, S in specs], [S for P, S in specs], ROWS,
+                                lut[hkey].astype(np.int32), ng, mode=mode)
+    allimg = images.numpy()
+    got = allimg[:maxg * 133].reshape(maxg, 133)
+    avg = allimg[maxg * 133:].reshape(maxg, 133)
+    gotc = counts.numpy("int64")[:ng]
+    assert np.array_equal(gotc, wc)
+    for g in range(ng):
+        assert bytes(got[g]) == bytes(want[g]), f"SUM group {g}"
+        # the aggregation's mode is AUTO: its AVG divides by the faithful rule
+        assert bytes(avg[g]) == O.avg_from_sum(bytes(want[g]), int(wc[g])), f"AVG group {g}"
