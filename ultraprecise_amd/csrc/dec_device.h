@@ -263,6 +263,74 @@ UPD_DEV u32 div_2by1(u32 u1, u32 u0, u32 d, u32 v, u32& r)
 // Algorithm D, with the divisor normalized to fill all NB limbs (so the
 // quotient loop has a static trip count of NX+1 digits) and the digit
 // estimate from div_2by1 refined by the second divisor limb.
+// One quotient digit of Knuth's Algorithm D (TAOCP 4.3.1): digit J of the
+// quotient from the window un[J .. J+NB] and the normalized divisor vn.
+template <int J, int NB, int NU>
+UPD_DEV u32 divmod_digit(Mag<NU>& un, const Mag<NB>& vn, u32 d1, u32 d0, u32 v)
+{
+    const u32 u2 = un.w[J + NB], u1 = un.w[J + NB - 1];
+    u32 qhat;
+    u64 rhat;
+    if (u2 >= d1) {
+        qhat = 0xffffffffu;
+        rhat = (u64)u1 + d1;
+    } else {
+        u32 r32;
+        qhat = div_2by1(u2, u1, d1, v, r32);
+        rhat = r32;
+    }
+    if constexpr (NB >= 2) {
+        const u32 u0 = un.w[J + NB - 2];
+#pragma unroll
+        for (int it = 0; it < 2; it++) {
+            bool fix = (rhat < (1ull << 32)) && ((u64)qhat * d0 > ((rhat << 32) | u0));
+            qhat -= fix ? 1u : 0u;
+            rhat += fix ? d1 : 0u;
+        }
+    }
+    // un[J .. J+NB] -= qhat * vn.  A zero digit changes nothing: the top
+    // digits of a dividend shorter than its static width are all zero, and
+    // skipping them (a wave-uniform branch when every lane's digit is zero)
+    // removes most of the work of a wide division
+    if (qhat != 0u) {
+        u64 kk = 0;
+        i64 t;
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            u64 p = (u64)qhat * vn.w[i];
+            t = (i64)un.w[i + J] - (i64)kk - (i64)(p & 0xffffffffull);
+            un.w[i + J] = (u32)t;
+            kk = (p >> 32) - (u64)(t >> 32);
+        }
+        t = (i64)un.w[J + NB] - (i64)kk;
+        un.w[J + NB] = (u32)t;
+        // rare add-back when the estimate was one too large
+        if (t < 0) {
+            qhat -= 1u;
+            u64 c = 0;
+#pragma unroll
+            for (int i = 0; i < NB; i++) {
+                u64 s = (u64)un.w[i + J] + vn.w[i] + c;
+                un.w[i + J] = (u32)s;
+                c = s >> 32;
+            }
+            un.w[J + NB] += (u32)c;
+        }
+    }
+    return qhat;
+}
+
+// digits J, J-1, ..., 0, each its own instantiation: every limb index is a
+// compile-time constant whatever the unroller decides, so the window stays
+// in registers (a loop over J that the compiler leaves rolled indexes the
+// window at run time and moves it to scratch memory)
+template <int J, int NB, int NU, int NQ>
+UPD_DEV void divmod_digits(Mag<NU>& un, const Mag<NB>& vn, u32 d1, u32 d0, u32 v, Mag<NQ>& Q)
+{
+    Q.w[J] = divmod_digit<J, NB, NU>(un, vn, d1, d0, v);
+    if constexpr (J > 0) divmod_digits<J - 1, NB, NU, NQ>(un, vn, d1, d0, v, Q);
+}
+
 template <int NX, int NB> UPD_DEV void divmod(const Mag<NX>& X, const Mag<NB>& B, Mag<NX + 1>& Q, Mag<NB>& R)
 {
     constexpr int NU = NX + NB + 1;
@@ -272,59 +340,7 @@ template <int NX, int NB> UPD_DEV void divmod(const Mag<NX>& X, const Mag<NB>& B
     const u32 d1 = vn.w[NB - 1];
     const u32 d0 = (NB >= 2) ? vn.w[NB - 2] : 0u;
     const u32 v = recip_word(d1);
-#pragma unroll
-    for (int j = NX; j >= 0; j--) {
-        const u32 u2 = un.w[j + NB], u1 = un.w[j + NB - 1];
-        u32 qhat;
-        u64 rhat;
-        if (u2 >= d1) {
-            qhat = 0xffffffffu;
-            rhat = (u64)u1 + d1;
-        } else {
-            u32 r32;
-            qhat = div_2by1(u2, u1, d1, v, r32);
-            rhat = r32;
-        }
-        if constexpr (NB >= 2) {
-            const u32 u0 = un.w[j + NB - 2];
-#pragma unroll
-            for (int it = 0; it < 2; it++) {
-                bool fix = (rhat < (1ull << 32)) && ((u64)qhat * d0 > ((rhat << 32) | u0));
-                qhat -= fix ? 1u : 0u;
-                rhat += fix ? d1 : 0u;
-            }
-        }
-        // un[j .. j+NB] -= qhat * vn.  A zero digit changes nothing: the top
-        // digits of a dividend shorter than its static width are all zero,
-        // and skipping them (a wave-uniform branch when every lane's digit is
-        // zero) removes most of the work of a wide division
-        if (qhat != 0u) {
-            u64 kk = 0;
-            i64 t;
-#pragma unroll
-            for (int i = 0; i < NB; i++) {
-                u64 p = (u64)qhat * vn.w[i];
-                t = (i64)un.w[i + j] - (i64)kk - (i64)(p & 0xffffffffull);
-                un.w[i + j] = (u32)t;
-                kk = (p >> 32) - (u64)(t >> 32);
-            }
-            t = (i64)un.w[j + NB] - (i64)kk;
-            un.w[j + NB] = (u32)t;
-            // rare add-back when the estimate was one too large
-            if (t < 0) {
-                qhat -= 1u;
-                u64 c = 0;
-#pragma unroll
-                for (int i = 0; i < NB; i++) {
-                    u64 s = (u64)un.w[i + j] + vn.w[i] + c;
-                    un.w[i + j] = (u32)s;
-                    c = s >> 32;
-                }
-                un.w[j + NB] += (u32)c;
-            }
-        }
-        Q.w[j] = qhat;
-    }
+    divmod_digits<NX, NB, NU, NX + 1>(un, vn, d1, d0, v, Q);
     R = shr_var<NB>(resize<NB>(un), k);
 }
 
