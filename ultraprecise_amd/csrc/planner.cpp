@@ -1458,6 +1458,12 @@ static void emit_row_loop_close(Emit& E, int rps) { E.os << (rps == 4 ? "    }\n
 // no longer pass through scratch (P = 150 sweep: 2.34 -> 1.87 ms).
 bool calc_one_row(const Plan& p) { return plan_out_of_line(p) && plan_ops(p) <= kOutOfLineOps; }
 
+// The calc kernel assembles rows in an LDS tile of block x B(P_out) bytes;
+// for rows wider than 64 bytes a 256-row tile takes enough LDS to cap the
+// blocks per CU, so those kernels run 64-row blocks (P = 150 sweep: 1.86 ->
+// 1.63 ms per 10 M rows; P = 300: 16.7 -> 16.2 ms).
+int calc_block(const Plan& p) { return p.outB > 64 ? 64 : 256; }
+
 std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, const std::string& name, const GenOptions& o)
 {
     Emit E;

@@ -156,6 +156,8 @@ constexpr int kAggRowsPerLaneLog2 = 14;
 // true when the calc kernel of this plan takes one row per thread with no
 // row loop (the launch needs ceil(rows / block) blocks)
 bool calc_one_row(const Plan& p);
+// threads (and rows) per block of the calc kernel of a plan
+int calc_block(const Plan& p);
 
 // calc kernel: one compact output row per input row (the reference's
 // calc_expr_<id>, AriesExprCalcNode.cpp:1285-1318)

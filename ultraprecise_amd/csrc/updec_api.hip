@@ -599,6 +599,7 @@ int updec_expr_compile(const char* text, const updec_column_desc* cols, int ncol
     e->types = types;
     e->name = "calc_expr_" + std::to_string(g_kernel_id.fetch_add(1));
     GenOptions o;
+    o.block = calc_block(e->plan);
     e->src = gen_calc_kernel(e->plan, types, "calc_expr", o);
     *out = e.release();
     return UPDEC_OK;
@@ -648,11 +649,13 @@ int updec_expr_calc(updec_expr_t e, const uint8_t* const* d_cols, int64_t rows, 
     }
     a.rows = (usize)rows;
     a.out = d_out;
-    int64_t tiles = (rows + 255) / 256;
+    const int T = calc_block(e->plan);
+    int64_t tiles = (rows + T - 1) / T;
     if (tiles > INT32_MAX) return fail(UPDEC_EINVAL, "too many rows for one launch");
-    unsigned grid = calc_one_row(e->plan) ? (unsigned)tiles : (unsigned)std::min<int64_t>(tiles, (int64_t)device_cus() * 32);
+    unsigned grid = calc_one_row(e->plan) ? (unsigned)tiles
+                                          : (unsigned)std::min<int64_t>(tiles, (int64_t)device_cus() * 32 * (256 / T));
     void* params[] = {&a};
-    HIPCHK(hipModuleLaunchKernel(e->fn, grid, 1, 1, 256, 1, 1, 0, S(stream), params, nullptr));
+    HIPCHK(hipModuleLaunchKernel(e->fn, grid, 1, 1, T, 1, 1, 0, S(stream), params, nullptr));
     return UPDEC_OK;
 }
 
@@ -669,6 +672,7 @@ int updec_expr_calc_blocks(updec_expr_t e, const updec_column_blocks* cols, cons
     if (!e->bmod) {
         GenOptions o;
         o.blocks = true;
+        o.block = calc_block(e->plan);
         e->bmod = ModuleCache::get().compile(gen_calc_kernel(e->plan, e->types, "calc_blocks", o), &err);
         if (!e->bmod) return fail(UPDEC_ECOMPILE, err);
         e->bfn = e->bmod->fn("calc_blocks", &err);
@@ -676,11 +680,13 @@ int updec_expr_calc_blocks(updec_expr_t e, const updec_column_blocks* cols, cons
     }
     a.out = d_out;
     a.valid = d_valid;
-    int64_t tiles = (rows + 255) / 256;
+    const int T = calc_block(e->plan);
+    int64_t tiles = (rows + T - 1) / T;
     if (tiles > INT32_MAX) return fail(UPDEC_EINVAL, "too many rows for one launch");
-    unsigned grid = calc_one_row(e->plan) ? (unsigned)tiles : (unsigned)std::min<int64_t>(tiles, (int64_t)device_cus() * 32);
+    unsigned grid = calc_one_row(e->plan) ? (unsigned)tiles
+                                          : (unsigned)std::min<int64_t>(tiles, (int64_t)device_cus() * 32 * (256 / T));
     void* params[] = {&a};
-    HIPCHK(hipModuleLaunchKernel(e->bfn, grid, 1, 1, 256, 1, 1, 0, S(stream), params, nullptr));
+    HIPCHK(hipModuleLaunchKernel(e->bfn, grid, 1, 1, T, 1, 1, 0, S(stream), params, nullptr));
     return UPDEC_OK;
 }
 
