@@ -1464,6 +1464,26 @@ bool calc_one_row(const Plan& p) { return plan_out_of_line(p) && plan_ops(p) <= 
 // 1.63 ms per 10 M rows; P = 300: 16.7 -> 16.2 ms).
 int calc_block(const Plan& p) { return p.outB > 64 ? 64 : 256; }
 
+// Widest limb array a plan keeps live: a node's own limbs, and for a
+// division the window of its long division (quotient + divisor + 1 limbs).
+static int plan_peak_limbs(const Plan& p)
+{
+    int peak = 0;
+    for (const PNode& n : p.n) {
+        peak = std::max(peak, n.W);
+        if ((n.kind == PK::Div || n.kind == PK::Mod) && n.b >= 0) peak = std::max(peak, n.W + p.n[n.b].W + 1);
+    }
+    return peak;
+}
+
+// A calc kernel whose plan keeps 48 or more limbs live runs one wave per
+// SIMD with the whole register file (512 VGPRs): at the default occupancy
+// target the compiler keeps those limb arrays partly in scratch (P = 300
+// sweep: 388 B per lane, 16.3 ms per 10 M rows; one wave per SIMD: no
+// scratch, 8.6 ms).  Narrower plans keep the default (P = 150: 1.6 ms, one
+// wave per SIMD would be 2.5 ms).
+static bool calc_one_wave(const Plan& p) { return plan_peak_limbs(p) >= 48; }
+
 std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, const std::string& name, const GenOptions& o)
 {
     Emit E;
@@ -1492,8 +1512,9 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
     // [i*OB, (i+1)*OB) of the block's span); other widths are assembled in an
     // LDS tile and copied out in dwords.
     const int VB = OB % 16 == 0 ? 16 : (OB == 8 || OB == 4) ? OB : 0;
-    E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name << "(" << (o.blocks ? "IterArgs" : "ColArgs")
-         << " a)\n{\n";
+    E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") "
+         << (calc_one_wave(p) ? "__attribute__((amdgpu_waves_per_eu(1, 1))) " : "") << name << "("
+         << (o.blocks ? "IterArgs" : "ColArgs") << " a)\n{\n";
     E.os << "  __shared__ u32 tile[(" << T << " * " << OB << " + 3) / 4];\n";
     E.os << "  u8* t8 = reinterpret_cast<u8*>(tile);\n";
     if (VB) E.os << "  const bool direct = ((usize)a.out & " << VB - 1 << "u) == 0;\n";
