@@ -269,6 +269,10 @@ template <int J, int NB, int NU>
 UPD_DEV u32 divmod_digit(Mag<NU>& un, const Mag<NB>& vn, u32 d1, u32 d0, u32 v)
 {
     const u32 u2 = un.w[J + NB], u1 = un.w[J + NB - 1];
+    // window below the divisor's top limb: the digit is 0 and nothing
+    // changes (the top digits of a dividend narrower than its static width;
+    // a wave-uniform branch in practice)
+    if (u2 == 0u && u1 < d1) return 0u;
     u32 qhat;
     u64 rhat;
     if (u2 >= d1) {
