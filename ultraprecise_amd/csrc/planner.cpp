@@ -1750,9 +1750,11 @@ static const char* kBlockReduceCount =
 // One-pass grouped aggregation.  agg_main: each block keeps an LDS hash
 // table from the 16-bit key to a dense slot (assigned on first sight) and
 // 64-bit atomic accumulators per (lane group, slot, plan); at the end it
-// writes one record per slot {key+1, sums, count}.  agg_final: every block
-// rebuilds the ascending key order from all records (bitmap + prefix
-// popcounts), then block (g, p) sums the records of group g.
+// writes one record per slot {key+1, sums, count} and ORs its keys into a
+// global key bitmap.  agg_final: every block numbers the groups in
+// ascending key order from that bitmap (prefix popcounts), then block
+// (g, p) sums the records of group g (and divides an AVG expression's sum
+// by the group's count).
 static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnType>& cols, const std::string& name,
                                    const GenOptions& o, int* acc_words, int* block_out)
 {
