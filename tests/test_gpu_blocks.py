@@ -39,7 +39,9 @@ def gather(host, B, pos):
 
 CASES = [("p*(1-d)", [(15, 2), (15, 2)]),
          ("(a+b)*c - a/d", [(38, 10)] * 4),
-         ("a % b + c", [(30, 0), (20, 0), (18, 2)])]
+         ("a % b + c", [(30, 0), (20, 0), (18, 2)]),
+         # extended plan with 126-byte rows: the 64-row block form
+         ("(a+b)*c - a/d", [(150, 50)] * 4)]
 
 
 @pytest.mark.parametrize("text,specs", CASES)
