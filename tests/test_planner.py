@@ -150,3 +150,34 @@ def test_generator_is_counter_based(U):
         b = bytearray(x[i * 16:(i + 1) * 16])
         b[-1] &= 0x7F
         assert len(str(int.from_bytes(bytes(b), "little"))) == 38
+
+
+def test_descriptor_layouts_match_header(U, tmp_path):
+    """the ctypes mirrors of the C ABI's descriptor structs have the
+    header's size and field offsets (built with the host C compiler)"""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("no host C compiler")
+    mirrors = {"updec_column_desc": U.ColumnDesc, "updec_column_blocks": U.ColumnBlocks,
+               "updec_agg_desc": U.AggDesc, "updec_gen_desc": U.GenDesc}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "updec.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.check_call(["gcc", "-I", inc, str(src), "-o", str(exe)])
+    got = {}
+    for line in subprocess.check_output([str(exe)]).decode().splitlines():
+        cname, key, val = line.split()
+        got[(cname, key)] = int(val)
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
