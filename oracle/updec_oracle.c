@@ -422,8 +422,14 @@ static uint32_t hit_shift_bits(uint32_t shift)
  *   (i)  bitlen(A) + est(10^s) - 1 <= 64 and bitlen(B) <= 64: the numerator
  *        and the doubled remainder are uint64 and wrap mod 2^64;
  *   (ii) |B| < 2^32: the round-up test is (uint32)(2R) >= |B| (wraps for R >= 2^31);
- *   (iii) otherwise, binary-search division: when bitlen(X) == bitlen(B) - 1 the
- *        search range collapses to {1} and the result is 1 with no round-up. */
+ *   (iii) otherwise, binary-search division (DivInt): the lower bound of the
+ *        search is 2^max(0, bitlen(X) - bitlen(B) - 1), so when X is shorter
+ *        than B by 1 to 64 bits the search starts and ends at 1 and the
+ *        negative residual X - B blocks the round-up: the result is 1.  (The
+ *        upper bound is then a shift by a negative count, which PTX clamps to
+ *        0, or for a gap of 33 to 64 bits a store into qmax's own header
+ *        bytes; neither moves the lower bound.)  A gap above 64 bits stores
+ *        outside qmax: no defined result; the exact rule is kept. */
 void od_div(od_dec* a, const od_dec* b, int mode)
 {
     int nw = NW(mode);
@@ -462,7 +468,7 @@ void od_div(od_dec* a, const od_dec* b, int mode)
     int roundup;
     if (mode == OD_FAITHFUL && nb == 1) {
         roundup = (uint32_t)(R[0] * 2u) >= b->v[0];
-    } else if (mode == OD_FAITHFUL && bx == bb - 1) {
+    } else if (mode == OD_FAITHFUL && bx < bb && bb - bx <= 64) {
         memset(Q, 0, sizeof Q); Q[0] = 1; roundup = 0;
     } else {
         /* 2R >= B  <=>  R >= B - R */
@@ -506,9 +512,11 @@ static void from_nullable_image(od_dec* d, const uint8_t* in)
  *        floor((lo+hi)/2) (HalfIntDecimal :2093-2119), products and
  *        comparisons in the 32-limb arithmetic, then the magnitude of the
  *        signed residual A - q*B (CopyValue keeps no sign).  For
- *        la < lb - 1 the reference's upper bound is a shift by a negative
- *        count, or a store below its limb array: no defined result; here
- *        the remainder A.
+ *        2 <= lb - la <= 64 the search starts and ends at q = 1 (the upper
+ *        bound is a negative shift, 0 on PTX, or a store into qmax's own
+ *        header bytes), so the result is B - A.  For lb - la > 64 the
+ *        reference stores outside qmax: no defined result; here the
+ *        remainder A.
  *   x % 0: the reference scans below the divisor's limbs: no defined
  *   result; here A is kept, as x / 0 keeps it.
  * Extended mode: A % B exactly (x % 0 -> A), same sign and scale rules. */
@@ -539,6 +547,9 @@ void od_mod(od_dec* a, const od_dec* b, int mode)
             }
         }
         R[0] = rem;
+    } else if (mode == OD_FAITHFUL && la + 1 < lb && lb - la <= 64) {
+        /* DivInt with qmin = 1 and no bisection step: |A - B| = B - A */
+        sub_w(R, b->v, A.v, nw);
     } else if (mode == OD_FAITHFUL && la + 1 >= lb) {
         od_dec ds = A, dt = *b, qmin, qmax, qmid, t;
         ds.sign = dt.sign = 0; ds.frac = dt.frac = 0;

@@ -8,7 +8,8 @@ A value is (sign, magnitude, scale).  Rules (decimal.cu):
   mul (1325-1351): sign xor, scale sum, magnitude product.
   div (2201-2333): scale Sa + 4; 0/x -> +0; x/0 -> |x| kept (error flag);
       q = round-half-up(|a| * 10^(Sb+4) / |b|); faithful mode keeps the
-      uint64, uint32 and single-bit-short path behaviours.
+      uint64 and uint32 path behaviours, and the bisection's quotient 1
+      for a numerator 1 to 64 bits shorter than the divisor.
 Faithful mode reduces magnitudes mod 2^1024 after every operation.
 """
 MOD1024 = 1 << 1024
@@ -94,7 +95,7 @@ def div(a, b, faithful=True):
     q, r = divmod(X, mb)
     if faithful and mb < (1 << 32):
         up = ((2 * r) & 0xFFFFFFFF) >= mb
-    elif faithful and X.bit_length() == mb.bit_length() - 1:
+    elif faithful and 1 <= mb.bit_length() - X.bit_length() <= 64:
         q, up = 1, False
     else:
         up = 2 * r >= mb
@@ -173,8 +174,8 @@ def get_decimal_hex(v):
 def mod(a, b, faithful=True):
     """ModCalc (decimal.cu:2621-2716) in closed form: scales dropped, sign
     xor, 0 % x = +0, x % 0 keeps |x|; faithful: when the reference bisects
-    (not both <= 64 bits, |b| >= 2^32) and bitlen(|a|) is bitlen(|b|) or one
-    less, the result is ||a| - |b||"""
+    (not both <= 64 bits, |b| >= 2^32) and bitlen(|a|) is bitlen(|b|) down
+    to bitlen(|b|) - 64, the result is ||a| - |b||"""
     sa, A = a[0], a[1]
     sb, B = b[0], b[1]
     if A == 0:
@@ -183,6 +184,6 @@ def mod(a, b, faithful=True):
     if B == 0:
         return sign, A, 0
     la, lb = A.bit_length(), B.bit_length()
-    if faithful and not (la <= 64 and lb <= 64) and B >= 1 << 32 and lb - 1 <= la <= lb:
+    if faithful and not (la <= 64 and lb <= 64) and B >= 1 << 32 and lb - 64 <= la <= lb:
         return sign, abs(A - B), 0
     return sign, A % B, 0

@@ -91,6 +91,16 @@ QUIRK_CASES = [
     ((1 << 70) + 5, 0, 3_000_000_001, 0),
     (1 << 86, 0, (1 << 101) - (1 << 97), 0),
     (-((1 << 100) + 12345), 3, (1 << 70) + 999, 1),
+    # path (iii) with the numerator 13, 2, 42 and 64 bits shorter than the
+    # divisor: DivInt's search starts and ends at 1 (1.00 / 12345678901.0000000000
+    # is 0.000001 in the reference)
+    (100, 2, 123456789010000000000, 10),
+    (1 << 70, 0, (1 << 85) + 77, 0),
+    (-(1 << 70), 0, (1 << 125) + 3, 0),
+    (1 << 40, 0, -((1 << 117) + (1 << 90)), 0),
+    # 65 bits shorter: the reference stores outside its qmax (no defined
+    # result); the build keeps the exact rule
+    (1 << 40, 0, (1 << 118) + 5, 0),
 ]
 
 

@@ -16,6 +16,9 @@ HIP_FLAGS = [
 ]
 
 
+SOURCES = ("updec_api.hip", "vm.hip", "codec.hip", "planner.cpp")
+
+
 def embed_device_header():
     src = open(os.path.join(CSRC, "dec_device.h")).read()
     delim = "UPDEC_EMBED"
@@ -31,16 +34,22 @@ def embed_device_header():
 def build(verbose=False):
     embed_device_header()
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    objs = []
-    for src in ("updec_api.hip", "vm.hip", "codec.hip", "planner.cpp"):
+    objs, cmds = [], []
+    for src in SOURCES:
         obj = os.path.join(CSRC, src.rsplit(".", 1)[0] + ".o")
         cmd = [hipcc] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
             cmd = [hipcc, "-x", "hip"] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
+        cmds.append(cmd)
         objs.append(obj)
+    # the translation units are independent: compile them side by side
+    procs = [subprocess.Popen(c) for c in cmds]
+    codes = [p.wait() for p in procs]
+    for c, rc in zip(cmds, codes):
+        if rc:
+            raise subprocess.CalledProcessError(rc, c)
     tmp = LIB + ".tmp"
     cmd = [hipcc, "-shared", "-fPIC", "--offload-arch=gfx950"] + objs + ["-lhiprtc", "-o", tmp]
     if verbose:

@@ -532,7 +532,8 @@ template <int W> UPD_DEV Dec<W> ddiv_rt(const Dec<W>& a, const Dec<W>& b, u32 S,
 #pragma unroll
         for (int i = 1; i < W; i++) top0 = top0 && bs.w[i] == 0u;
         const bool path_ii = !path_i && top0;
-        const bool path_iii_one = !path_i && !path_ii && (bitlen(X) + 1u == blb);
+        const u32 blx = bitlen(X);
+        const bool path_iii_one = !path_i && !path_ii && blx < blb && blb - blx <= 64u;
         if (path_ii) rnd = ((u32)(R.w[0] * 2u) >= bs.w[0]) ? 1u : 0u;
         if (path_iii_one) {
             Q = mzero<W + 1>();
@@ -707,7 +708,12 @@ template <int WO, int WA, int WB> UPD_DEV Dec<WO> dmul(const Dec<WA>& a, const D
 //   (i)  bitlen(a) + est(10^S) - 1 <= 64 and bitlen(b) <= 64: uint64 numerator
 //        and doubled remainder, both wrapping mod 2^64;
 //   (ii) |b| < 2^32: the round-up test is (uint32)(2R) >= |b|;
-//   (iii) otherwise: when bitlen(X) == bitlen(b) - 1 the result is 1.
+//   (iii) otherwise (DivInt's bisection, decimal.cu:2125-2191): its lower
+//        bound is 2^max(0, bitlen(X) - bitlen(b) - 1), so when X is 1 to 64
+//        bits shorter than b the search starts and ends at 1 and the
+//        negative residual blocks the round-up: the result is 1.  A gap
+//        above 64 bits stores outside the reference's qmax (no defined
+//        result): the exact rule is kept.
 template <int WO, int WA, int WB, int S, int FAITH>
 UPD_DEV Dec<WO> ddiv(const Dec<WA>& a, const Dec<WB>& b, u32* err = nullptr)
 {
@@ -730,7 +736,8 @@ UPD_DEV Dec<WO> ddiv(const Dec<WA>& a, const Dec<WB>& b, u32* err = nullptr)
 #pragma unroll
         for (int i = 1; i < WB; i++) top0 = top0 && bs.w[i] == 0u;
         const bool path_ii = !path_i && top0;
-        const bool path_iii_one = !path_i && !path_ii && (bitlen(X) + 1u == blb);
+        const u32 blx = bitlen(X);
+        const bool path_iii_one = !path_i && !path_ii && blx < blb && blb - blx <= 64u;
         if (path_ii) rnd = ((u32)(R.w[0] * 2u) >= bs.w[0]) ? 1u : 0u;
         if (path_iii_one) {
             Q = mzero<WX + 1>();
@@ -783,10 +790,12 @@ UPD_DEV Dec<WO> ddiv(const Dec<WA>& a, const Dec<WB>& b, u32* err = nullptr)
 //   otherwise DivInt's bisection (decimal.cu:2125-2191), whose quotient
 //   search starts at [2^(la-lb-1), 2^(la-lb+1)]:
 //     la > lb: |a| mod |b|;
-//     la = lb or la = lb - 1: the search stops at quotient 1 and the
-//       remainder |a| - |b| is kept without its sign: ||a| - |b|| (FAITH);
-//     la <= lb - 2: the reference shifts by a negative count or stores
-//       below its limb array (no defined result); |a| mod |b| = |a| here.
+//     lb - 64 <= la <= lb: the search starts at quotient 1 and stops there
+//       (for la <= lb - 2 its upper bound is a negative shift, 0 on PTX, or
+//       a store into qmax's own header bytes), and the remainder |a| - |b|
+//       is kept without its sign: ||a| - |b|| (FAITH);
+//     la < lb - 64: the reference stores outside qmax (no defined result);
+//       |a| mod |b| = |a| here.
 // x % 0: the reference scans below the divisor's limbs (no defined result);
 // |a| is kept, as x / 0 keeps it.  Without FAITH: |a| mod |b| throughout.
 template <int WO, int WA, int WB, int FAITH> UPD_DEV Dec<WO> dmod(const Dec<WA>& a, const Dec<WB>& b)
@@ -812,7 +821,7 @@ template <int WO, int WA, int WB, int FAITH> UPD_DEV Dec<WO> dmod(const Dec<WA>&
 #pragma unroll
         for (int i = 1; i < WB; i++) small_b = small_b && b.m.w[i] == 0u;
         const bool bisect = !(la <= 64u && lb <= 64u) && !small_b;
-        if (bisect && la <= lb && la + 1u >= lb) {
+        if (bisect && la <= lb && la + 64u >= lb) {
             constexpr int WM = WA > WB ? WA : WB;
             const Mag<WM> x = resize<WM>(a.m), y = resize<WM>(b.m);
             Mag<WM> d;

@@ -413,18 +413,34 @@ void infer_frontend_types(Expr& e, Mode mode)
     switch (e.kind) {
     case NK::Column: return;
     case NK::Int: e.P = digits_of(e.ival); e.S = 0; return;
-    case NK::Dec: {
-        HostDec t = e.dval;
-        t.check_and_set_real_precision();
-        e.P = t.prec;
-        e.S = t.frac;
+    case NK::Dec:
+        // the literal's own type is StringToDecimal's (digit count, leading
+        // zeros included, and the digits after the point), as the engine
+        // bridge gives it (AriesExprBridge.cpp:381-399)
+        e.P = e.dval.prec;
+        e.S = e.dval.frac;
         return;
-    }
     case NK::Cast: infer_frontend_types(*e.ch[0], mode); return;
     case NK::Calc: break;
     }
     for (auto& c : e.ch) infer_frontend_types(*c, mode);
-    int Pa = e.ch[0]->P, Sa = e.ch[0]->S, Pb = e.ch[1]->P, Sb = e.ch[1]->S;
+    // the frontend types a CALC from GetDecimalValue of its operands
+    // (CommonBiaodashi.cpp:2990-3030): a decimal literal there has its
+    // CheckAndSetRealPrecision precision (and scale 0 when it is zero)
+    auto fe = [](const Expr& c, int* P, int* S) {
+        if (c.kind == NK::Dec) {
+            HostDec t = c.dval;
+            t.check_and_set_real_precision();
+            *P = t.prec;
+            *S = t.frac;
+        } else {
+            *P = c.P;
+            *S = c.S;
+        }
+    };
+    int Pa, Sa, Pb, Sb;
+    fe(*e.ch[0], &Pa, &Sa);
+    fe(*e.ch[1], &Pb, &Sb);
     int P = 0, S = 0;
     switch (e.op) {
     case Op::Add:
@@ -952,6 +968,13 @@ bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode,
             n.P = e.P;
             n.S = e.S;
         } else {
+            // RemoveZeroNode drops the integer operands of an integer-typed
+            // calculation below a decimal one, leaving a calculation node
+            // without operands: the reference has no defined result for it
+            if (e.ch.size() != 2) {
+                *err = "integer-typed calculation inside a decimal expression";
+                return -1;
+            }
             int a = rec(*e.ch[0]);
             if (a < 0) return -1;
             int b = rec(*e.ch[1]);
