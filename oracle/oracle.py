@@ -48,6 +48,7 @@ def _load():
         "od_to_compact": (None, [D, P, ctypes.c_int]),
         "od_from_string": (ctypes.c_int, [D, ctypes.c_char_p, ctypes.c_int]),
         "od_cast": (None, [D, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "od_truncate": (None, [D, ctypes.c_int, ctypes.c_int]),
         "od_from_int32_scale": (None, [D, ctypes.c_int32, ctypes.c_int32, ctypes.c_int]),
         "od_from_int64": (None, [D, I64]),
         "od_rescale_up": (None, [D, ctypes.c_int, ctypes.c_int]),

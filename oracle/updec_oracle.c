@@ -290,6 +290,38 @@ void od_cast(od_dec* d, int P, int S, int mode)
     d->prec = P; d->frac = S;
 }
 
+/* Decimal::truncate(p) (decimal.cu:541-570), with prec and frac the value's:
+ *   p > 0: to scale p, floor-dividing by 10^(frac - p) or multiplying by
+ *          10^(p - frac); prec moves by the same count; frac = p;
+ *   p <= 0, q = -p, intg = prec - frac: when intg > q the magnitude becomes
+ *          floor(M / 10^(frac + q)) * 10^q and prec = intg; otherwise it is
+ *          zeroed (the sign is kept) and prec = 1; frac = 0. */
+void od_truncate(od_dec* d, int p, int mode)
+{
+    if (p > 0) {
+        if (d->frac >= p) {
+            od_shr10(d, d->frac - p, mode);
+            d->prec -= d->frac - p;
+        } else {
+            od_shl10(d, p - d->frac, mode);
+            d->prec += p - d->frac;
+        }
+        d->frac = p;
+    } else {
+        int q = -p;
+        int intg = d->prec - d->frac;
+        if (intg > q) {
+            od_shr10(d, d->frac + q, mode);
+            od_shl10(d, q, mode);
+            d->prec = intg;
+        } else {
+            memset(d->v, 0, sizeof d->v);
+            d->prec = 1;
+        }
+        d->frac = 0;
+    }
+}
+
 /* Decimal(int32_t t, int32_t scale, ...) (decimal.cu:230-260): |t| * 10^scale, frac = scale */
 void od_from_int32_scale(od_dec* d, int32_t t, int32_t scale, int mode)
 {
@@ -685,10 +717,11 @@ int od_to_string(const od_dec* d, char* buf, int cap)
 
 /* ---------------- plan ---------------- */
 
-enum { N_COL, N_CONST, N_NEG, N_ADD, N_SUB, N_MUL, N_DIV, N_CAST, N_MOD };
+enum { N_COL, N_CONST, N_NEG, N_ADD, N_SUB, N_MUL, N_DIV, N_CAST, N_MOD, N_TRUNC };
 
 typedef struct pnode {
     int kind, P, S, col, a, b;
+    int tp;            /* N_TRUNC: the truncate argument p */
     od_dec k;
 } pnode;
 
@@ -758,6 +791,14 @@ static const char* parse_node(od_plan* p, const char* s, int* out)
         s = parse_node(p, s, &c);
         if (!s) return NULL;
         p->n[id].kind = N_CAST; p->n[id].a = c; p->n[id].P = P; p->n[id].S = S;
+    } else if (!strcmp(op, "TRUNC")) {
+        int c;
+        int P = (int)strtol(s, (char**)&s, 10);
+        int S = (int)strtol(s, (char**)&s, 10);
+        int tp = (int)strtol(s, (char**)&s, 10);
+        s = parse_node(p, s, &c);
+        if (!s) return NULL;
+        p->n[id].kind = N_TRUNC; p->n[id].a = c; p->n[id].P = P; p->n[id].S = S; p->n[id].tp = tp;
     } else {
         int kind = !strcmp(op, "ADD") ? N_ADD : !strcmp(op, "SUB") ? N_SUB : !strcmp(op, "MUL") ? N_MUL
                  : !strcmp(op, "DIV") ? N_DIV : !strcmp(op, "MOD") ? N_MOD : -1;
@@ -810,6 +851,13 @@ static void eval_node(const evctx* c, int id, int64_t row, od_dec* out)
     case N_CONST: *out = nd->k; return;
     case N_NEG: eval_node(c, nd->a, row, out); od_neg(out); return;
     case N_CAST: eval_node(c, nd->a, row, out); od_cast(out, nd->P, nd->S, c->mode); return;
+    case N_TRUNC:
+        /* the argument's prec and frac are its plan type */
+        eval_node(c, nd->a, row, out);
+        out->prec = c->p->n[nd->a].P;
+        out->frac = c->p->n[nd->a].S;
+        od_truncate(out, nd->tp, c->mode);
+        return;
     default: break;
     }
     od_dec rb;

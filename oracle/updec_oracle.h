@@ -40,6 +40,7 @@ void od_from_compact(od_dec* d, const uint8_t* buf, int P, int S);   /* decimal.
 void od_to_compact(const od_dec* d, uint8_t* buf, int len);          /* decimal.cu:337-341 */
 int  od_from_string(od_dec* d, const char* s, int mode);             /* decimal.cu:2980-3025 */
 void od_cast(od_dec* d, int P, int S, int mode);                     /* decimal.cu:471-533 */
+void od_truncate(od_dec* d, int p, int mode);                        /* decimal.cu:541-570 */
 void od_from_int32_scale(od_dec* d, int32_t t, int32_t scale, int mode); /* decimal.cu:230-260 */
 void od_from_int64(od_dec* d, int64_t t);                            /* decimal.cu:301-309 */
 void od_rescale_up(od_dec* d, int scale, int mode);                  /* decimal.cu:263-268 */
@@ -76,6 +77,8 @@ void od_clear_overflow(void);
  * plan := '(' OP ' ' P ' ' S ' ' plan ' ' plan ')'  OP in ADD SUB MUL DIV
  *       | '(' 'NEG' ' ' plan ')'
  *       | '(' 'CAST' ' ' P ' ' S ' ' plan ')'   od_cast to DECIMAL(P,S)
+ *       | '(' 'TRUNC' ' ' P ' ' S ' ' p ' ' plan ')'   od_truncate(p) with the
+ *                                              argument's plan type as prec/frac
  *       | 'c' IDX ':' P ':' S          compact column operand
  *       | 'k' ('+'|'-') HEX ':' S      constant: big-endian hex magnitude, scale
  */

@@ -187,3 +187,19 @@ def mod(a, b, faithful=True):
     if faithful and not (la <= 64 and lb <= 64) and B >= 1 << 32 and lb - 64 <= la <= lb:
         return sign, abs(A - B), 0
     return sign, A % B, 0
+
+
+def truncate(v, P, p):
+    """Decimal::truncate(p) (decimal.cu:541-570) in closed form on a value of
+    type (P, S): returns (sign, magnitude, scale, precision).  p > 0: to scale
+    p by truncation or padding; p <= 0: the integer part with its last -p
+    digits zeroed when the type has more than -p integer digits, else 0 (the
+    sign is kept)."""
+    sign, m, S = v
+    if p > 0:
+        m = m // 10 ** (S - p) if S >= p else m * 10 ** (p - S)
+        return sign, m, p, P - S + p
+    q = -p
+    if P - S > q:
+        return sign, m // 10 ** (S + q) * 10 ** q, 0, P - S
+    return sign, 0, 0, 1

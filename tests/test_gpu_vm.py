@@ -48,6 +48,8 @@ CASES = [
     ("(a+b)*c - a/d", [(n, 150, 50) for n in "abcd"]),
     ("a/b + cast(c, 40, 3) - cast(a*b, 60, 30)", [(n, 38, 10) for n in "abc"]),
     ("1.5 - a*2 + 0.25/b", [(n, 18, 4) for n in "ab"]),
+    ("truncate(a, 2) + truncate(b, -3) - truncate(a*b, 0)", [(n, 20, 6) for n in "ab"]),
+    ("truncate(a, -14) + truncate(-b, -20) * truncate(a, 9)", [(n, 20, 6) for n in "ab"]),
 ]
 
 

@@ -66,14 +66,38 @@ def test_reference_expression_all_2(O):
     assert Fraction(O.to_str(r)) == Fraction(round(w * 10 ** 6), 10 ** 6)
 
 
-@pytest.mark.parametrize("case", [t for t in KA["truncate"] if t["p"] > 0])
-def test_reference_cast_down_up(O, case):
+@pytest.mark.parametrize("case", KA["truncate"], ids=[str(t["line"]) for t in KA["truncate"]])
+def test_reference_truncate(O, case):
+    """Decimal(P, S, text).truncate(p) (test_decimal.cu:672-745), every p"""
+    import ctypes
     P, S, text = case["a"]
     d = O.dec_cast(text, P, S)
-    O.lib.od_cast(O.ctypes.byref(d) if hasattr(O, "ctypes") else d, P, case["p"], 0) if False else None
+    O.lib.od_truncate(ctypes.byref(d), case["p"], 0)
+    assert O.to_str(d) == case["expect"]
+
+
+@pytest.mark.parametrize("case", [t for t in KA["truncate"] if t["p"] > 0])
+def test_reference_cast_down_up(O, case):
+    """for p > 0 truncate and cast to scale p agree"""
     import ctypes
+    P, S, text = case["a"]
+    d = O.dec_cast(text, P, S)
     O.lib.od_cast(ctypes.byref(d), P, case["p"], 0)
     assert O.to_str(d) == case["expect"]
+
+
+def test_truncate_against_closed_form(O):
+    import ctypes
+    rng = random.Random(41)
+    for _ in range(3000):
+        P = rng.randrange(1, 80)
+        S = rng.randrange(0, P + 1)
+        v = (rng.getrandbits(1), rng.randrange(0, 10 ** P), S)
+        p = rng.randrange(-P - 3, 40)
+        d = mk(O, v)
+        d.prec = P
+        O.lib.od_truncate(ctypes.byref(d), p, 1)
+        assert (d.sign, d.magnitude, d.frac, d.prec) == pyref.truncate(v, P, p), (v, P, p)
 
 
 @pytest.mark.parametrize("c", KA["compact_round_trip"])

@@ -181,3 +181,16 @@ def test_descriptor_layouts_match_header(U, tmp_path):
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
 
+
+
+@pytest.mark.parametrize("text,why", [
+    ("a*1", "rewrites to a constant or integer"),       # RemoveZeroNode makes x*1 integer 0 in the reference
+    ("1*b", "rewrites to a constant or integer"),
+    ("a*0", "rewrites to a constant or integer"),
+    ("cast(7, 20, 3)", "integer operand"),               # an integer literal left as an operand
+    ("a + cast(7, 20, 3)", "integer operand"),
+    ("(1-2)%a", "integer-typed calculation"),            # RemoveZeroNode leaves the integer SUB without operands
+])
+def test_documented_rejections(U, text, why):
+    with pytest.raises(U.UpdecError, match=why):
+        U.Expr(text, [("a", 18, 4), ("b", 15, 2)])

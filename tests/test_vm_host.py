@@ -55,6 +55,10 @@ CASES = [
     ("a/b + cast(c, 40, 3) - cast(a*b, 60, 30)", [(n, 38, 10) for n in "abc"]),
     ("1.5 - a*2 + 0.25/b", [(n, 18, 4) for n in "ab"]),
     ("cast(a, 10, 0) - cast(b, 60, 45)", [(n, 38, 20) for n in "ab"]),
+    # truncate(p): to a scale, to the integer part with -p digits zeroed,
+    # and to zero when the type has no more than -p integer digits
+    ("truncate(a, 2) + truncate(b, -3) - truncate(a*b, 0)", [(n, 20, 6) for n in "ab"]),
+    ("truncate(a, -14) + truncate(-b, -20) * truncate(a, 9)", [(n, 20, 6) for n in "ab"]),
 ]
 
 

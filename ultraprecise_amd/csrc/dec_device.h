@@ -451,6 +451,16 @@ template <int WO, int WI, int N> UPD_DEV Dec<WO> dshr10(const Dec<WI>& a)
     return r;
 }
 
+// truncate(p) for p <= 0 when the argument's type has no more than -p
+// integer digits (decimal.cu:561-565): the magnitude is zeroed, the sign kept
+template <int WO, int WI> UPD_DEV Dec<WO> dtrunc_zero(const Dec<WI>& a)
+{
+    Dec<WO> r;
+    r.m = mzero<WO>();
+    r.neg = a.neg;
+    return r;
+}
+
 // ---- run-time scale variants (the precompiled interpreter, where the
 // shift counts are data).  Same results as the compile-time forms above;
 // every value is held at one register width W.

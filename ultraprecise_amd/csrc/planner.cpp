@@ -371,6 +371,28 @@ struct Parser {
                 e->ch = {arg};
                 return e;
             }
+            if (id == "truncate" && lx.peek() == '(') {
+                lx.i++;
+                ExprP arg = expr();
+                if (!arg) return nullptr;
+                bool ok = lx.eat(',');
+                bool neg = ok && lx.eat('-');
+                lx.ws();
+                size_t b = lx.i;
+                while (lx.i < lx.s.size() && isdigit((unsigned char)lx.s[lx.i])) lx.i++;
+                const size_t nd = lx.i - b;
+                ok = ok && nd > 0 && nd <= 6 && lx.eat(')');
+                if (!ok) {
+                    err = "truncate expects truncate(expr, p) with an integer p";
+                    return nullptr;
+                }
+                long p = std::stol(lx.s.substr(b, nd));
+                auto e = std::make_shared<Expr>();
+                e->kind = NK::Trunc;
+                e->ival = neg ? -p : p;
+                e->ch = {arg};
+                return e;
+            }
             for (size_t k = 0; k < names.size(); k++) {
                 if (names[k] == id) {
                     auto e = std::make_shared<Expr>();
@@ -401,6 +423,17 @@ ExprP parse_expr(const std::string& text, const std::vector<std::string>& names,
     return e;
 }
 
+void trunc_type(int P, int S, int p, int* outP, int* outS)
+{
+    if (p > 0) {
+        *outP = P - S + p;
+        *outS = p;
+    } else {
+        *outP = P - S > -p ? P - S : 1;
+        *outS = 0;
+    }
+}
+
 static int digits_of(int64_t v)
 {
     HostDec d = HostDec::from_int(v, 10);
@@ -421,6 +454,10 @@ void infer_frontend_types(Expr& e, Mode mode)
         e.S = e.dval.frac;
         return;
     case NK::Cast: infer_frontend_types(*e.ch[0], mode); return;
+    case NK::Trunc:
+        infer_frontend_types(*e.ch[0], mode);
+        trunc_type(e.ch[0]->P, e.ch[0]->S, (int)e.ival, &e.P, &e.S);
+        return;
     case NK::Calc: break;
     }
     for (auto& c : e.ch) infer_frontend_types(*c, mode);
@@ -504,9 +541,10 @@ BT* clone_bt(Arena& A, const Expr& e, Mode mode, bool* ok)
     BT* b = A.bt();
     b->kind = e.kind; b->op = e.op; b->col = e.col; b->ival = e.ival; b->dval = e.dval;
     b->P = e.P; b->S = e.S; b->reverse = e.reverse;
-    if (e.kind == NK::Cast) {
+    if (e.kind == NK::Cast || e.kind == NK::Trunc) {
         b->fn = align_optimize(std::make_shared<Expr>(e), mode);
         if (!b->fn) *ok = false;
+        else { b->P = b->fn->P; b->S = b->fn->S; }   // truncate's type follows its rewritten argument
         return b;
     }
     for (auto& c : e.ch) b->ch.push_back(clone_bt(A, *c, mode, ok));
@@ -608,7 +646,7 @@ bool is_decimal_calc(const MT* m)
     // only CALC nodes of decimal result type are rewritten; a tree with a
     // division or a decimal operand anywhere below is decimal-typed
     std::function<bool(const BT*)> has_dec = [&](const BT* b) -> bool {
-        if (b->kind == NK::Column || b->kind == NK::Dec || b->kind == NK::Cast) return true;
+        if (b->kind == NK::Column || b->kind == NK::Dec || b->kind == NK::Cast || b->kind == NK::Trunc) return true;
         if (b->kind == NK::Calc && b->op == Op::Div) return true;
         for (auto* c : b->ch)
             if (has_dec(c)) return true;
@@ -852,7 +890,7 @@ BT* multi_to_bin(Arena& A, MT* m, Mode mode)
 ExprP to_expr(const BT* b)
 {
     auto e = std::make_shared<Expr>();
-    if (b->kind == NK::Cast) {
+    if (b->kind == NK::Cast || b->kind == NK::Trunc) {
         *e = *b->fn;
         e->reverse = b->reverse;
         return e;
@@ -867,11 +905,12 @@ ExprP to_expr(const BT* b)
 
 ExprP align_optimize(const ExprP& e, Mode mode)
 {
-    if (e && e->kind == NK::Cast) {
+    if (e && (e->kind == NK::Cast || e->kind == NK::Trunc)) {
         ExprP arg = align_optimize(e->ch[0], mode);
         if (!arg) return nullptr;
         auto c = std::make_shared<Expr>(*e);
         c->ch = {arg};
+        if (e->kind == NK::Trunc) trunc_type(arg->P, arg->S, (int)e->ival, &c->P, &c->S);
         return c;
     }
     if (!e || e->kind != NK::Calc) return e;
@@ -919,6 +958,14 @@ static int op_bits(const PNode& n, const PNode& A, const PNode* B, int ba, int b
         if (n.S > A.S) return ba + pow10_bits(n.S - A.S);
         if (n.S < A.S) return std::max(1, ba - (pow10_bits(A.S - n.S) - 1));
         return ba;
+    case PK::Trunc:
+        if (n.tp > 0) {
+            if (n.S > A.S) return ba + pow10_bits(n.S - A.S);
+            if (n.S < A.S) return std::max(1, ba - (pow10_bits(A.S - n.S) - 1));
+            return ba;
+        }
+        if (A.P - A.S <= -n.tp) return 1;
+        return std::max(1, ba - (pow10_bits(A.S - n.tp) - 1)) + pow10_bits(-n.tp);
     default: return n.bits;
     }
 }
@@ -932,7 +979,8 @@ bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode,
     std::map<std::string, int> seen;
     auto intern = [&](const PNode& n) -> int {
         std::string key = std::to_string((int)n.kind) + "," + std::to_string(n.P) + "," + std::to_string(n.S) + "," +
-                          std::to_string(n.col) + "," + std::to_string(n.a) + "," + std::to_string(n.b);
+                          std::to_string(n.col) + "," + std::to_string(n.a) + "," + std::to_string(n.b) + "," +
+                          std::to_string(n.tp);
         if (n.kind == PK::Const) key += std::string(",") + (n.k.sign ? "-" : "+") + n.k.hex() + ":" + std::to_string(n.k.frac);
         auto it = seen.find(key);
         if (it != seen.end()) return it->second;
@@ -967,6 +1015,14 @@ bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode,
             n.a = a;
             n.P = e.P;
             n.S = e.S;
+        } else if (e.kind == NK::Trunc) {
+            int a = rec(*e.ch[0]);
+            if (a < 0) return -1;
+            n.kind = PK::Trunc;
+            n.a = a;
+            n.tp = (int)e.ival;
+            trunc_type(p.n[a].P, p.n[a].S, n.tp, &n.P, &n.S);
+            if (n.P != e.P || n.S != e.S) { *err = "internal: truncate type differs from its node"; return -1; }
         } else {
             // RemoveZeroNode drops the integer operands of an integer-typed
             // calculation below a decimal one, leaving a calculation node
@@ -1050,6 +1106,7 @@ std::string Plan::text() const
         case PK::Const: return std::string("k") + (x.k.sign ? "-" : "+") + x.k.hex() + ":" + std::to_string(x.S);
         case PK::Neg: return "(NEG " + rec(x.a) + ")";
         case PK::Cast: snprintf(buf, sizeof buf, "(CAST %d %d ", x.P, x.S); return buf + rec(x.a) + ")";
+        case PK::Trunc: snprintf(buf, sizeof buf, "(TRUNC %d %d %d ", x.P, x.S, x.tp); return buf + rec(x.a) + ")";
         default: break;
         }
         const char* op = x.kind == PK::Add   ? "ADD"
@@ -1076,6 +1133,7 @@ bool plan_is_faithful_envelope(const Plan& p, const std::vector<ColumnType>& col
         }
         // operator<< reads one row of the 38-row 10^n table (decimal.cu:2028-2058)
         if (x.kind == PK::Cast && x.S - p.n[x.a].S > 38) return false;
+        if (x.kind == PK::Trunc && (x.tp > 0 ? x.tp - p.n[x.a].S : -x.tp) > 38) return false;
         // the reference's MOD bisection multiplies quotient candidates by the
         // divisor modulo 2^1024: exact only while the dividend is below 2^1023
         if (x.kind == PK::Mod && p.n[x.a].bits > 1023) return false;
@@ -1215,6 +1273,24 @@ std::string emit_plan(Emit& E, const Plan& p, const std::vector<ColumnType>& col
             E.os << var[x.a] << ");\n";
             break;
         }
+        case PK::Trunc: {
+            const PNode& A = p.n[x.a];
+            E.os << ind << "const Dec<" << x.W << "> " << v << " = ";
+            if (x.tp > 0) {
+                if (x.S > A.S) E.os << "dscale<" << x.W << ", " << A.W << ", " << (x.S - A.S) << ">(";
+                else if (x.S < A.S) E.os << "dshr10<" << x.W << ", " << A.W << ", " << (A.S - x.S) << ">(";
+                else E.os << "dresize<" << x.W << ">(";
+                E.os << var[x.a] << ");\n";
+            } else if (A.P - A.S > -x.tp) {
+                // floor(M / 10^(S + q)) * 10^q, the sign kept
+                E.os << "dscale<" << x.W << ", " << x.W << ", " << -x.tp << ">(dshr10<" << x.W << ", " << A.W << ", "
+                     << (A.S - x.tp) << ">(" << var[x.a] << "));\n";
+            } else {
+                // no integer digits survive: zero magnitude, the sign kept
+                E.os << "dtrunc_zero<" << x.W << ", " << A.W << ">(" << var[x.a] << ");\n";
+            }
+            break;
+        }
         }
     }
     return var[p.root];
@@ -1242,6 +1318,7 @@ int plan_cost(const Plan& p)
         case PK::Add:
         case PK::Sub: cost += x.W; break;
         case PK::Cast: cost += wa * (1 + std::abs(x.S - p.n[x.a].S) / 9); break;
+        case PK::Trunc: cost += wa * (2 + (p.n[x.a].S + std::abs(x.tp)) / 9); break;
         default: break;
         }
     }
@@ -2112,7 +2189,8 @@ struct VmBuilder {
         case PK::Col:
         case PK::Const: r = 0; break;
         case PK::Neg:
-        case PK::Cast: r = std::max(1, need(x.a)); break;
+        case PK::Cast:
+        case PK::Trunc: r = std::max(1, need(x.a)); break;
         default: {
             int l = need(x.a), q = need(x.b);
             r = l == q ? std::max(1, l + (l > 0 ? 1 : 0)) : std::max(l, q);
@@ -2153,6 +2231,24 @@ struct VmBuilder {
         return true;
     }
 
+    // a constant +0 (keyed apart from every plan node)
+    bool zero_constant(updec_vm::Operand* o)
+    {
+        auto it = const_index.find(-1);
+        int k;
+        if (it != const_index.end()) {
+            k = it->second;
+        } else {
+            k = (int)const_index.size();
+            if (k >= 256) { err = "too many constants"; return false; }
+            const_index[-1] = k;
+            out.consts.push_back(0u);
+            for (int w = 0; w < out.W; w++) out.consts.push_back(0u);
+        }
+        *o = {updec_vm::K_CONST, (uint8_t)k, 0};
+        return true;
+    }
+
     bool emit(updec_vm::Op op, int dst, int arg, updec_vm::Operand a, updec_vm::Operand b)
     {
         if ((int)out.ins.size() >= updec_vm::kMaxIns) { err = "interpreter program too long"; return false; }
@@ -2178,7 +2274,7 @@ struct VmBuilder {
         if (x.kind == PK::Const) return constant(id, res);
         // b stays a (never read) constant operand for the unary operations
         updec_vm::Operand a{}, b{updec_vm::K_CONST, 0, 0};
-        const bool unary = x.kind == PK::Neg || x.kind == PK::Cast;
+        const bool unary = x.kind == PK::Neg || x.kind == PK::Cast || x.kind == PK::Trunc;
         if (unary) {
             if (!gen(x.a, &a)) return false;
         } else if (need(x.b) > need(x.a)) {
@@ -2209,6 +2305,21 @@ struct VmBuilder {
         }
         case PK::Mul: ok = emit(updec_vm::OP_MUL, dst, 0, a, b); break;
         case PK::Div: ok = emit(updec_vm::OP_DIV, dst, p.n[x.b].S + 4, a, b); break;
+        case PK::Trunc: {
+            const updec_vm::Operand self{updec_vm::K_REG, (uint8_t)dst, 0};
+            if (x.tp > 0) {
+                if (x.S > A.S) ok = emit(updec_vm::OP_UP, dst, x.S - A.S, a, {});
+                else if (x.S < A.S) ok = emit(updec_vm::OP_DOWN, dst, A.S - x.S, a, {});
+                else ok = emit(updec_vm::OP_MOV, dst, 0, a, {});
+            } else if (A.P - A.S > -x.tp) {
+                ok = emit(updec_vm::OP_DOWN, dst, A.S - x.tp, a, {}) && emit(updec_vm::OP_UP, dst, -x.tp, self, {});
+            } else {
+                // zero magnitude with the argument's sign: a times +0
+                updec_vm::Operand z;
+                ok = zero_constant(&z) && emit(updec_vm::OP_MUL, dst, 0, a, z);
+            }
+            break;
+        }
         case PK::Mod: err = "MOD runs on the JIT path only (no interpreter instruction)"; return false;
         default: err = "internal: unexpected plan node"; return false;
         }

@@ -48,9 +48,11 @@ struct ColumnType {
 // Expression tree, mirroring the reference's BinaryTree/MultipleTree
 // (AriesEngineShell.cpp:19-95) closely enough to apply the same rewrite.
 // Cast is cast(expr, P, S): Decimal(P, S).cast(value) (decimal.cu:471-533),
-// multiply up or truncate down to scale S.  It is a function node: the
-// rewrite treats it as an operand and rewrites its argument on its own.
-enum class NK { Column, Int, Dec, Calc, Cast };
+// multiply up or truncate down to scale S.  Trunc is truncate(expr, p):
+// Decimal::truncate(p) (decimal.cu:541-570) with the argument's type as its
+// prec and frac; p (in ival) may be negative.  Both are function nodes: the
+// rewrite treats them as operands and rewrites their argument on its own.
+enum class NK { Column, Int, Dec, Calc, Cast, Trunc };
 enum class Op { Add = 0, Sub = 1, Mul = 2, Div = 3, Mod = 4 };
 
 struct Expr {
@@ -74,16 +76,21 @@ ExprP parse_expr(const std::string& text, const std::vector<std::string>& names,
 // Calc*TargetPrecision, decimal.cu:1353-1358,1568-1572,1783-1786,1997-2000).
 void infer_frontend_types(Expr& e, Mode mode);
 
+// Type of truncate(x, p) for x of type (P, S): the prec and frac that
+// Decimal::truncate leaves (decimal.cu:541-570).
+void trunc_type(int P, int S, int p, int* outP, int* outS);
+
 // The reference's AlignOptimize (AriesEngineShell.cpp:646-685) applied to
 // one calculation tree.  Returns the rewritten tree whose nodes carry the
 // per-node (P,S) of ListToBinTree (AriesEngineShell.cpp:90-144).
 ExprP align_optimize(const ExprP& e, Mode mode);
 
 // Typed evaluation plan
-enum class PK { Col, Const, Neg, Add, Sub, Mul, Div, Cast, Mod };
+enum class PK { Col, Const, Neg, Add, Sub, Mul, Div, Cast, Mod, Trunc };
 struct PNode {
     PK kind;
     int P = 0, S = 0;
+    int tp = 0;              // Trunc: the truncate argument p
     int col = -1;
     HostDec k;               // Const
     int a = -1, b = -1;      // children

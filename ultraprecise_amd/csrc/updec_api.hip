@@ -630,9 +630,21 @@ int updec_expr_source(updec_expr_t e, char* buf, size_t cap, size_t* needed)
     return copy_text(e->src, buf, cap, needed);
 }
 
+// every column pointer of a launch over rows > 0 must be set: a NULL column
+// is refused on the host instead of faulting the device
+static int check_cols(const uint8_t* const* d_cols, size_t ncols, int64_t rows)
+{
+    if (rows <= 0 || ncols == 0) return UPDEC_OK;
+    if (!d_cols) return fail(UPDEC_EINVAL, "null column array");
+    for (size_t i = 0; i < ncols; i++)
+        if (!d_cols[i]) return fail(UPDEC_EINVAL, "null column " + std::to_string(i));
+    return UPDEC_OK;
+}
+
 int updec_expr_calc(updec_expr_t e, const uint8_t* const* d_cols, int64_t rows, uint8_t* d_out, updec_stream_t stream)
 {
     if (!e || rows < 0 || (rows && (!d_cols || !d_out))) return fail(UPDEC_EINVAL, "bad argument");
+    if (int rc = check_cols(d_cols, e->types.size(), rows)) return rc;
     if (reinterpret_cast<uintptr_t>(d_out) & 3u) return fail(UPDEC_EINVAL, "output must be 4-byte aligned");
     if (rows == 0) return UPDEC_OK;
     std::string err;
@@ -907,6 +919,7 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
                   int32_t* ngroups_out, uint16_t* keys_out, updec_stream_t stream)
 {
     if (!a || rows < 0 || !d_images || !d_counts || !ngroups_out) return fail(UPDEC_EINVAL, "bad argument");
+    if (int rc = check_cols(d_cols, a->types.size(), rows)) return rc;
     if (max_groups < 1 || max_groups > 255) return fail(UPDEC_EINVAL, "max_groups must be in [1, 255]");
     if (a->has_filter && !d_filter) return fail(UPDEC_EINVAL, "filter column missing");
     if (a->nkeys >= 1 && !d_key0) return fail(UPDEC_EINVAL, "key column missing");
