@@ -84,10 +84,13 @@ def headline(U, W, steps, warmup, cpu_rows, threads):
         "column_bits": bits,
     }
     # CPU baseline: the oracle's restatement of the reference's host Decimal
-    # path on a bounded sample of the same rows
+    # path over the same rows (all of them by default), built -march=native on
+    # this host; its group images and counts are then compared byte for byte
+    # with the GPU's from the last timed step
     if cpu_rows > 0:
         from oracle import oracle as O
         import numpy as np
+        native = O.use_native()
         n = min(cpu_rows, rows)
         host = [t.numpy(count=n * U.compact_bytes(c[1])) for t, c in zip(dev, C["cols"])]
         hf1, hf2 = f1.numpy(count=n), f2.numpy(count=n)
@@ -106,11 +109,25 @@ def headline(U, W, steps, warmup, cpu_rows, threads):
         for p in plans:
             img, cnt = O.plan_group_sum(p, host, P, S, n, gid, len(keys), threads=threads)
             sums.append(img)
-        for k in (0, 1, 4):
-            for g in range(len(keys)):
-                O.avg_from_sum(bytes(sums[k][g]), int(cnt[g]))
+        avgs = {k: [O.avg_from_sum(bytes(sums[k][g]), int(cnt[g])) for g in range(len(keys))] for k in (0, 1, 4)}
         cpu_s = time.perf_counter() - t0
-        res["cpu"] = {"rows": n, "seconds": cpu_s, "mrows_per_s": n / cpu_s / 1e6, "threads": threads}
+        model, visible, allowed = O.host_cpu()
+        res["cpu"] = {"rows": n, "seconds": cpu_s, "mrows_per_s": n / cpu_s / 1e6, "threads": threads,
+                      "cpu_model": model, "cpus_visible": visible, "cpus_allowed": allowed,
+                      "build": "gcc -O3 -march=native" if native else "gcc -O3 (portable)"}
+        if n == rows:
+            got = images.numpy(count=maxg * (ne + 3) * 133).reshape(-1, 133)
+            gcnt = counts.numpy("int64", count=maxg)
+            same = ng == len(keys) and all(int(gcnt[g]) == int(cnt[g]) for g in range(ng))
+            for k in range(ne):
+                for g in range(ng):
+                    same = same and bytes(got[g * ne + k]) == bytes(sums[k][g])
+            for j, k in enumerate((0, 1, 4)):
+                for g in range(ng):
+                    same = same and bytes(got[maxg * ne + g * 3 + j]) == avgs[k][g]
+            res["gpu_equals_cpu_all_rows"] = bool(same)
+            if not same:
+                log("headline: GPU images differ from the CPU oracle over all rows")
     return res
 
 
@@ -262,6 +279,56 @@ def codec(U, W, steps, cpu_rows):
         cpu_s = time.perf_counter() - t0
         out["parse_text"]["cpu"] = {"rows": n, "seconds": cpu_s, "mrows_per_s": n / cpu_s / 1e6, "threads": 1,
                                     "kind": "port"}
+    return out
+
+
+def segsum_bench(U, W, steps, cpu_rows, threads):
+    """the segmented SUM (aggregate_column over `associated`/`groups`): the
+    Q1 price column, 60,000,000 rows of DECIMAL(15,2), in the order a sort
+    by (flag1, flag2) gives (associated = the stably sorted row numbers,
+    groups = the six segment starts), and the same column in row order cut
+    into 1,000,000 segments of 60 rows"""
+    import numpy as np
+    C = W.CONFIG3
+    rows = C["rows"]
+    P, S = 15, 2
+    B = U.compact_bytes(P)
+    col = U.gen_compact_device(W.gen_desc(U, C["seed"], 1, C["cols"][1]), rows)
+    f1 = U.gen_byte_host(C["seed"], W.COL_FLAG1, C["flag1"], rows)
+    f2 = U.gen_byte_host(C["seed"], W.COL_FLAG2, C["flag2"], rows)
+    key = (f1.astype(np.int32) << 8) | f2
+    assoc = np.argsort(key, kind="stable").astype(np.int32)
+    skey = key[assoc]
+    starts = np.flatnonzero(np.concatenate([[True], skey[1:] != skey[:-1]])).astype(np.int32)
+    del key, skey
+    cases = {"by_key_6_segments": (U.DeviceBuffer.from_numpy(assoc), starts),
+             "row_order_1M_segments": (None, (np.arange(1_000_000, dtype=np.int64) * (rows // 1_000_000)).astype(np.int32))}
+    out = {}
+    ev0, ev1 = U.Event(), U.Event()
+    for name, (da, groups) in cases.items():
+        dg = U.DeviceBuffer.from_numpy(groups)
+        img = U.DeviceBuffer(len(groups) * 133)
+        U.segsum_compact(col, P, S, rows, da, dg, len(groups), img)
+        U.device_sync()
+        ev0.record()
+        for _ in range(steps):
+            U.segsum_compact(col, P, S, rows, da, dg, len(groups), img)
+        ev1.record()
+        ms = ev0.elapsed_ms(ev1) / steps
+        alg = rows * (B + (4 if da is not None else 0)) + len(groups) * (4 + 133)
+        r = {"rows": rows, "segments": len(groups), "ms": ms, "mrows_per_s": rows / ms / 1e3, "alg_bytes": alg,
+             "achieved_gbps": alg / ms / 1e6}
+        if cpu_rows > 0 and name.startswith("by_key"):
+            from oracle import oracle as O
+            hcol = col.numpy(count=rows * B)
+            t0 = time.perf_counter()
+            want = O.segsum_compact(hcol, P, S, rows, assoc, groups, threads=threads)
+            cpu_s = time.perf_counter() - t0
+            r["cpu"] = {"rows": rows, "seconds": cpu_s, "mrows_per_s": rows / cpu_s / 1e6, "threads": threads,
+                        "kind": "port"}
+            r["gpu_equals_cpu_all_rows"] = bool(np.array_equal(img.numpy().reshape(-1, 133), want))
+        out[name] = r
+        del dg, img
     return out
 
 
@@ -431,10 +498,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-rows", type=int, default=2_000_000, help="rows of the CPU baseline sample")
+    ap.add_argument("--cpu-rows", type=int, default=60_000_000,
+                    help="rows of the CPU baseline (all 60,000,000 by default; then also compared with the GPU)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="headline only")
-    ap.add_argument("--extra", default="copy,plain,sweep,series,serving,codec,rsa", help="comma list of extra workloads")
+    ap.add_argument("--extra", default="copy,plain,sweep,series,serving,codec,rsa,segsum",
+                    help="comma list of extra workloads")
     ap.add_argument("--sweep-rows", type=int, default=0)
     ap.add_argument("--series-cpu-rows", type=int, default=50_000, help="rows of the series CPU sample")
     ap.add_argument("--codec-cpu-rows", type=int, default=1_000_000, help="rows of the text-ingest CPU sample")
@@ -451,7 +520,14 @@ def main():
     from ultraprecise_amd import workloads as W
     U.init(0)
     info = U.device_info()
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    # the CPU leg's threads: every CPU this process may run on, up to the CPU
+    # share the GPU host gives one GPU (OMP_NUM_THREADS there)
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS") or allowed)
+    threads = args.cpu_threads or max(1, min(allowed, share))
     log("device:", info)
     h = headline(U, W, args.steps, args.warmup, args.cpu_rows, threads)
     log("headline:", h)
@@ -472,6 +548,9 @@ def main():
     if "rsa" in want:
         extra["rsa_10M"] = rsa(U, W, max(3, args.steps // 4), args.rsa_cpu_rows, threads)
         log("rsa:", extra["rsa_10M"])
+    if "segsum" in want:
+        extra["segsum_60M"] = segsum_bench(U, W, max(3, args.steps // 4), args.cpu_rows, threads)
+        log("segsum:", extra["segsum_60M"])
     if "codec" in want:
         extra["codec_10M"] = codec(U, W, max(3, args.steps // 4), args.codec_cpu_rows)
         log("codec:", extra["codec_10M"])
@@ -506,9 +585,15 @@ def main():
         },
         "cpu_baseline": None if cpu is None else {
             "value": round(cpu["mrows_per_s"], 4), "unit": "Mrows/s", "cores": cpu["threads"], "kind": "port",
-            "sample": f"first {cpu['rows']:,} of {h['rows']:,} rows, same query (4 SUM + 3 AVG + COUNT), "
-                      f"oracle restatement of the reference's host Decimal, {cpu['threads']} threads",
+            "sample": (f"all {h['rows']:,} rows" if cpu["rows"] == h["rows"] else
+                       f"first {cpu['rows']:,} of {h['rows']:,} rows") +
+                      f", same query (4 SUM + 3 AVG + COUNT), oracle restatement of the reference's host Decimal "
+                      f"({cpu['build']}), {cpu['threads']} threads on {cpu['cpu_model']} "
+                      f"({cpu['cpus_allowed']} of {cpu['cpus_visible']} CPUs allowed; the host's share for one GPU "
+                      f"is the thread count)",
+            "cpu_model": cpu["cpu_model"],
         },
+        "exact": {"headline_gpu_equals_cpu_all_rows": h.get("gpu_equals_cpu_all_rows")},
         "device": info["name"],
         "extra": extra,
     }

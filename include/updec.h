@@ -103,6 +103,8 @@ int updec_expr_info(updec_expr_t e, int32_t* precision, int32_t* scale, int32_t*
  * needed size in *needed) */
 int updec_expr_plan(updec_expr_t e, char* buf, size_t cap, size_t* needed);
 int updec_expr_source(updec_expr_t e, char* buf, size_t cap, size_t* needed);
+/* source of the column-iterator variant (updec_expr_calc_iter) */
+int updec_expr_source_iter(updec_expr_t e, char* buf, size_t cap, size_t* needed);
 /* Evaluate every row into compact output rows of out_bytes each:
  * AEExprDynKernelNode::Process / RunKernelFunction
  * (src/AriesEngine/AriesExprCalcNode.cpp:543-575) and
@@ -131,6 +133,50 @@ typedef struct updec_column_blocks {
  * byte per row, 1 = value; may be NULL).  d_out 4-byte aligned. */
 int updec_expr_calc_blocks(updec_expr_t e, const updec_column_blocks* cols, const int32_t* d_indices, int64_t rows,
                            uint8_t* d_out, uint8_t* d_valid, updec_stream_t stream);
+
+/* Column iterators: AriesColumnDataIterator (src/AriesColumnDataIterator.hxx:8-41)
+ * with its operator[] (src/AriesColumnDataIterator.cu:1-117), one per column
+ * of a launch.  The column's data is nblocks device blocks of rows; block k
+ * holds positions [d_starts[k], d_starts[k+1]) (the last one up to
+ * `positions`), d_starts[0] = 0, ascending.  A row is the compact decimal
+ * (B(P) bytes), or with `nullable` the reference's nullable_type<CompactDecimal>:
+ * a flag byte (0 = NULL) then the B(P) compact bytes.  Without an index,
+ * launch row i reads position i.  With an index, launch row i reads the
+ * position the index holds at i: UPDEC_INDEX_INT32 entries are int32 (-1 =
+ * NULL), UPDEC_INDEX_NULLABLE_INT32 entries are nullable_type<int32_t> (5
+ * packed bytes: flag, then the int32; flag 0 = NULL); the index is one array
+ * (nindex_blocks = 1, d_index, 4-byte aligned for int32) or nindex_blocks
+ * blocks (d_index_blocks, with block k starting at launch row
+ * d_index_starts[k]; device arrays).  A NULL index entry, a position outside
+ * [0, positions) or a NULL row gives a NULL value.  Every pointer is
+ * device memory. */
+#define UPDEC_INDEX_NONE 0
+#define UPDEC_INDEX_INT32 1
+#define UPDEC_INDEX_NULLABLE_INT32 2
+typedef struct updec_column_iter {
+    const uint8_t* const* d_blocks;
+    const int64_t* d_starts;
+    int32_t nblocks;
+    int32_t nullable;
+    int64_t positions;
+    int32_t index_kind;
+    int32_t nindex_blocks;
+    const void* d_index;
+    const void* const* d_index_blocks;
+    const int64_t* d_index_starts;
+} updec_column_iter;
+/* updec_expr_calc over column iterators, the reference's calc_expr_<id>
+ * with its input iterators (AriesExprCalcNode.cpp:1292-1296, load and store
+ * code :76-108, :1402-1425).  A result is NULL when a column it reads is NULL,
+ * or when a division or MOD with an operand that can be NULL (a column with
+ * nullable rows or an index, or an expression over one) has a zero divisor
+ * (nullable_type operator/ and %, src/datatypes/aries_types.hxx:626-663).
+ * Output row i: with out_nullable = 0, B(P) compact bytes (zero for NULL);
+ * with out_nullable = 1, the reference's nullable output row, a flag byte
+ * then B(P) compact bytes (zero for NULL).  d_valid (optional): one byte per
+ * row, 1 = value, 0 = NULL.  d_out 4-byte aligned. */
+int updec_expr_calc_iter(updec_expr_t e, const updec_column_iter* cols, int64_t rows, uint8_t* d_out,
+                         int32_t out_nullable, uint8_t* d_valid, updec_stream_t stream);
 
 /* ---------------- fused aggregation ----------------
  * Replaces AEExprAggFunctionNode::RunKernelFunction
@@ -177,6 +223,8 @@ int updec_agg_last_kernel_ms(updec_agg_t a, float* ms);
 /* generated source for a given group count of the two-pass form, or of the
  * one-pass form with ngroups = 0 (no device needed) */
 int updec_agg_source(updec_agg_t a, int32_t ngroups, char* buf, size_t cap, size_t* needed);
+/* source of the column-iterator variant (updec_agg_run_iter) for ngroups >= 1 */
+int updec_agg_source_iter(updec_agg_t a, int32_t ngroups, char* buf, size_t cap, size_t* needed);
 void updec_agg_free(updec_agg_t a);
 /* updec_agg_run over block columns and indices, for aggregations without
  * filter and keys (one group): SUM of each expression over the positions
@@ -185,6 +233,18 @@ void updec_agg_free(updec_agg_t a);
  * then one AVG image per avg_mask bit; d_counts: one int64. */
 int updec_agg_run_blocks(updec_agg_t a, const updec_column_blocks* cols, const int32_t* d_indices, int64_t rows,
                          uint8_t* d_images, int64_t* d_counts, updec_stream_t stream);
+/* updec_agg_run over column iterators, with the descriptor's filter and
+ * keys: d_filter, d_key0 and d_key1 hold one value per launch row (already
+ * gathered, as the filter and group production of the reference hand
+ * them).  SUM(expr_k) adds the non-NULL values of expr_k (agg_sum_t of
+ * nullable_type, src/CudaAcc/algorithm/operators.hxx:1175-1195); counts[g]
+ * is COUNT(*) of the rows of group g that pass the filter.  Groups come
+ * from the two-pass form (key discovery, then one pass).  AVG through
+ * avg_mask needs non-NULL counts, so it is refused (UPDEC_EUNSUPPORTED) when
+ * a column can be NULL. */
+int updec_agg_run_iter(updec_agg_t a, const updec_column_iter* cols, int64_t rows, const int32_t* d_filter,
+                       const uint8_t* d_key0, const uint8_t* d_key1, uint8_t* d_images, int64_t* d_counts,
+                       int32_t max_groups, int32_t* ngroups_out, uint16_t* keys_out, updec_stream_t stream);
 
 /* AVG = SUM / COUNT per group: DivisionInt64 (src/CudaAcc/AriesSqlOperator_helpers.cu:717-764),
  * division_data_int64 (src/CudaAcc/AriesEngineAlgorithm.cu:9476-9535), do_div
@@ -196,8 +256,16 @@ int updec_div_int64(const uint8_t* d_sums, int64_t stride_bytes, const int64_t* 
 /* Segmented SUM of a compact column in `associated` order with segment
  * starts `groups`: aggregate_column<CompactDecimal> SUM
  * (src/CudaAcc/AriesEngineAlgorithm.h:832-857) through segreduce
- * (src/CudaAcc/algorithm/kernel_segreduce.hxx:535-612).  d_associated may
- * be NULL (identity).  d_out: ngroups 133-byte images. */
+ * (src/CudaAcc/algorithm/kernel_segreduce.hxx:535-612).  Position i reads
+ * row d_associated[i] (d_associated may be NULL: row i); segment s sums
+ * positions [groups[s], groups[s+1]) (the last up to `rows`); positions
+ * before groups[0] are in no segment.  Starts must ascend (as the
+ * reference's group production gives them); they are read on the device
+ * only, so descending starts give meaningless sums but no access outside
+ * the buffers.  d_out: ngroups 133-byte images (prec P, frac S, +0 for an
+ * empty segment).  Rows of at most 128 bytes (the 32-limb Decimal) and at
+ * most 2^31 positions.  Stream-ordered; allocates its accumulators with
+ * hipMallocAsync on the stream. */
 int updec_segsum_compact(const uint8_t* d_col, int precision, int scale, int64_t rows, const int32_t* d_associated,
                          const int32_t* d_groups, int32_t ngroups, uint8_t* d_out, int mode, updec_stream_t stream);
 
@@ -243,6 +311,15 @@ void updec_module_free(updec_module_t m);
 /* hipRTC-compile a source for gfx950 without loading it (no device needed);
  * returns UPDEC_ECOMPILE with the compiler log on failure */
 int updec_check_compile(const char* src, size_t len);
+/* Module cache counters since the process started: hipRTC compiles, hits in
+ * the in-memory LRU (100 modules, keyed by source text as the reference's
+ * DynamicKernel.cu:10-58) and loads from the on-disk store of compiled code
+ * objects ($UPDEC_KERNEL_CACHE, "off" disables; else
+ * $XDG_CACHE_HOME/updec or ~/.cache/updec).  Generated sources carry no
+ * constant values (constants are kernel data, the reference's constValues),
+ * so expressions that differ only in literals of the same limb widths share
+ * one compile. */
+int updec_module_cache_stats(int64_t* rtc_compiles, int64_t* memory_hits, int64_t* disk_hits);
 
 /* ---------------- interpreter and batches ----------------
  * The same plans as the expression path, run by an interpreter compiled once

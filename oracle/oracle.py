@@ -35,10 +35,10 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
-def _load():
-    if not os.path.exists(LIB_PATH):
+def _load(path=LIB_PATH):
+    if not os.path.exists(path):
         build()
-    lib = ctypes.CDLL(LIB_PATH)
+    lib = ctypes.CDLL(path)
     D = ctypes.POINTER(OdDec)
     P, I64 = ctypes.c_void_p, ctypes.c_int64
     sig = {
@@ -88,6 +88,40 @@ def _load():
 
 
 lib = _load()
+
+
+def use_native():
+    """Rebuild the oracle with -O3 -march=native for the host this process
+    runs on (make native) and route every call below through it.  Used by
+    the CPU baseline leg only, before its timed region; returns the library
+    path, or None when the native build fails (the portable build stays)."""
+    import subprocess
+    global lib
+    path = os.path.join(_HERE, "liboracle_native.so")
+    try:
+        subprocess.check_call(["make", "-s", "-B", "-C", _HERE, "native"], stdout=subprocess.DEVNULL)
+        lib = _load(path)
+    except (OSError, subprocess.CalledProcessError):
+        return None
+    return path
+
+
+def host_cpu():
+    """(model name, logical CPUs visible, CPUs this process may run on)"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, allowed
 
 
 def dec(s, mode=FAITHFUL):

@@ -1,5 +1,5 @@
-"""CPU: the block-column kernels (AriesColumnDataIterator inputs) that the C
-ABI generates only at first use, generated here by the planner built for
+"""CPU: the column-iterator kernels (AriesColumnDataIterator inputs) that the
+C ABI generates only at first use, generated here by the planner built for
 the host and compiled for gfx950 with hipRTC; the row mapping they contain."""
 import ctypes
 import os
@@ -40,6 +40,7 @@ def test_block_kernels_compile(U, gen, text, cols, agg):
     src = gen(text, cols, agg)
     assert not src.startswith("error"), src
     assert "IterArgs a" in src and "col_load<" in src
-    assert "a.idx ? (i64)a.idx[" in src and "a.rowlimit" in src
+    # every column load goes through its iterator and reports NULL
+    assert "const ColIter& t = a.it[c];" in src and ", row, nc" in src
     assert "load_quad" not in src and "stage_wave" not in src
     U.check_compile(src)

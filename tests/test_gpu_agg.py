@@ -236,14 +236,18 @@ def test_fused_avg_modes(U, O, gpu, mode, nkeys):
             assert bytes(fused[g, j]) == O.avg_from_sum(bytes(got[g, k]), int(gotc[g]), mode=om), (g, k)
 
 
-@pytest.mark.parametrize("P,S", [(15, 2), (38, 10), (76, 20)])
-def test_segsum_compact_bit_exact(U, O, gpu, P, S):
-    rng = np.random.default_rng(P)
+@pytest.mark.parametrize("P,S", [(15, 2), (38, 10), (76, 20), (150, 50), (290, 90)])
+@pytest.mark.parametrize("nseg", [40, 20_000])
+def test_segsum_compact_bit_exact(U, O, gpu, P, S, nseg):
+    """large segments (every tile inside one) and small ones (tiles crossing
+    many starts), with empty segments, a segment at the last row and
+    positions in a random `associated` order"""
+    rng = np.random.default_rng(P + nseg)
     rows = 300_000
     host = U.gen_compact_host(U.gen_desc(U.GEN_DIGITS, P, 9, P, digits=P - 3, neg_permille=500), rows)
     assoc = rng.permutation(rows).astype(np.int32)
-    cuts = np.sort(rng.choice(np.arange(1, rows), size=40, replace=False))
-    groups = np.concatenate([[0], cuts, [rows - 5, rows - 5]]).astype(np.int32)  # includes empty segments
+    cuts = np.sort(rng.choice(np.arange(1, rows), size=nseg, replace=False))
+    groups = np.concatenate([[0], cuts, [rows - 5, rows - 5, rows - 1]]).astype(np.int32)  # includes empty segments
     groups = np.sort(groups)
     col = U.DeviceBuffer.from_numpy(host)
     da = U.DeviceBuffer.from_numpy(assoc)
@@ -253,3 +257,26 @@ def test_segsum_compact_bit_exact(U, O, gpu, P, S):
     got = out.numpy().reshape(-1, 133)
     want = O.segsum_compact(host, P, S, rows, assoc, groups)
     assert np.array_equal(got, want)
+
+
+def test_segsum_compact_identity_order_and_edges(U, O, gpu):
+    """no `associated` (position i is row i); positions before the first
+    start belong to no segment; zero rows give +0 images"""
+    P, S = 38, 10
+    rows = 50_000
+    host = U.gen_compact_host(U.gen_desc(U.GEN_DIGITS, P, 11, 2, digits=30, neg_permille=500), rows)
+    groups = np.array([100, 100, 4096, 4097, 8191, 30_000], dtype=np.int32)
+    col = U.DeviceBuffer.from_numpy(host)
+    dg = U.DeviceBuffer.from_numpy(groups)
+    out = U.DeviceBuffer(len(groups) * 133).zero()
+    U.segsum_compact(col, P, S, rows, None, dg, len(groups), out)
+    got = out.numpy().reshape(-1, 133)
+    want = O.segsum_compact(host, P, S, rows, None, groups)
+    assert np.array_equal(got, want)
+    # zero rows: every start is 0, every segment empty
+    g0 = np.zeros(4, dtype=np.int32)
+    dg0 = U.DeviceBuffer.from_numpy(g0)
+    out0 = U.DeviceBuffer(len(g0) * 133).zero()
+    U.segsum_compact(col, P, S, 0, None, dg0, len(g0), out0)
+    zero = O.segsum_compact(host, P, S, 0, None, g0)
+    assert np.array_equal(out0.numpy().reshape(-1, 133), zero)

@@ -160,7 +160,7 @@ def test_descriptor_layouts_match_header(U, tmp_path):
     if not shutil.which("gcc"):
         pytest.skip("no host C compiler")
     mirrors = {"updec_column_desc": U.ColumnDesc, "updec_column_blocks": U.ColumnBlocks,
-               "updec_agg_desc": U.AggDesc, "updec_gen_desc": U.GenDesc}
+               "updec_agg_desc": U.AggDesc, "updec_gen_desc": U.GenDesc, "updec_column_iter": U.ColumnIterDesc}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "updec.h"', "int main(void) {"]
     for cname, py in mirrors.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
@@ -194,3 +194,37 @@ def test_descriptor_layouts_match_header(U, tmp_path):
 def test_documented_rejections(U, text, why):
     with pytest.raises(U.UpdecError, match=why):
         U.Expr(text, [("a", 18, 4), ("b", 15, 2)])
+
+
+def test_constants_are_kernel_data(U):
+    """generated sources carry no constant values: expressions that differ
+    only in literals of the same limb widths give one source text (one
+    hipRTC compile); the values travel in the plan's constant block"""
+    cols = [("p", 15, 2), ("d", 15, 2), ("t", 15, 2)]
+    a = U.Expr("p*(1-d)*(1.25+t)", cols)
+    b = U.Expr("p*(3-d)*(7.50+t)", cols)
+    assert a.source == b.source
+    assert a.plan != b.plan
+    assert "0x7d" not in a.source and "a.kc[" in a.source
+    ga = U.Agg(["p*(1-d)", "d+0.05"], cols, nkeys=1).source(0)
+    gb = U.Agg(["p*(2-d)", "d+0.07"], cols, nkeys=1).source(0)
+    assert ga == gb
+    # a literal too wide for the first one's limbs changes the source
+    c = U.Expr("p*(12345678901-d)*(1.25+t)", cols)
+    assert c.source != a.source
+    U.check_compile(a.source)
+
+
+def test_iterator_kernels_compile(U):
+    """the column-iterator variants (NULL flags, index kinds, nullable
+    output) of the calc kernel and of the grouped, filtered aggregation"""
+    from ultraprecise_amd import workloads as W
+    cols = [("a", 38, 10), ("b", 38, 10), ("c", 15, 2)]
+    for text in ("a/b + c", "(a+b)*c - a/c", "a % b"):
+        e = U.Expr(text, cols)
+        U.check_compile(e.iter_source())
+    q1 = [(c[0], c[1], c[2]) for c in W.CONFIG3["cols"]]
+    a = U.Agg(W.CONFIG3["exprs"], q1, filter=W.CONFIG3["filter"], nkeys=2)
+    src = a.source(6, iterators=True)
+    assert "col_load<7>(a, 1, row, nc1)" in src and "p2_nn" in src
+    U.check_compile(src)

@@ -64,6 +64,62 @@ def _blocks_array(columns):
     return arr
 
 
+INDEX_NONE, INDEX_INT32, INDEX_NULLABLE_INT32 = 0, 1, 2
+
+
+class ColumnIterDesc(ctypes.Structure):
+    _fields_ = [("d_blocks", ctypes.c_void_p), ("d_starts", ctypes.c_void_p), ("nblocks", ctypes.c_int32),
+                ("nullable", ctypes.c_int32), ("positions", ctypes.c_int64), ("index_kind", ctypes.c_int32),
+                ("nindex_blocks", ctypes.c_int32), ("d_index", ctypes.c_void_p), ("d_index_blocks", ctypes.c_void_p),
+                ("d_index_starts", ctypes.c_void_p)]
+
+
+class ColumnIter:
+    """A column as the reference's AriesColumnDataIterator: data `blocks`
+    (DeviceBuffers or views) with `counts` rows each, rows of B(P) bytes or,
+    with nullable=True, a flag byte then B(P) bytes; optional index blocks
+    (DeviceBuffers of int32 positions, -1 = NULL, or of 5-byte
+    nullable_type<int32> entries with index_kind=INDEX_NULLABLE_INT32) with
+    `index_counts` entries each.  Keeps the device arrays alive."""
+
+    def __init__(self, blocks, counts, nullable=False, index=None, index_counts=None, index_kind=INDEX_INT32):
+        import numpy as np
+        if len(blocks) != len(counts) or not blocks:
+            raise ValueError("one row count per block")
+        self.blocks = list(blocks)
+        starts = np.zeros(len(counts), dtype=np.int64)
+        starts[1:] = np.cumsum(counts)[:-1]
+        self.positions = int(np.sum(counts))
+        self._ptrs = DeviceBuffer.from_numpy(np.array([b.ptr for b in blocks], dtype=np.uint64))
+        self._starts = DeviceBuffer.from_numpy(starts)
+        self.desc = ColumnIterDesc(self._ptrs.ptr, self._starts.ptr, len(blocks), int(bool(nullable)), self.positions,
+                                   INDEX_NONE, 0, None, None, None)
+        self.index = None
+        if index is not None:
+            index = list(index)
+            self.index = index
+            self.desc.index_kind = index_kind
+            self.desc.nindex_blocks = len(index)
+            if len(index) == 1:
+                self.desc.d_index = index[0].ptr
+            else:
+                if index_counts is None or len(index_counts) != len(index):
+                    raise ValueError("one entry count per index block")
+                ist = np.zeros(len(index), dtype=np.int64)
+                ist[1:] = np.cumsum(index_counts)[:-1]
+                self._iptrs = DeviceBuffer.from_numpy(np.array([b.ptr for b in index], dtype=np.uint64))
+                self._istarts = DeviceBuffer.from_numpy(ist)
+                self.desc.d_index_blocks = self._iptrs.ptr
+                self.desc.d_index_starts = self._istarts.ptr
+
+
+def _iter_array(columns):
+    arr = (ColumnIterDesc * max(1, len(columns)))()
+    for i, c in enumerate(columns):
+        arr[i] = c.desc
+    return arr
+
+
 class GenDesc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("precision", ctypes.c_int32), ("digits", ctypes.c_int32),
                 ("neg_permille", ctypes.c_int32), ("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64),
@@ -106,6 +162,7 @@ def _load():
                                            ctypes.POINTER(I32)]),
         "updec_expr_plan": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
         "updec_expr_source": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
+        "updec_expr_source_iter": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
         "updec_expr_calc": (ctypes.c_int, [P, ctypes.POINTER(P), I64, P, P]),
         "updec_expr_free": (None, [P]),
         "updec_agg_compile": (ctypes.c_int, [ctypes.POINTER(AggDesc), ctypes.POINTER(P)]),
@@ -113,6 +170,7 @@ def _load():
                                          ctypes.POINTER(ctypes.c_uint16), P]),
         "updec_agg_last_kernel_ms": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_float)]),
         "updec_agg_source": (ctypes.c_int, [P, I32, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
+        "updec_agg_source_iter": (ctypes.c_int, [P, I32, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
         "updec_agg_free": (None, [P]),
         "updec_div_int64": (ctypes.c_int, [P, I64, P, P, I64, ctypes.c_int, P]),
         "updec_segsum_compact": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I64, P, P, I32, P, ctypes.c_int, P]),
@@ -120,6 +178,7 @@ def _load():
         "updec_launch": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, P, SZ, P]),
         "updec_module_free": (None, [P]),
         "updec_check_compile": (ctypes.c_int, [ctypes.c_char_p, SZ]),
+        "updec_module_cache_stats": (ctypes.c_int, [ctypes.POINTER(I64)] * 3),
         "updec_gen_compact": (ctypes.c_int, [ctypes.POINTER(GenDesc), I64, I64, P, ctypes.c_int, P]),
         "updec_gen_int32": (ctypes.c_int, [U64, ctypes.c_uint32, I32, I32, I64, I64, P, ctypes.c_int, P]),
         "updec_gen_byte": (ctypes.c_int, [U64, ctypes.c_uint32, P, I32, I64, I64, P, ctypes.c_int, P]),
@@ -137,6 +196,9 @@ def _load():
         "updec_parse_compact": (ctypes.c_int, [P, P, I64, I32, I32, P, P]),
         "updec_expr_calc_blocks": (ctypes.c_int, [P, ctypes.POINTER(ColumnBlocks), P, I64, P, P, P]),
         "updec_agg_run_blocks": (ctypes.c_int, [P, ctypes.POINTER(ColumnBlocks), P, I64, P, P, P]),
+        "updec_expr_calc_iter": (ctypes.c_int, [P, ctypes.POINTER(ColumnIterDesc), I64, P, I32, P, P]),
+        "updec_agg_run_iter": (ctypes.c_int, [P, ctypes.POINTER(ColumnIterDesc), I64, P, P, P, P, P, I32,
+                                              ctypes.POINTER(I32), ctypes.POINTER(ctypes.c_uint16), P]),
         "updec_format_stride": (ctypes.c_int, [I32, I32, I32, ctypes.POINTER(I64)]),
         "updec_format_compact": (ctypes.c_int, [P, I64, I32, I32, I32, P, I64, P]),
     }
@@ -158,11 +220,12 @@ EXPORTED = ["updec_last_error", "updec_version", "updec_init", "updec_compact_by
             "updec_expr_compile", "updec_expr_info", "updec_expr_plan", "updec_expr_source", "updec_expr_calc",
             "updec_expr_free", "updec_agg_compile", "updec_agg_run", "updec_agg_last_kernel_ms", "updec_agg_source",
             "updec_agg_free", "updec_div_int64", "updec_segsum_compact", "updec_compile", "updec_launch",
-            "updec_module_free", "updec_check_compile", "updec_gen_compact", "updec_gen_int32", "updec_gen_byte",
+            "updec_module_free", "updec_check_compile", "updec_module_cache_stats", "updec_gen_compact", "updec_gen_int32", "updec_gen_byte",
             "updec_vm_compile", "updec_vm_info", "updec_vm_listing", "updec_vm_free", "updec_batch_create",
             "updec_batch_tuning", "updec_batch_add", "updec_batch_pending", "updec_batch_launch", "updec_batch_free",
             "updec_parse_compact", "updec_format_stride", "updec_format_compact", "updec_expr_calc_blocks",
-            "updec_agg_run_blocks"]
+            "updec_agg_run_blocks", "updec_expr_calc_iter", "updec_agg_run_iter", "updec_expr_source_iter",
+            "updec_agg_source_iter"]
 
 
 def lib():
@@ -330,12 +393,23 @@ class Expr:
     def source(self):
         return _text(_lib.updec_expr_source, self._h)
 
+    def iter_source(self):
+        """source of the column-iterator variant (calc_iter)"""
+        return _text(_lib.updec_expr_source_iter, self._h)
+
     def calc_blocks(self, columns, rows, out, indices=None, valid=None, stream=None):
         """columns: BlockColumns in descriptor order; output row i is the
         expression at position indices[i] (int32 DeviceBuffer, -1 = NULL)
         or i; valid (optional DeviceBuffer, one byte per row) gets 1 / 0"""
         _check(_lib.updec_expr_calc_blocks(self._h, _blocks_array(columns), _ptr(indices), rows, _ptr(out), _ptr(valid),
                                            _stream(stream)))
+
+    def calc_iter(self, columns, rows, out, out_nullable=False, valid=None, stream=None):
+        """columns: ColumnIters in descriptor order; output row i of out_bytes
+        (or 1 + out_bytes with out_nullable: flag byte first); valid
+        (optional DeviceBuffer, one byte per row) gets 1 / 0 (NULL)"""
+        _check(_lib.updec_expr_calc_iter(self._h, _iter_array(columns), rows, _ptr(out), int(bool(out_nullable)),
+                                         _ptr(valid), _stream(stream)))
 
     def calc(self, columns, rows, out, stream=None):
         """columns: DeviceBuffers in descriptor order; out: DeviceBuffer of
@@ -376,14 +450,26 @@ class Agg:
         self._h = h
         self.nkeys = nkeys
 
-    def source(self, ngroups=1):
-        return _text(_lib.updec_agg_source, self._h, ctypes.c_int32(ngroups))
+    def source(self, ngroups=1, iterators=False):
+        fn = _lib.updec_agg_source_iter if iterators else _lib.updec_agg_source
+        return _text(fn, self._h, ctypes.c_int32(ngroups))
 
     def run_blocks(self, columns, rows, images, counts, indices=None, stream=None):
         """SUM of each expression over BlockColumns, at positions indices[0..rows)
         (or 0..rows); NULL positions are skipped.  Ungrouped, unfiltered."""
         _check(_lib.updec_agg_run_blocks(self._h, _blocks_array(columns), _ptr(indices), rows, _ptr(images),
                                          _ptr(counts), _stream(stream)))
+
+    def run_iter(self, columns, rows, images, counts, filter_col=None, key0=None, key1=None, max_groups=255,
+                 stream=None):
+        """SUM / COUNT(*) over ColumnIters with the filter and keys given per
+        launch row; NULL values add to no SUM.  Returns (ngroups, keys)."""
+        ng = ctypes.c_int32(0)
+        keys = (ctypes.c_uint16 * max(1, max_groups))()
+        _check(_lib.updec_agg_run_iter(self._h, _iter_array(columns), rows, _ptr(filter_col), _ptr(key0), _ptr(key1),
+                                       _ptr(images), _ptr(counts), max_groups, ctypes.byref(ng), keys,
+                                       _stream(stream)))
+        return ng.value, [keys[i] for i in range(ng.value)]
 
     def run(self, columns, rows, images, counts, filter_col=None, key0=None, key1=None, max_groups=255, stream=None):
         """Returns (ngroups, keys).  images: DeviceBuffer of at least
@@ -447,6 +533,13 @@ def column_maxbits(col, precision, rows, stream=None):
     b = ctypes.c_int32(0)
     _check(_lib.updec_column_maxbits(_ptr(col), precision, rows, ctypes.byref(b), _stream(stream)))
     return b.value
+
+
+def module_cache_stats():
+    """(hipRTC compiles, in-memory hits, on-disk hits) since the process started"""
+    c, m, d = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    _check(_lib.updec_module_cache_stats(ctypes.byref(c), ctypes.byref(m), ctypes.byref(d)))
+    return c.value, m.value, d.value
 
 
 def check_compile(src):

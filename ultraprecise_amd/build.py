@@ -16,7 +16,7 @@ HIP_FLAGS = [
 ]
 
 
-SOURCES = ("updec_api.hip", "vm.hip", "codec.hip", "planner.cpp")
+SOURCES = ("updec_api.hip", "vm.hip", "codec.hip", "segsum.hip", "planner.cpp")
 
 
 def embed_device_header():

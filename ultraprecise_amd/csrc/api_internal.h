@@ -1,5 +1,5 @@
 // api_internal.h -- helpers shared by the C ABI translation units
-// (updec_api.hip, vm.hip).
+// (updec_api.hip, vm.hip, segsum.hip).
 #pragma once
 #include <string>
 #include <vector>

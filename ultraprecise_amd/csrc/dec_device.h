@@ -849,11 +849,12 @@ template <int WO, int WA, int WB, int FAITH> UPD_DEV Dec<WO> dmod(const Dec<WA>&
 // at row*B, sign in bit 7 of the last byte.  Loads are the aligned dwords
 // covering the row; the last rows of a column, whose covering dwords would
 // run past the column, take a byte path.
-template <int B> UPD_DEV Dec<(B + 3) / 4> load_compact(const u8* __restrict__ col, usize row, usize colbytes)
+// Decode the B-byte compact value at byte offset `off` of a 4-byte aligned
+// buffer of `colbytes` bytes (load_compact below: off = row * B).
+template <int B> UPD_DEV Dec<(B + 3) / 4> load_compact_off(const u8* __restrict__ col, usize off, usize colbytes)
 {
     constexpr int ND = (B + 6) / 4;
     constexpr int NW = (B + 3) / 4;
-    const usize off = row * (usize)B;
     const usize d0 = off >> 2;
     const u32 sh = (u32)(off & 3u);
     u32 dw[ND + 1];
@@ -888,6 +889,11 @@ template <int B> UPD_DEV Dec<(B + 3) / 4> load_compact(const u8* __restrict__ co
     r.neg = (r.m.w[NW - 1] >> SB) & 1u;
     r.m.w[NW - 1] &= ~(1u << SB);
     return r;
+}
+
+template <int B> UPD_DEV Dec<(B + 3) / 4> load_compact(const u8* __restrict__ col, usize row, usize colbytes)
+{
+    return load_compact_off<B>(col, row * (usize)B, colbytes);
 }
 
 // ---- wave-staged loads

@@ -1089,6 +1089,11 @@ bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode,
     int r = rec(*rw);
     if (r < 0) return false;
     p.root = r;
+    for (PNode& x : p.n)
+        if (x.kind == PK::Const) {
+            x.koff = p.kwords;
+            p.kwords += x.W + 1;
+        }
     p.outP = rw->P;
     p.outS = rw->S;
     p.outB = compact_bytes(p.outP);
@@ -1096,14 +1101,27 @@ bool build_plan(const ExprP& rw, const std::vector<ColumnType>& cols, Mode mode,
     return true;
 }
 
-std::string Plan::text() const
+std::vector<uint32_t> Plan::const_data() const
+{
+    std::vector<uint32_t> d((size_t)kwords, 0u);
+    for (const PNode& x : n)
+        if (x.kind == PK::Const) {
+            d[(size_t)x.koff] = x.k.sign ? 1u : 0u;
+            for (int w = 0; w < x.W; w++) d[(size_t)x.koff + 1 + (size_t)w] = x.k.v[w];
+        }
+    return d;
+}
+
+std::string Plan::text(bool values) const
 {
     std::function<std::string(int)> rec = [&](int id) -> std::string {
         const PNode& x = n[id];
         char buf[64];
         switch (x.kind) {
         case PK::Col: snprintf(buf, sizeof buf, "c%d:%d:%d", x.col, x.P, x.S); return buf;
-        case PK::Const: return std::string("k") + (x.k.sign ? "-" : "+") + x.k.hex() + ":" + std::to_string(x.S);
+        case PK::Const:
+            if (!values) return "k@" + std::to_string(x.koff) + ":" + std::to_string(x.S);
+            return std::string("k") + (x.k.sign ? "-" : "+") + x.k.hex() + ":" + std::to_string(x.S);
         case PK::Neg: return "(NEG " + rec(x.a) + ")";
         case PK::Cast: snprintf(buf, sizeof buf, "(CAST %d %d ", x.P, x.S); return buf + rec(x.a) + ")";
         case PK::Trunc: snprintf(buf, sizeof buf, "(TRUNC %d %d %d ", x.P, x.S, x.tp); return buf + rec(x.a) + ")";
@@ -1195,8 +1213,21 @@ namespace {
 struct Emit {
     std::ostringstream os;
     int tmp = 0;
+    // column-iterator inputs (IterArgs): column loads report NULL in a flag
+    // and every plan keeps its own NULL flag <prefix>nn
+    bool iter = false;
     std::string fresh(const char* p) { return std::string(p) + std::to_string(tmp++); }
 };
+
+// bit c set when column c appears in the subtree of node i
+static uint32_t subtree_cols(const Plan& p, int i)
+{
+    const PNode& x = p.n[i];
+    uint32_t m = x.kind == PK::Col ? (1u << x.col) : 0u;
+    if (x.a >= 0) m |= subtree_cols(p, x.a);
+    if (x.b >= 0) m |= subtree_cols(p, x.b);
+    return m;
+}
 
 // Emit straight-line code for one plan; returns the variable of the root.
 // Column loads are shared across plans through `loaded`.
@@ -1204,6 +1235,8 @@ std::string emit_plan(Emit& E, const Plan& p, const std::vector<ColumnType>& col
                       const std::string& ind, const std::string& prefix)
 {
     const bool faith = p.mode == Mode::Faithful;
+    const std::string nnv = prefix + "nn";
+    if (E.iter) E.os << ind << "u32 " << nnv << " = 1u;\n";
     std::vector<std::string> var(p.n.size());
     for (size_t i = 0; i < p.n.size(); i++) {
         const PNode& x = p.n[i];
@@ -1215,16 +1248,26 @@ std::string emit_plan(Emit& E, const Plan& p, const std::vector<ColumnType>& col
             int B = compact_bytes(cols[c].P);
             if (loaded[c].empty()) {
                 loaded[c] = "c" + std::to_string(c);
-                E.os << ind << "const Dec<" << (B + 3) / 4 << "> " << loaded[c] << " = col_load<" << B << ">(a, " << c
-                     << ", row);\n";
+                if (E.iter) {
+                    // NULL position, NULL row or position past the column: flag 0
+                    E.os << ind << "u32 n" << loaded[c] << " = 1u;\n";
+                    E.os << ind << "const Dec<" << (B + 3) / 4 << "> " << loaded[c] << " = col_load<" << B << ">(a, " << c
+                         << ", row, n" << loaded[c] << ");\n";
+                } else {
+                    E.os << ind << "const Dec<" << (B + 3) / 4 << "> " << loaded[c] << " = col_load<" << B << ">(a, " << c
+                         << ", row);\n";
+                }
             }
+            if (E.iter) E.os << ind << nnv << " &= n" << loaded[c] << ";\n";
             E.os << ind << "const Dec<" << x.W << "> " << v << " = dresize<" << x.W << ">(" << loaded[c] << ");\n";
             break;
         }
         case PK::Const: {
+            // sign and limbs from the launch's constant array (kernel data)
+            const int kb = p.kbase + x.koff;
             E.os << ind << "Dec<" << x.W << "> " << v << ";\n";
-            for (int w = 0; w < x.W; w++) E.os << ind << v << ".m.w[" << w << "] = 0x" << std::hex << x.k.v[w] << std::dec << "u;\n";
-            E.os << ind << v << ".neg = " << x.k.sign << "u;\n";
+            E.os << ind << v << ".neg = a.kc[" << kb << "];\n";
+            for (int w = 0; w < x.W; w++) E.os << ind << v << ".m.w[" << w << "] = a.kc[" << kb + 1 + w << "];\n";
             break;
         }
         case PK::Neg:
@@ -1293,6 +1336,17 @@ std::string emit_plan(Emit& E, const Plan& p, const std::vector<ColumnType>& col
         }
         }
     }
+    if (E.iter) {
+        // a division or MOD with a nullable operand is NULL when its divisor
+        // is zero (nullable_type operator/ and %, aries_types.hxx:626-663)
+        for (size_t i = 0; i < p.n.size(); i++) {
+            const PNode& x = p.n[i];
+            if (x.kind != PK::Div && x.kind != PK::Mod) continue;
+            const uint32_t m = subtree_cols(p, (int)i);
+            E.os << ind << "if ((a.nullcols & " << m << "u) && is_zero<" << p.n[x.b].W << ">(" << var[x.b] << ".m)) " << nnv
+                 << " = 0u;\n";
+        }
+    }
     return var[p.root];
 }
 
@@ -1340,9 +1394,11 @@ bool plan_out_of_line(const Plan& p) { return plan_ops(p) > kOutOfLineOps || pla
 void emit_plan_function(Emit& E, const Plan& p, const std::vector<ColumnType>& cols, const std::string& fname)
 {
     const int W = p.n[p.root].W;
-    E.os << "template <class A> __device__ __noinline__ Dec<" << W << "> " << fname << "(const A& a, usize row)\n{\n";
+    E.os << "template <class A> __device__ __noinline__ Dec<" << W << "> " << fname << "(const A& a, usize row"
+         << (E.iter ? ", u32& nn_out" : "") << ")\n{\n";
     std::vector<std::string> loaded(cols.size());
     std::string rv = emit_plan(E, p, cols, loaded, "  ", "");
+    if (E.iter) E.os << "  nn_out = nn;\n";
     E.os << "  return " << rv << ";\n}\n";
 }
 
@@ -1352,7 +1408,9 @@ std::string emit_plan_any(Emit& E, const Plan& p, const std::vector<ColumnType>&
 {
     if (!plan_out_of_line(p)) return emit_plan(E, p, cols, loaded, ind, prefix);
     const std::string v = prefix + "r";
-    E.os << ind << "const Dec<" << p.n[p.root].W << "> " << v << " = " << fname << "(a, row);\n";
+    if (E.iter) E.os << ind << "u32 " << prefix << "nn = 1u;\n";
+    E.os << ind << "const Dec<" << p.n[p.root].W << "> " << v << " = " << fname << "(a, row"
+         << (E.iter ? ", " + prefix + "nn" : std::string()) << ");\n";
     return v;
 }
 
@@ -1375,30 +1433,70 @@ const char* kArgsStruct =
     "  return (g & 7u) ? (usize)b : (usize)((b & 7u) * (g >> 3) + (b >> 3));\n"
     "}\n"
     "#define UPD_MAXCOLS 16\n"
-    "struct ColArgs { const u8* col[UPD_MAXCOLS]; usize colbytes[UPD_MAXCOLS]; usize rows; u8* out; };\n"
+    // kc: the plans' constants (sign, limbs per constant), the reference's
+    // constValues (AriesExprCalcNode.cpp:346-349)
+    "struct ColArgs { const u8* col[UPD_MAXCOLS]; usize colbytes[UPD_MAXCOLS]; usize rows; u8* out;\n"
+    "                 const u32* __restrict__ kc; };\n"
     "struct AggArgs { const u8* col[UPD_MAXCOLS]; usize colbytes[UPD_MAXCOLS]; usize rows;\n"
-    "                 const i32* fcol; const u8* key0; const u8* key1; const u8* lut; u32* partials; };\n"
-    // block-column inputs (AriesColumnDataIterator, AriesColumnDataIterator.cu:1-117):
-    // column c is nblocks[c] blocks of compact rows, block k starting at row
-    // starts[c][k]; rows of the launch are taken through idx (row i reads
-    // position idx[i], -1 = NULL) when idx is set
-    "struct IterArgs { const u8* const* blocks[UPD_MAXCOLS]; const i64* starts[UPD_MAXCOLS];\n"
-    "                  i32 nblocks[UPD_MAXCOLS]; usize colrows[UPD_MAXCOLS]; const i32* idx; usize rows;\n"
-    "                  usize rowlimit; u8* out; u8* valid; u32* partials; };\n"
+    "                 const i32* fcol; const u8* key0; const u8* key1; const u8* lut; u32* partials;\n"
+    "                 const u32* __restrict__ kc; };\n"
+    // column iterators (AriesColumnDataIterator, AriesColumnDataIterator.hxx:8-41,
+    // operator[] in AriesColumnDataIterator.cu:1-117): column c is nblocks
+    // data blocks starting at positions starts[k] (positions in all), rows of
+    // `stride` bytes (a flag byte first when flags bit 0 marks nullable data);
+    // with an index (flags bits 1-2: 1 int32 with -1 = NULL, 2 nullable_type<int32>
+    // of 5 packed bytes) launch row i reads the position the index holds at i,
+    // from one array (idx) or from index blocks starting at rows istarts[k]
+    "struct ColIter { const u8* const* blocks; const i64* starts; usize positions; i32 nblocks; u32 stride;\n"
+    "                 const void* idx; const void* const* iblocks; const i64* istarts; i32 niblocks; u32 flags; };\n"
+    "struct IterArgs { ColIter it[UPD_MAXCOLS]; usize rows; const i32* fcol; const u8* key0; const u8* key1;\n"
+    "                  const u8* lut; u32* partials; u8* out; u8* valid; const u32* __restrict__ kc;\n"
+    "                  u32 outnull; u32 nullcols; u32 skipnull; u32 pad; };\n"
     "template <int B> __device__ __forceinline__ Dec<(B + 3) / 4> col_load(const ColArgs& a, int c, usize row)\n"
     "{ return load_compact<B>(a.col[c], row, a.colbytes[c]); }\n"
     "template <int B> __device__ __forceinline__ Dec<(B + 3) / 4> col_load(const AggArgs& a, int c, usize row)\n"
     "{ return load_compact<B>(a.col[c], row, a.colbytes[c]); }\n"
-    // row < rowlimit is checked by the caller; the block is the last one
-    // starting at or before the row (the iterator's bounds_upper search)
-    "template <int B> __device__ __forceinline__ Dec<(B + 3) / 4> col_load(const IterArgs& a, int c, usize row)\n"
+    // the value of column c at launch row i; nn = 0 (and a zero value) for a
+    // NULL index entry, a position outside the column or a NULL row
+    "template <int B> __device__ __forceinline__ Dec<(B + 3) / 4> col_load(const IterArgs& a, int c, usize i, u32& nn)\n"
     "{\n"
-    "  const i64* st = a.starts[c];\n"
-    "  int lo = 0, hi = a.nblocks[c];\n"
-    "  while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if ((usize)st[mid] <= row) lo = mid; else hi = mid; }\n"
-    "  const usize first = (usize)st[lo];\n"
-    "  const usize end = lo + 1 < a.nblocks[c] ? (usize)st[lo + 1] : a.colrows[c];\n"
-    "  return load_compact<B>(a.blocks[c][lo], row - first, (end - first) * (usize)B);\n"
+    "  const ColIter& t = a.it[c];\n"
+    "  Dec<(B + 3) / 4> z;\n"
+    "  z.m = mzero<(B + 3) / 4>();\n"
+    "  z.neg = 0u;\n"
+    "  i64 pos = (i64)i;\n"
+    "  const u32 kind = (t.flags >> 1) & 3u;\n"
+    "  if (kind) {\n"
+    "    const u8* ib = reinterpret_cast<const u8*>(t.idx);\n"
+    "    usize off = i;\n"
+    "    if (t.niblocks > 1) {\n"
+    "      int lo = 0, hi = t.niblocks;\n"
+    "      while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if ((usize)t.istarts[mid] <= i) lo = mid; else hi = mid; }\n"
+    "      ib = reinterpret_cast<const u8*>(t.iblocks[lo]);\n"
+    "      off = i - (usize)t.istarts[lo];\n"
+    "    }\n"
+    "    if (kind == 1u) {\n"
+    "      pos = (i64)reinterpret_cast<const i32*>(ib)[off];\n"
+    "    } else {\n"
+    "      const u8* e = ib + off * 5u;\n"
+    "      pos = e[0] ? (i64)(i32)((u32)e[1] | ((u32)e[2] << 8) | ((u32)e[3] << 16) | ((u32)e[4] << 24)) : (i64)-1;\n"
+    "    }\n"
+    "  }\n"
+    "  if (pos < 0 || (usize)pos >= t.positions) { nn = 0u; return z; }\n"
+    "  int lo = 0, hi = t.nblocks;\n"
+    "  while (hi - lo > 1) { const int mid = (lo + hi) >> 1; if ((usize)t.starts[mid] <= (usize)pos) lo = mid; else hi = mid; }\n"
+    "  const usize first = (usize)t.starts[lo];\n"
+    "  const usize end = lo + 1 < t.nblocks ? (usize)t.starts[lo + 1] : t.positions;\n"
+    "  const usize base = (usize)t.blocks[lo];\n"
+    "  const u8* al = reinterpret_cast<const u8*>(base & ~(usize)3);\n"
+    "  const usize sh = base & 3u;\n"
+    "  const usize off = sh + ((usize)pos - first) * t.stride;\n"
+    "  const usize lim = sh + (end - first) * t.stride;\n"
+    "  if (t.flags & 1u) {\n"
+    "    if (!al[off]) { nn = 0u; return z; }\n"
+    "    return load_compact_off<B>(al, off + 1u, lim);\n"
+    "  }\n"
+    "  return load_compact_off<B>(al, off, lim);\n"
     "}\n";
 
 // Columns of narrow rows (B <= 32) are read through a per-wave LDS stage
@@ -1500,12 +1598,15 @@ static std::string emit_row_loop_open(Emit& E, const AggSpec& s, const std::vect
                                       bool blocks = false)
 {
     if (blocks) {
-        // block-column inputs: row i reads position idx[i] (or i); NULL
-        // positions (-1) and positions past the columns add nothing
-        E.os << "  for (usize i_ = lblk * T + tid; i_ < a.rows; i_ += (usize)gridDim.x * T) {\n";
-        E.os << "    const i64 pos_ = a.idx ? (i64)a.idx[i_] : (i64)i_;\n";
-        E.os << "    if (pos_ < 0 || (usize)pos_ >= a.rowlimit) continue;\n";
-        E.os << "    const usize row = (usize)pos_;\n";
+        // column iterators: the plans load their columns at launch row
+        // `row` through the iterators (NULL flags per plan); the filter and
+        // key columns are given per launch row
+        E.os << "  for (usize row = lblk * T + tid; row < a.rows; row += (usize)gridDim.x * T) {\n";
+        if (s.filter_col >= 0) E.os << "    const i32 fv = a.fcol[row];\n";
+        if (keys) {
+            if (s.nkeys == 2) E.os << "    const u32 key = ((u32)a.key0[row] << 8) | a.key1[row];\n";
+            else E.os << "    const u32 key = a.key0[row];\n";
+        }
         return "    ";
     }
     if (rps == 1) {
@@ -1587,89 +1688,94 @@ static bool calc_one_wave(const Plan& p) { return plan_peak_limbs(p) >= 48; }
 std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, const std::string& name, const GenOptions& o)
 {
     Emit E;
+    E.iter = o.blocks;
     const int T = o.block;
     const int OB = p.outB;
     E.os << "#include \"dec_device.h\"\nusing namespace updec;\n" << kArgsStruct;
-    E.os << "// plan: " << p.text() << "\n";
+    E.os << "// plan: " << p.text(false) << "\n";
+    const bool one_row = calc_one_row(p);
+    std::vector<std::string> loaded(cols.size());
+    // emit_plan_any calls the out-of-line evaluator; one_row inlines instead
+    auto body = [&](const std::string& ind) {
+        return one_row ? emit_plan(E, p, cols, loaded, ind, "") : emit_plan_any(E, p, cols, loaded, ind, "", plan_fn_name(0));
+    };
     if (OB == 0) {
         // a result of precision 0 has no bytes (B(0) = 0, decimal.cu:92):
-        // nothing is evaluated or stored; the block form still reports
-        // which rows are valid
+        // nothing is stored; the iterator form still reports which rows
+        // are NULL (and writes the flag byte of a nullable output)
         E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") " << name << "("
              << (o.blocks ? "IterArgs" : "ColArgs") << " a)\n{\n";
         if (o.blocks) {
             E.os << "  for (usize row = (usize)blockIdx.x * " << T << " + threadIdx.x; row < a.rows; row += (usize)gridDim.x * "
                  << T << ") {\n";
-            E.os << "    const i64 pos_ = a.idx ? (i64)a.idx[row] : (i64)row;\n";
-            E.os << "    if (a.valid) a.valid[row] = (pos_ >= 0 && (usize)pos_ < a.rowlimit) ? 1u : 0u;\n  }\n";
+            emit_plan(E, p, cols, loaded, "    ", "");
+            E.os << "    if (a.valid) a.valid[row] = (u8)nn;\n";
+            E.os << "    if (a.outnull) a.out[row] = (u8)nn;\n  }\n";
         }
         E.os << "}\n";
         return E.os.str();
     }
-    if (!calc_one_row(p)) emit_plan_functions(E, {p}, cols);
+    if (!one_row) emit_plan_functions(E, {p}, cols);
     // Rows whose image is 4, 8 or a multiple of 16 bytes go straight to HBM
     // as dword / dwordx2 / dwordx4 stores (coalesced: lane i writes bytes
-    // [i*OB, (i+1)*OB) of the block's span); other widths are assembled in an
-    // LDS tile and copied out in dwords.
-    const int VB = OB % 16 == 0 ? 16 : (OB == 8 || OB == 4) ? OB : 0;
+    // [i*OB, (i+1)*OB) of the block's span); other widths, and every row of
+    // the iterator form (whose rows may carry a flag byte), are assembled in
+    // an LDS tile and copied out in dwords.
+    const int VB = o.blocks ? 0 : OB % 16 == 0 ? 16 : (OB == 8 || OB == 4) ? OB : 0;
+    // output row stride: OB, or OB + 1 for the iterator form's nullable
+    // output (flag byte, then the compact row: GenSaveDataCode,
+    // AriesExprCalcNode.cpp:76-108)
+    const std::string OS = o.blocks ? "os_" : std::to_string(OB);
     E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") "
          << (calc_one_wave(p) ? "__attribute__((amdgpu_waves_per_eu(1, 1))) " : "") << name << "("
          << (o.blocks ? "IterArgs" : "ColArgs") << " a)\n{\n";
-    E.os << "  __shared__ u32 tile[(" << T << " * " << OB << " + 3) / 4];\n";
+    E.os << "  __shared__ u32 tile[(" << T << " * " << (o.blocks ? OB + 1 : OB) << " + 3) / 4];\n";
     E.os << "  u8* t8 = reinterpret_cast<u8*>(tile);\n";
+    if (o.blocks) E.os << "  const usize os_ = " << OB << "u + (a.outnull ? 1u : 0u);\n";
     if (VB) E.os << "  const bool direct = ((usize)a.out & " << VB - 1 << "u) == 0;\n";
     else E.os << "  const bool direct = false;\n";
-    const bool one_row = calc_one_row(p);
     if (one_row)   // one block per T rows; `continue` leaves the do-while
         E.os << "  do {\n    const usize base = (usize)blockIdx.x * " << T << ";\n    if (base >= a.rows) break;\n";
     else
         E.os << "  for (usize base = xcd_block() * " << T << "; base < a.rows; base += (usize)gridDim.x * " << T << ") {\n";
     E.os << "    const usize row = base + threadIdx.x;\n";
     E.os << "    if (row < a.rows) {\n";
-    std::vector<std::string> loaded(cols.size());
-    // emit_plan_any calls the out-of-line evaluator; one_row inlines instead
-    auto body = [&](const std::string& ind) {
-        return one_row ? emit_plan(E, p, cols, loaded, ind, "") : emit_plan_any(E, p, cols, loaded, ind, "", plan_fn_name(0));
-    };
     if (o.blocks) {
-        // output row `row` evaluates input position idx[row] (or row); a
-        // NULL position (-1), or one past the columns, gives a zero row
-        // with valid = 0
-        E.os << "      const i64 pos_ = a.idx ? (i64)a.idx[row] : (i64)row;\n";
-        E.os << "      const bool ok_ = pos_ >= 0 && (usize)pos_ < a.rowlimit;\n";
-        E.os << "      if (a.valid) a.valid[row] = ok_ ? 1u : 0u;\n";
-        E.os << "      Mag<" << (OB + 3) / 4 << "> enc = mzero<" << (OB + 3) / 4 << ">();\n";
-        E.os << "      if (ok_) {\n";
-        E.os << "        const usize row = (usize)pos_;\n";
-        std::string rv = body("        ");
-        E.os << "        enc = encode_compact<" << OB << ", " << p.n[p.root].W << ">(" << rv << ");\n";
-        E.os << "      }\n";
+        // launch row `row` reads every column through its iterator; a NULL
+        // value gives a zero row, flag 0 and valid 0
+        std::string rv = body("      ");
+        E.os << "      const Mag<" << (OB + 3) / 4 << "> enc = nn ? encode_compact<" << OB << ", " << p.n[p.root].W << ">(" << rv
+             << ") : mzero<" << (OB + 3) / 4 << ">();\n";
+        E.os << "      if (a.valid) a.valid[row] = (u8)nn;\n";
+        E.os << "      u8* tr = t8 + threadIdx.x * os_;\n";
+        E.os << "      if (a.outnull) *tr++ = (u8)nn;\n";
+        E.os << "#pragma unroll\n      for (int b = 0; b < " << OB << "; b++) tr[b] = (u8)(enc.w[b >> 2] >> (8 * (b & 3)));\n";
     } else {
         std::string rv = body("      ");
         E.os << "      const Mag<" << (OB + 3) / 4 << "> enc = encode_compact<" << OB << ", " << p.n[p.root].W << ">(" << rv
              << ");\n";
-    }
-    if (VB) {
-        E.os << "      if (direct) {\n";
-        if (VB == 4) {
-            E.os << "        reinterpret_cast<u32*>(a.out)[row] = enc.w[0];\n";
-        } else if (VB == 8) {
-            E.os << "        u32x2 v; v[0] = enc.w[0]; v[1] = enc.w[1];\n";
-            E.os << "        reinterpret_cast<u32x2*>(a.out)[row] = v;\n";
-        } else {
-            E.os << "        u32x4* d4 = reinterpret_cast<u32x4*>(a.out + row * " << OB << ");\n";
-            for (int q = 0; q < OB / 16; q++)
-                E.os << "        { u32x4 v; v[0] = enc.w[" << 4 * q << "]; v[1] = enc.w[" << 4 * q + 1 << "]; v[2] = enc.w["
-                     << 4 * q + 2 << "]; v[3] = enc.w[" << 4 * q + 3 << "]; d4[" << q << "] = v; }\n";
+        if (VB) {
+            E.os << "      if (direct) {\n";
+            if (VB == 4) {
+                E.os << "        reinterpret_cast<u32*>(a.out)[row] = enc.w[0];\n";
+            } else if (VB == 8) {
+                E.os << "        u32x2 v; v[0] = enc.w[0]; v[1] = enc.w[1];\n";
+                E.os << "        reinterpret_cast<u32x2*>(a.out)[row] = v;\n";
+            } else {
+                E.os << "        u32x4* d4 = reinterpret_cast<u32x4*>(a.out + row * " << OB << ");\n";
+                for (int q = 0; q < OB / 16; q++)
+                    E.os << "        { u32x4 v; v[0] = enc.w[" << 4 * q << "]; v[1] = enc.w[" << 4 * q + 1 << "]; v[2] = enc.w["
+                         << 4 * q + 2 << "]; v[3] = enc.w[" << 4 * q + 3 << "]; d4[" << q << "] = v; }\n";
+            }
+            E.os << "      } else {\n";
         }
-        E.os << "      } else {\n";
+        E.os << "#pragma unroll\n      for (int b = 0; b < " << OB << "; b++) t8[threadIdx.x * " << OB
+             << " + b] = (u8)(enc.w[b >> 2] >> (8 * (b & 3)));\n";
+        if (VB) E.os << "      }\n";
     }
-    E.os << "#pragma unroll\n      for (int b = 0; b < " << OB << "; b++) t8[threadIdx.x * " << OB
-         << " + b] = (u8)(enc.w[b >> 2] >> (8 * (b & 3)));\n";
-    if (VB) E.os << "      }\n";
     E.os << "    }\n    if (direct) continue;\n    __syncthreads();\n";
-    E.os << "    const usize n = (a.rows - base < " << T << " ? a.rows - base : " << T << ") * " << OB << ";\n";
-    E.os << "    u8* dst = a.out + base * " << OB << ";\n";
+    E.os << "    const usize n = (a.rows - base < " << T << " ? a.rows - base : " << T << ") * " << OS << ";\n";
+    E.os << "    u8* dst = a.out + base * " << OS << ";\n";
     E.os << "    for (usize i = threadIdx.x; i < n / 4; i += " << T << ") reinterpret_cast<u32*>(dst)[i] = tile[i];\n";
     E.os << "    for (usize i = (n / 4) * 4 + threadIdx.x; i < n; i += " << T << ") dst[i] = t8[i];\n";
     E.os << "    __syncthreads();\n  }" << (one_row ? " while (0);" : "") << "\n}\n";
@@ -1741,7 +1847,7 @@ static void emit_agg_prelude(Emit& E, const AggSpec& s, const AccCfg& c, int nsl
 {
     const int NP = (int)s.plans.size();
     E.os << "#include \"dec_device.h\"\nusing namespace updec;\n" << kArgsStruct;
-    for (int i = 0; i < NP; i++) E.os << "// plan " << i << ": " << s.plans[i].text() << "\n";
+    for (int i = 0; i < NP; i++) E.os << "// plan " << i << ": " << s.plans[i].text(false) << "\n";
     E.os << "#define NP " << NP << "\n#define AW2 " << agg_acc_words(s) << "\n#define T " << c.T << "\n#define SPLIT "
          << c.SPLIT << "\n#define NWAVE (T / 64)\n#define NCOPY (NWAVE * SPLIT)\n#define NS " << nslots
          << "\n#define AWR " << c.AWR << "\n";
@@ -1810,9 +1916,21 @@ static void emit_agg_row(Emit& E, const AggSpec& s, const std::vector<ColumnType
         const Plan& p = s.plans[i];
         const int W = p.n[p.root].W;
         std::string rv = emit_plan_any(E, p, cols, loaded, ind, "p" + std::to_string(i) + "_", plan_fn_name(i));
-        E.os << ind << "slot_add<" << W << ">(&aslot[sbase + " << i << " * AWR], stored<" << p.outB << ", " << W << ">(" << rv
-             << "));\n";
+        // the iterator form skips a NULL value (agg_sum_t of nullable_type,
+        // operators.hxx:1175-1195)
+        E.os << ind << (E.iter ? "if (p" + std::to_string(i) + "_nn) " : std::string()) << "slot_add<" << W
+             << ">(&aslot[sbase + " << i << " * AWR], stored<" << p.outB << ", " << W << ">(" << rv << "));\n";
     }
+}
+
+// the iterator form's row count: every row (COUNT(*)), or with skipnull
+// (the single shared index of updec_agg_run_blocks) only rows whose values
+// are not NULL
+static std::string iter_count_expr(const AggSpec& s)
+{
+    std::string all;
+    for (int i = 0; i < (int)s.plans.size(); i++) all += (i ? " & p" : "p") + std::to_string(i) + "_nn";
+    return "(a.skipnull ? (" + all + ") : 1u)";
 }
 
 static void emit_agg_final_prec(Emit& E, const AggSpec& s)
@@ -2056,6 +2174,7 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
 {
     if (s.onepass) return gen_agg_onepass(s, cols, name, o, acc_words, block_out);
     Emit E;
+    E.iter = o.blocks;
     const int NP = (int)s.plans.size();
     const int NG = s.ngroups;
     const int AW2 = agg_acc_words(s);
@@ -2078,7 +2197,6 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
     E.os << "extern \"C\" __global__ void __launch_bounds__(" << c.T << ") " << name << "_main("
          << (o.blocks ? "IterArgs" : "AggArgs") << " a)\n{\n";
     E.os << "  const u32 tid = threadIdx.x;\n  const u32 wave = tid >> 6, lane = tid & 63u;\n";
-    if (o.blocks && (!reg || s.filter_col >= 0 || s.nkeys > 0)) return std::string();
     const StagePlan sp = (reg && !o.blocks && agg_rows_per_step(s, cols, o) == 1) ? stage_plan(s, cols) : StagePlan();
     emit_stage_decl(E, sp, c.T / 64);
     if (!reg) {
@@ -2101,14 +2219,15 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
             const Plan& p = s.plans[i];
             const int W = p.n[p.root].W;
             std::string rv = emit_plan_any(E, p, cols, loaded, bi, "p" + std::to_string(i) + "_", plan_fn_name(i));
-            E.os << bi << "acc_add<AW1, " << W << ">(acc" << i << ", stored<" << p.outB << ", " << W << ">(" << rv << "));\n";
+            E.os << bi << (o.blocks ? "if (p" + std::to_string(i) + "_nn) " : std::string()) << "acc_add<AW1, " << W
+                 << ">(acc" << i << ", stored<" << p.outB << ", " << W << ">(" << rv << "));\n";
         }
-        E.os << bi << "cnt += 1u;\n";
+        E.os << bi << "cnt += " << (o.blocks ? iter_count_expr(s) : std::string("1u")) << ";\n";
     } else {
         E.os << bi << "const u32 g = a.lut[key];\n";
         E.os << bi << "const u32 sbase = (copy * NS + g) * NP * AWR;\n";
         emit_agg_row(E, s, cols, loaded, bi);
-        E.os << bi << "atomicAdd(&acnt[copy * NS + g], 1u);\n";
+        E.os << bi << "atomicAdd(&acnt[copy * NS + g], " << (o.blocks ? iter_count_expr(s) : std::string("1u")) << ");\n";
     }
     emit_row_loop_close(E, rps);
     E.os << "  __shared__ u32 red[NWAVE * PW];\n";

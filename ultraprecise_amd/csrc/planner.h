@@ -97,13 +97,24 @@ struct PNode {
     int bits = 0;            // magnitude bound: value < 2^bits (faithful mode: <= 1024, values wrap)
     int ubits = 0;           // the same bound without the 32-limb wrap
     int W = 1;               // limbs
+    int koff = -1;           // Const: word offset of {sign, W limbs} in the plan's constant block
 };
 struct Plan {
     std::vector<PNode> n;
     int root = -1;
     Mode mode = Mode::Faithful;
     int outP = 0, outS = 0, outB = 0;   // materialized result type and compact bytes
-    std::string text() const;           // canonical plan text (see oracle's od_plan_parse)
+    // Constants are kernel data, as the reference's constValues
+    // (AriesExprCalcNode.cpp:346-349): the generated source reads constant
+    // k's sign and limbs from the launch's constant array at kbase + koff,
+    // so plans that differ only in constant values of the same widths share
+    // one source text (one hipRTC compile).
+    int kwords = 0;                     // words of this plan's constant block
+    int kbase = 0;                      // offset of the block in the launch's constant array
+    std::vector<uint32_t> const_data() const;   // the block: per constant, sign then W limbs
+    // canonical plan text (see oracle's od_plan_parse); with values = false
+    // constants print as their slot and scale ("k@3:2"), the source's shape
+    std::string text(bool values = true) const;
 };
 
 int compact_bytes(int P);   // __PREC_BYTE_ARRAY (decimal.cu:92), closed form past 299

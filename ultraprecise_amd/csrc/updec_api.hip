@@ -3,6 +3,8 @@
 // statistics, group discovery, AVG division).
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <atomic>
@@ -73,8 +75,93 @@ struct Module {
 };
 using ModuleP = std::shared_ptr<Module>;
 
+uint64_t fnv1a(const void* p, size_t n, uint64_t h = 1469598103934665603ull)
+{
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+// On-disk store of compiled code objects, so a kernel compiled by one
+// process is loaded by the next without hipRTC: one file per source, named
+// by a hash of the source and of the device library text, holding both so a
+// hash collision or a changed library is detected and recompiled.
+// Directory: $UPDEC_KERNEL_CACHE ("off" disables), else
+// $XDG_CACHE_HOME/updec, else $HOME/.cache/updec.  Failures to read or
+// write it only cost a compile.
+class DiskCache {
+  public:
+    DiskCache()
+    {
+        const char* env = getenv("UPDEC_KERNEL_CACHE");
+        if (env && *env) {
+            if (strcmp(env, "off") != 0) dir_ = env;
+        } else if (const char* x = getenv("XDG_CACHE_HOME"); x && *x) {
+            dir_ = std::string(x) + "/updec";
+        } else if (const char* h = getenv("HOME"); h && *h) {
+            dir_ = std::string(h) + "/.cache/updec";
+        }
+        lib_hash_ = fnv1a(kDecDeviceSource, sizeof(kDecDeviceSource) - 1);
+    }
+    bool load(const std::string& src, std::vector<char>* code)
+    {
+        if (dir_.empty()) return false;
+        FILE* f = fopen(path(src).c_str(), "rb");
+        if (!f) return false;
+        bool ok = false;
+        uint64_t hdr[3];
+        if (fread(hdr, sizeof hdr, 1, f) == 1 && hdr[0] == kMagic && hdr[1] == lib_hash_ && hdr[2] == src.size()) {
+            std::string s(src.size(), '\0');
+            if (fread(&s[0], 1, s.size(), f) == s.size() && s == src) {
+                long pos = ftell(f);
+                fseek(f, 0, SEEK_END);
+                long end = ftell(f);
+                if (pos > 0 && end > pos) {
+                    code->resize((size_t)(end - pos));
+                    fseek(f, pos, SEEK_SET);
+                    ok = fread(code->data(), 1, code->size(), f) == code->size();
+                }
+            }
+        }
+        fclose(f);
+        return ok;
+    }
+    void store(const std::string& src, const std::vector<char>& code)
+    {
+        if (dir_.empty()) return;
+        for (size_t i = 0; i <= dir_.size(); i++) {   // mkdir -p
+            if (i == dir_.size() || (dir_[i] == '/' && i > 0)) (void)mkdir(dir_.substr(0, i).c_str(), 0755);
+        }
+        const std::string fin = path(src);
+        const std::string tmp = fin + ".tmp" + std::to_string((unsigned long)getpid());
+        FILE* f = fopen(tmp.c_str(), "wb");
+        if (!f) return;
+        const uint64_t hdr[3] = {kMagic, lib_hash_, (uint64_t)src.size()};
+        bool ok = fwrite(hdr, sizeof hdr, 1, f) == 1 && fwrite(src.data(), 1, src.size(), f) == src.size() &&
+                  fwrite(code.data(), 1, code.size(), f) == code.size();
+        ok = (fclose(f) == 0) && ok;
+        if (!ok || rename(tmp.c_str(), fin.c_str()) != 0) (void)remove(tmp.c_str());
+    }
+
+  private:
+    static constexpr uint64_t kMagic = 0x314b504455ull;   // "UPDK1"
+    std::string path(const std::string& src) const
+    {
+        char name[48];
+        snprintf(name, sizeof name, "/%016llx.hsaco",
+                 (unsigned long long)fnv1a(src.data(), src.size(), lib_hash_ ^ 0x9e3779b97f4a7c15ull));
+        return dir_ + name;
+    }
+    std::string dir_;
+    uint64_t lib_hash_ = 0;
+};
+
+std::atomic<int64_t> g_rtc_compiles{0}, g_mem_hits{0}, g_disk_hits{0};
+
 // LRU of compiled modules keyed by the full source text, as the
-// reference's dynamic kernel manager (DynamicKernel.cu:10-58, LRU_COUNT 100)
+// reference's dynamic kernel manager (DynamicKernel.cu:10-58, LRU_COUNT 100).
+// Generated sources carry no constant values (they are kernel data), so
+// queries that differ only in their literals share a module.
 class ModuleCache {
   public:
     static ModuleCache& get()
@@ -89,11 +176,18 @@ class ModuleCache {
             auto it = map_.find(src);
             if (it != map_.end()) {
                 lru_.splice(lru_.begin(), lru_, it->second.second);
+                g_mem_hits++;
                 return it->second.first;
             }
         }
         std::vector<char> code;
-        if (!rtc(src, &code, err)) return nullptr;
+        if (disk_.load(src, &code)) {
+            g_disk_hits++;
+        } else {
+            if (!rtc(src, &code, err)) return nullptr;
+            g_rtc_compiles++;
+            disk_.store(src, code);
+        }
         auto m = std::make_shared<Module>();
         hipError_t e = hipModuleLoadData(&m->mod, code.data());
         if (e != hipSuccess) {
@@ -142,6 +236,7 @@ class ModuleCache {
 
   private:
     static constexpr size_t kCap = 100;
+    DiskCache disk_;
     std::mutex mu_;
     std::list<std::string> lru_;
     std::unordered_map<std::string, std::pair<ModuleP, std::list<std::string>::iterator>> map_;
@@ -215,6 +310,7 @@ struct ColArgs {
     usize colbytes[UPD_MAXCOLS];
     usize rows;
     u8* out;
+    const u32* kc;  // the plans' constants (kernel data)
 };
 struct AggArgs {
     const u8* col[UPD_MAXCOLS];
@@ -225,25 +321,47 @@ struct AggArgs {
     const u8* key1;
     const u8* lut;
     u32* partials;
+    const u32* kc;
 };
-// block-column kernel arguments (the generated kernels' IterArgs)
+// column-iterator kernel arguments (the generated kernels' ColIter and
+// IterArgs)
+struct ColIter {
+    const u8* const* blocks;
+    const i64* starts;
+    usize positions;
+    i32 nblocks;
+    u32 stride;  // bytes per row: B, or B + 1 for nullable rows
+    const void* idx;
+    const void* const* iblocks;
+    const i64* istarts;
+    i32 niblocks;
+    u32 flags;  // bit 0 nullable rows; bits 1-2 index kind (0 none, 1 int32, 2 nullable int32)
+};
+static_assert(sizeof(ColIter) == 64, "ColIter layout");
 struct IterArgs {
-    const u8* const* blocks[UPD_MAXCOLS];
-    const i64* starts[UPD_MAXCOLS];
-    i32 nblocks[UPD_MAXCOLS];
-    usize colrows[UPD_MAXCOLS];
-    const i32* idx;
+    ColIter it[UPD_MAXCOLS];
     usize rows;
-    usize rowlimit;
+    const i32* fcol;
+    const u8* key0;
+    const u8* key1;
+    const u8* lut;
+    u32* partials;
     u8* out;
     u8* valid;
-    u32* partials;
+    const u32* kc;
+    u32 outnull;
+    u32 nullcols;  // bit c: column c can be NULL (divisions by zero under it give NULL)
+    u32 skipnull;  // count only rows whose values are not NULL
+    u32 pad;
 };
-static_assert(sizeof(IterArgs) == 496, "IterArgs layout");
+static_assert(sizeof(IterArgs) == 1112, "IterArgs layout");
 
-// validate block-column descriptors and fill the kernel arguments
-static int fill_iter(IterArgs* a, const std::vector<ColumnType>& types, const updec_column_blocks* cols,
-                     const int32_t* d_indices, int64_t rows)
+// validate block-column descriptors (one shared int32 index) and fill the
+// iterator arguments: every column gets the shared index and, so that a
+// position is NULL in all columns or in none, the smallest column's row
+// count; no division-by-zero NULLs (nullcols = 0), as this form defines
+static int fill_blocks(IterArgs* a, const std::vector<ColumnType>& types, const updec_column_blocks* cols,
+                       const int32_t* d_indices, int64_t rows)
 {
     *a = IterArgs{};
     usize limit = ~(usize)0;
@@ -251,18 +369,60 @@ static int fill_iter(IterArgs* a, const std::vector<ColumnType>& types, const up
         const updec_column_blocks& c = cols[i];
         if (c.nblocks < 1 || !c.d_blocks || !c.d_starts || c.rows < 0)
             return api_fail(UPDEC_EINVAL, "column " + std::to_string(i) + ": no blocks");
-        a->blocks[i] = c.d_blocks;
-        a->starts[i] = reinterpret_cast<const i64*>(c.d_starts);
-        a->nblocks[i] = c.nblocks;
-        a->colrows[i] = (usize)c.rows;
         limit = std::min(limit, (usize)c.rows);
     }
     if (types.empty()) limit = 0;
     // without indices row i reads position i: every column must hold the rows
     if (!d_indices && (usize)rows > limit) return api_fail(UPDEC_EINVAL, "rows exceed the columns' rows");
-    a->idx = d_indices;
+    for (size_t i = 0; i < types.size(); i++) {
+        ColIter& t = a->it[i];
+        t.blocks = cols[i].d_blocks;
+        t.starts = reinterpret_cast<const i64*>(cols[i].d_starts);
+        t.nblocks = cols[i].nblocks;
+        t.positions = limit;
+        t.stride = (u32)compact_bytes(types[i].P);
+        t.idx = d_indices;
+        t.niblocks = d_indices ? 1 : 0;
+        t.flags = d_indices ? (UPDEC_INDEX_INT32 << 1) : 0u;
+    }
     a->rows = (usize)rows;
-    a->rowlimit = limit;
+    a->skipnull = 1;
+    return UPDEC_OK;
+}
+
+// validate column iterators (updec_column_iter) and fill the arguments
+static int fill_iters(IterArgs* a, const std::vector<ColumnType>& types, const updec_column_iter* cols, int64_t rows)
+{
+    *a = IterArgs{};
+    if (!types.empty() && !cols) return api_fail(UPDEC_EINVAL, "null column iterator array");
+    for (size_t i = 0; i < types.size(); i++) {
+        const updec_column_iter& c = cols[i];
+        const std::string col = "column " + std::to_string(i) + ": ";
+        if (c.nblocks < 1 || !c.d_blocks || !c.d_starts || c.positions < 0) return api_fail(UPDEC_EINVAL, col + "no blocks");
+        if (c.index_kind < UPDEC_INDEX_NONE || c.index_kind > UPDEC_INDEX_NULLABLE_INT32)
+            return api_fail(UPDEC_EINVAL, col + "unknown index kind");
+        if (c.index_kind == UPDEC_INDEX_NONE && rows > c.positions)
+            return api_fail(UPDEC_EINVAL, col + "rows exceed the positions of a column without index");
+        if (c.index_kind != UPDEC_INDEX_NONE) {
+            if (c.nindex_blocks < 1) return api_fail(UPDEC_EINVAL, col + "index without blocks");
+            if (c.nindex_blocks == 1 && !c.d_index) return api_fail(UPDEC_EINVAL, col + "null index");
+            if (c.nindex_blocks > 1 && (!c.d_index_blocks || !c.d_index_starts))
+                return api_fail(UPDEC_EINVAL, col + "null index block arrays");
+        }
+        ColIter& t = a->it[i];
+        t.blocks = c.d_blocks;
+        t.starts = reinterpret_cast<const i64*>(c.d_starts);
+        t.nblocks = c.nblocks;
+        t.positions = (usize)c.positions;
+        t.stride = (u32)compact_bytes(types[i].P) + (c.nullable ? 1u : 0u);
+        t.idx = c.d_index;
+        t.iblocks = c.d_index_blocks;
+        t.istarts = reinterpret_cast<const i64*>(c.d_index_starts);
+        t.niblocks = c.index_kind == UPDEC_INDEX_NONE ? 0 : c.nindex_blocks;
+        t.flags = (c.nullable ? 1u : 0u) | ((u32)c.index_kind << 1);
+        if (c.nullable || c.index_kind != UPDEC_INDEX_NONE) a->nullcols |= 1u << i;
+    }
+    a->rows = (usize)rows;
     return UPDEC_OK;
 }
 
@@ -366,9 +526,29 @@ __global__ void __launch_bounds__(64) k_div_int64(const u8* sums, int64_t stride
 
 // ------------------------------------------------------------------ objects
 
+// The constants of an expression or aggregation, uploaded to the device at
+// the first launch and passed to every launch (the reference's constValues,
+// AriesExprCalcNode.cpp:346-349).
+struct ConstBlock {
+    std::vector<u32> host;
+    u32* dev = nullptr;
+    int upload(hipStream_t st)
+    {
+        if (dev || host.empty()) return UPDEC_OK;
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&dev), host.size() * sizeof(u32)));
+        HIPCHK(hipMemcpyAsync(dev, host.data(), host.size() * sizeof(u32), hipMemcpyHostToDevice, st));
+        return UPDEC_OK;
+    }
+    ~ConstBlock()
+    {
+        if (dev) (void)hipFree(dev);
+    }
+};
+
 struct updec_expr_s {
     Plan plan;
     std::vector<ColumnType> types;
+    ConstBlock consts;
     std::string name, src;
     ModuleP mod;
     hipFunction_t fn = nullptr;
@@ -380,8 +560,7 @@ struct updec_expr_s {
 struct updec_agg_s {
     AggSpec spec;
     std::vector<ColumnType> types;
-    ModuleP bmod;   // the block-column variant of the one-group form
-    int bblock = 256;
+    ConstBlock consts;   // every plan's block, at the plan's kbase
     int nkeys = 0, has_filter = 0;
     std::string last_src;
     struct GenMod {
@@ -597,6 +776,7 @@ int updec_expr_compile(const char* text, const updec_column_desc* cols, int ncol
     std::string err;
     if (!make_plan(text, names, types, mode, &e->plan, &err)) return fail(UPDEC_ECOMPILE, err);
     e->types = types;
+    e->consts.host = e->plan.const_data();
     e->name = "calc_expr_" + std::to_string(g_kernel_id.fetch_add(1));
     GenOptions o;
     o.block = calc_block(e->plan);
@@ -630,6 +810,15 @@ int updec_expr_source(updec_expr_t e, char* buf, size_t cap, size_t* needed)
     return copy_text(e->src, buf, cap, needed);
 }
 
+int updec_expr_source_iter(updec_expr_t e, char* buf, size_t cap, size_t* needed)
+{
+    if (!e) return fail(UPDEC_EINVAL, "null expression");
+    GenOptions o;
+    o.blocks = true;
+    o.block = calc_block(e->plan);
+    return copy_text(gen_calc_kernel(e->plan, e->types, "calc_iter", o), buf, cap, needed);
+}
+
 // every column pointer of a launch over rows > 0 must be set: a NULL column
 // is refused on the host instead of faulting the device
 static int check_cols(const uint8_t* const* d_cols, size_t ncols, int64_t rows)
@@ -654,6 +843,7 @@ int updec_expr_calc(updec_expr_t e, const uint8_t* const* d_cols, int64_t rows, 
         e->fn = e->mod->fn("calc_expr", &err);
         if (!e->fn) return fail(UPDEC_ECOMPILE, err);
     }
+    if (int rc = e->consts.upload(S(stream))) return rc;
     ColArgs a{};
     for (size_t i = 0; i < e->types.size(); i++) {
         a.col[i] = d_cols[i];
@@ -661,6 +851,7 @@ int updec_expr_calc(updec_expr_t e, const uint8_t* const* d_cols, int64_t rows, 
     }
     a.rows = (usize)rows;
     a.out = d_out;
+    a.kc = e->consts.dev;
     const int T = calc_block(e->plan);
     int64_t tiles = (rows + T - 1) / T;
     if (tiles > INT32_MAX) return fail(UPDEC_EINVAL, "too many rows for one launch");
@@ -671,35 +862,56 @@ int updec_expr_calc(updec_expr_t e, const uint8_t* const* d_cols, int64_t rows, 
     return UPDEC_OK;
 }
 
-int updec_expr_calc_blocks(updec_expr_t e, const updec_column_blocks* cols, const int32_t* d_indices, int64_t rows,
-                           uint8_t* d_out, uint8_t* d_valid, updec_stream_t stream)
+// launch the iterator variant of an expression's calc kernel over filled arguments
+static int calc_iter_launch(updec_expr_t e, IterArgs* a, int64_t rows, uint8_t* d_out, uint8_t* d_valid, int out_nullable,
+                            updec_stream_t stream)
 {
-    if (!e || rows < 0 || (rows && (!d_out || (!cols && !e->types.empty())))) return fail(UPDEC_EINVAL, "bad argument");
-    if (reinterpret_cast<uintptr_t>(d_out) & 3u) return fail(UPDEC_EINVAL, "output must be 4-byte aligned");
-    IterArgs a;
-    int rc = fill_iter(&a, e->types, cols, d_indices, rows);
-    if (rc != UPDEC_OK) return rc;
     if (rows == 0) return UPDEC_OK;
     std::string err;
     if (!e->bmod) {
         GenOptions o;
         o.blocks = true;
         o.block = calc_block(e->plan);
-        e->bmod = ModuleCache::get().compile(gen_calc_kernel(e->plan, e->types, "calc_blocks", o), &err);
+        e->bmod = ModuleCache::get().compile(gen_calc_kernel(e->plan, e->types, "calc_iter", o), &err);
         if (!e->bmod) return fail(UPDEC_ECOMPILE, err);
-        e->bfn = e->bmod->fn("calc_blocks", &err);
+        e->bfn = e->bmod->fn("calc_iter", &err);
         if (!e->bfn) return fail(UPDEC_ECOMPILE, err);
     }
-    a.out = d_out;
-    a.valid = d_valid;
+    if (int rc2 = e->consts.upload(S(stream))) return rc2;
+    a->out = d_out;
+    a->valid = d_valid;
+    a->kc = e->consts.dev;
+    a->outnull = out_nullable ? 1u : 0u;
     const int T = calc_block(e->plan);
     int64_t tiles = (rows + T - 1) / T;
     if (tiles > INT32_MAX) return fail(UPDEC_EINVAL, "too many rows for one launch");
     unsigned grid = calc_one_row(e->plan) ? (unsigned)tiles
                                           : (unsigned)std::min<int64_t>(tiles, (int64_t)device_cus() * 32 * (256 / T));
-    void* params[] = {&a};
+    void* params[] = {a};
     HIPCHK(hipModuleLaunchKernel(e->bfn, grid, 1, 1, T, 1, 1, 0, S(stream), params, nullptr));
     return UPDEC_OK;
+}
+
+int updec_expr_calc_blocks(updec_expr_t e, const updec_column_blocks* cols, const int32_t* d_indices, int64_t rows,
+                           uint8_t* d_out, uint8_t* d_valid, updec_stream_t stream)
+{
+    if (!e || rows < 0 || (rows && (!d_out || (!cols && !e->types.empty())))) return fail(UPDEC_EINVAL, "bad argument");
+    if (reinterpret_cast<uintptr_t>(d_out) & 3u) return fail(UPDEC_EINVAL, "output must be 4-byte aligned");
+    IterArgs a;
+    int rc = fill_blocks(&a, e->types, cols, d_indices, rows);
+    if (rc != UPDEC_OK) return rc;
+    return calc_iter_launch(e, &a, rows, d_out, d_valid, 0, stream);
+}
+
+int updec_expr_calc_iter(updec_expr_t e, const updec_column_iter* cols, int64_t rows, uint8_t* d_out,
+                         int32_t out_nullable, uint8_t* d_valid, updec_stream_t stream)
+{
+    if (!e || rows < 0 || (rows && !d_out)) return fail(UPDEC_EINVAL, "bad argument");
+    if (reinterpret_cast<uintptr_t>(d_out) & 3u) return fail(UPDEC_EINVAL, "output must be 4-byte aligned");
+    IterArgs a;
+    int rc = fill_iters(&a, e->types, cols, rows);
+    if (rc != UPDEC_OK) return rc;
+    return calc_iter_launch(e, &a, rows, d_out, d_valid, out_nullable, stream);
 }
 
 void updec_expr_free(updec_expr_t e) { delete e; }
@@ -716,6 +928,10 @@ int updec_agg_compile(const updec_agg_desc* d, updec_agg_t* out)
         Plan p;
         if (!make_plan(d->exprs[k], names, a->types, d->mode, &p, &err))
             return fail(UPDEC_ECOMPILE, "expr " + std::to_string(k) + ": " + err);
+        // constants of plan k follow those of plans 0..k-1
+        p.kbase = (int)a->consts.host.size();
+        const std::vector<u32> kd = p.const_data();
+        a->consts.host.insert(a->consts.host.end(), kd.begin(), kd.end());
         a->spec.plans.push_back(std::move(p));
     }
     a->has_filter = d->has_filter;
@@ -740,9 +956,10 @@ static std::string form_name(const updec_agg_s* a, bool onepass)
     return onepass ? "agg_keys" : (a->nkeys == 0 ? "agg_sum" : "agg_lut");
 }
 
-static int agg_module(updec_agg_t a, int ngroups, ModuleP* mod, int* block, std::string* err)
+static int agg_module(updec_agg_t a, int ngroups, bool iter, ModuleP* mod, int* block, std::string* err)
 {
-    auto it = a->by_ngroups.find(ngroups);
+    const int mkey = ngroups | (iter ? 1 << 20 : 0);
+    auto it = a->by_ngroups.find(mkey);
     if (it != a->by_ngroups.end()) {
         *mod = it->second.mod;
         *block = it->second.block;
@@ -752,6 +969,7 @@ static int agg_module(updec_agg_t a, int ngroups, ModuleP* mod, int* block, std:
     AggSpec s = a->spec;
     s.ngroups = ngroups;
     GenOptions o;
+    o.blocks = iter;
     int aw = 0, blk = 256;
     std::string src = gen_agg_kernels(s, a->types, form_name(a, false), o, &aw, &blk);
     if (src.empty()) {
@@ -760,7 +978,7 @@ static int agg_module(updec_agg_t a, int ngroups, ModuleP* mod, int* block, std:
     }
     ModuleP m = ModuleCache::get().compile(src, err);
     if (!m) return UPDEC_ECOMPILE;
-    a->by_ngroups[ngroups] = {m, src, blk};
+    a->by_ngroups[mkey] = {m, src, blk};
     a->last_src = src;
     *mod = m;
     *block = blk;
@@ -781,6 +999,7 @@ static void fill_args(updec_agg_t a, AggArgs* args, const uint8_t* const* d_cols
     args->key1 = d_key1;
     args->lut = a->d_lut;
     args->partials = a->d_partials;
+    args->kc = a->consts.dev;
 }
 
 // rps: rows a lane takes per loop step (1, or 4 for row quads)
@@ -871,6 +1090,7 @@ static int agg_run_onepass(updec_agg_t a, const uint8_t* const* d_cols, const in
     a->stat_dirty = true;
     u32* stat = a->d_meta + (size_t)a->stat_phase * kAggOnepassStatWords;
     u32* next = a->d_meta + (size_t)(1 - a->stat_phase) * kAggOnepassStatWords;
+    if (int krc = a->consts.upload(st)) return krc;
     AggArgs args;
     fill_args(a, &args, d_cols, d_filter, d_key0, d_key1, rows);
     u32* ovf = stat;
@@ -914,24 +1134,15 @@ static int agg_run_onepass(updec_agg_t a, const uint8_t* const* d_cols, const in
     return UPDEC_OK;
 }
 
-int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_filter, const uint8_t* d_key0,
-                  const uint8_t* d_key1, int64_t rows, uint8_t* d_images, int64_t* d_counts, int32_t max_groups,
-                  int32_t* ngroups_out, uint16_t* keys_out, updec_stream_t stream)
+// Two-pass form: key discovery into a host-numbered table, then one
+// generated kernel pass with a group LUT (or no keys: one group).  With
+// `iter` the columns come through column iterators (the kernel variant
+// generated with iterator loads); else the contiguous columns d_cols.
+static int agg_run_twopass(updec_agg_t a, const IterArgs* iter, const uint8_t* const* d_cols, const int32_t* d_filter,
+                           const uint8_t* d_key0, const uint8_t* d_key1, int64_t rows, uint8_t* d_images,
+                           int64_t* d_counts, int32_t max_groups, int32_t* ngroups_out, uint16_t* keys_out,
+                           hipStream_t st)
 {
-    if (!a || rows < 0 || !d_images || !d_counts || !ngroups_out) return fail(UPDEC_EINVAL, "bad argument");
-    if (int rc = check_cols(d_cols, a->types.size(), rows)) return rc;
-    if (max_groups < 1 || max_groups > 255) return fail(UPDEC_EINVAL, "max_groups must be in [1, 255]");
-    if (a->has_filter && !d_filter) return fail(UPDEC_EINVAL, "filter column missing");
-    if (a->nkeys >= 1 && !d_key0) return fail(UPDEC_EINVAL, "key column missing");
-    if (a->nkeys == 2 && !d_key1) return fail(UPDEC_EINVAL, "second key column missing");
-    hipStream_t st = S(stream);
-    if (a->onepass) {
-        bool overflowed = false;
-        int rc = agg_run_onepass(a, d_cols, d_filter, d_key0, d_key1, rows, d_images, d_counts, max_groups, ngroups_out,
-                                 keys_out, st, &overflowed);
-        if (rc != UPDEC_OK || !overflowed) return rc;
-        // more distinct keys in a block than the one-pass table holds: two-pass form
-    }
     int ngroups = 1;
     std::vector<uint16_t> keys;
     if (a->nkeys > 0) {
@@ -970,7 +1181,7 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
     std::string err;
     ModuleP mod;
     int T = 256;
-    int mrc = agg_module(a, ngroups, &mod, &T, &err);
+    int mrc = agg_module(a, ngroups, iter != nullptr, &mod, &T, &err);
     if (mrc != UPDEC_OK) return fail(mrc, err);
     hipFunction_t fmain = mod->fn(form_name(a, false) + "_main", &err);
     hipFunction_t ffin = fmain ? mod->fn(form_name(a, false) + "_final", &err) : nullptr;
@@ -979,20 +1190,36 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
     const int NP = (int)a->spec.plans.size();
     const int aw = agg_acc_words(a->spec);
     const size_t PW = (size_t)NP * ngroups * aw + 2 * (size_t)ngroups;
-    const int64_t grid = agg_grid(fmain, rows, T, agg_rows_per_step(a->spec, a->types, GenOptions()));
+    GenOptions go;
+    go.blocks = iter != nullptr;
+    const int64_t grid = agg_grid(fmain, rows, T, agg_rows_per_step(a->spec, a->types, go));
     if ((size_t)grid * PW > a->partial_words) {
         if (a->d_partials) HIPCHK(hipFree(a->d_partials));
         a->d_partials = nullptr;
         HIPCHK(hipMalloc((void**)&a->d_partials, (size_t)grid * PW * sizeof(u32)));
         a->partial_words = (size_t)grid * PW;
     }
+    if (int krc = a->consts.upload(st)) return krc;
     AggArgs args;
-    fill_args(a, &args, d_cols, d_filter, d_key0, d_key1, rows);
+    IterArgs iargs;
+    void* pm[1];
+    if (iter) {
+        iargs = *iter;
+        iargs.fcol = d_filter;
+        iargs.key0 = d_key0;
+        iargs.key1 = d_key1;
+        iargs.lut = a->d_lut;
+        iargs.partials = a->d_partials;
+        iargs.kc = a->consts.dev;
+        pm[0] = &iargs;
+    } else {
+        fill_args(a, &args, d_cols, d_filter, d_key0, d_key1, rows);
+        pm[0] = &args;
+    }
     if (!a->ev0) {
         HIPCHK(hipEventCreate(&a->ev0));
         HIPCHK(hipEventCreate(&a->ev1));
     }
-    void* pm[] = {&args};
     HIPCHK(hipEventRecord(a->ev0, st));
     HIPCHK(hipModuleLaunchKernel(fmain, (uint32_t)grid, 1, 1, T, 1, 1, 0, st, pm, nullptr));
     HIPCHK(hipEventRecord(a->ev1, st));
@@ -1006,51 +1233,59 @@ int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_
     return launch_avgs(a, d_images, d_counts, ngroups, max_groups, st);
 }
 
+int updec_agg_run(updec_agg_t a, const uint8_t* const* d_cols, const int32_t* d_filter, const uint8_t* d_key0,
+                  const uint8_t* d_key1, int64_t rows, uint8_t* d_images, int64_t* d_counts, int32_t max_groups,
+                  int32_t* ngroups_out, uint16_t* keys_out, updec_stream_t stream)
+{
+    if (!a || rows < 0 || !d_images || !d_counts || !ngroups_out) return fail(UPDEC_EINVAL, "bad argument");
+    if (int rc = check_cols(d_cols, a->types.size(), rows)) return rc;
+    if (max_groups < 1 || max_groups > 255) return fail(UPDEC_EINVAL, "max_groups must be in [1, 255]");
+    if (a->has_filter && !d_filter) return fail(UPDEC_EINVAL, "filter column missing");
+    if (a->nkeys >= 1 && !d_key0) return fail(UPDEC_EINVAL, "key column missing");
+    if (a->nkeys == 2 && !d_key1) return fail(UPDEC_EINVAL, "second key column missing");
+    hipStream_t st = S(stream);
+    if (a->onepass) {
+        bool overflowed = false;
+        int rc = agg_run_onepass(a, d_cols, d_filter, d_key0, d_key1, rows, d_images, d_counts, max_groups, ngroups_out,
+                                 keys_out, st, &overflowed);
+        if (rc != UPDEC_OK || !overflowed) return rc;
+        // more distinct keys in a block than the one-pass table holds: two-pass form
+    }
+    return agg_run_twopass(a, nullptr, d_cols, d_filter, d_key0, d_key1, rows, d_images, d_counts, max_groups,
+                           ngroups_out, keys_out, st);
+}
+
 int updec_agg_run_blocks(updec_agg_t a, const updec_column_blocks* cols, const int32_t* d_indices, int64_t rows,
                          uint8_t* d_images, int64_t* d_counts, updec_stream_t stream)
 {
     if (!a || rows < 0 || !d_images || !d_counts || (!cols && !a->types.empty())) return fail(UPDEC_EINVAL, "bad argument");
     if (a->has_filter || a->nkeys > 0)
-        return fail(UPDEC_EUNSUPPORTED, "block-column inputs take ungrouped, unfiltered aggregations");
+        return fail(UPDEC_EUNSUPPORTED, "block-column inputs take ungrouped, unfiltered aggregations (see updec_agg_run_iter)");
     IterArgs args;
-    int rc = fill_iter(&args, a->types, cols, d_indices, rows);
+    int rc = fill_blocks(&args, a->types, cols, d_indices, rows);
     if (rc != UPDEC_OK) return rc;
-    std::string err;
-    if (!a->bmod) {
-        AggSpec s = a->spec;
-        s.ngroups = 1;
-        GenOptions o;
-        o.blocks = true;
-        int aw = 0;
-        std::string src = gen_agg_kernels(s, a->types, "agg_blocks", o, &aw, &a->bblock);
-        if (src.empty()) return fail(UPDEC_EUNSUPPORTED, "no block-column form for this aggregation");
-        a->bmod = ModuleCache::get().compile(src, &err);
-        if (!a->bmod) return fail(UPDEC_ECOMPILE, err);
-    }
-    hipFunction_t fmain = a->bmod->fn("agg_blocks_main", &err);
-    hipFunction_t ffin = fmain ? a->bmod->fn("agg_blocks_final", &err) : nullptr;
-    if (!fmain || !ffin) return fail(UPDEC_ECOMPILE, err);
-    hipStream_t st = S(stream);
-    const int T = a->bblock;
-    const int NP = (int)a->spec.plans.size();
-    const size_t PW = (size_t)NP * agg_acc_words(a->spec) + 2;
-    const int64_t grid = agg_grid(fmain, rows, T, 1);
-    if ((size_t)grid * PW > a->partial_words) {
-        if (a->d_partials) HIPCHK(hipFree(a->d_partials));
-        a->d_partials = nullptr;
-        HIPCHK(hipMalloc((void**)&a->d_partials, (size_t)grid * PW * sizeof(u32)));
-        a->partial_words = (size_t)grid * PW;
-    }
-    args.partials = a->d_partials;
-    void* pm[] = {&args};
-    HIPCHK(hipModuleLaunchKernel(fmain, (uint32_t)grid, 1, 1, T, 1, 1, 0, st, pm, nullptr));
-    const u32* cp = a->d_partials;
-    u32 nb = (u32)grid;
-    uint8_t* img = d_images;
-    uint64_t* cnt = reinterpret_cast<uint64_t*>(d_counts);
-    void* pf[] = {&cp, &nb, &img, &cnt};
-    HIPCHK(hipModuleLaunchKernel(ffin, (unsigned)(NP + 1), 1, 1, 256, 1, 1, 0, st, pf, nullptr));
-    return launch_avgs(a, d_images, d_counts, 1, 1, st);
+    int32_t ng = 0;
+    return agg_run_twopass(a, &args, nullptr, nullptr, nullptr, nullptr, rows, d_images, d_counts, 1, &ng, nullptr,
+                           S(stream));
+}
+
+int updec_agg_run_iter(updec_agg_t a, const updec_column_iter* cols, int64_t rows, const int32_t* d_filter,
+                       const uint8_t* d_key0, const uint8_t* d_key1, uint8_t* d_images, int64_t* d_counts,
+                       int32_t max_groups, int32_t* ngroups_out, uint16_t* keys_out, updec_stream_t stream)
+{
+    if (!a || rows < 0 || !d_images || !d_counts || !ngroups_out) return fail(UPDEC_EINVAL, "bad argument");
+    if (max_groups < 1 || max_groups > 255) return fail(UPDEC_EINVAL, "max_groups must be in [1, 255]");
+    if (a->has_filter && !d_filter) return fail(UPDEC_EINVAL, "filter column missing");
+    if (a->nkeys >= 1 && !d_key0) return fail(UPDEC_EINVAL, "key column missing");
+    if (a->nkeys == 2 && !d_key1) return fail(UPDEC_EINVAL, "second key column missing");
+    IterArgs args;
+    int rc = fill_iters(&args, a->types, cols, rows);
+    if (rc != UPDEC_OK) return rc;
+    if (args.nullcols && a->spec.avg_mask)
+        return fail(UPDEC_EUNSUPPORTED, "AVG over columns that can be NULL needs the non-NULL count: divide the SUM "
+                                        "images with updec_div_int64 instead of avg_mask");
+    return agg_run_twopass(a, &args, nullptr, d_filter, d_key0, d_key1, rows, d_images, d_counts, max_groups,
+                           ngroups_out, keys_out, S(stream));
 }
 
 int updec_agg_last_kernel_ms(updec_agg_t a, float* ms)
@@ -1075,6 +1310,20 @@ int updec_agg_source(updec_agg_t a, int32_t ngroups, char* buf, size_t cap, size
     return copy_text(src, buf, cap, needed);
 }
 
+int updec_agg_source_iter(updec_agg_t a, int32_t ngroups, char* buf, size_t cap, size_t* needed)
+{
+    if (!a || ngroups < 1 || ngroups > 255) return fail(UPDEC_EINVAL, "bad argument");
+    AggSpec s = a->spec;
+    s.onepass = false;
+    s.ngroups = ngroups;
+    GenOptions o;
+    o.blocks = true;
+    int aw = 0, blk = 0;
+    std::string src = gen_agg_kernels(s, a->types, form_name(a, false), o, &aw, &blk);
+    if (src.empty()) return fail(UPDEC_EUNSUPPORTED, "accumulators do not fit in LDS");
+    return copy_text(src, buf, cap, needed);
+}
+
 void updec_agg_free(updec_agg_t a) { delete a; }
 
 int updec_div_int64(const uint8_t* d_sums, int64_t stride_bytes, const int64_t* d_counts, uint8_t* d_out, int64_t n,
@@ -1093,107 +1342,11 @@ int updec_div_int64(const uint8_t* d_sums, int64_t stride_bytes, const int64_t* 
     return UPDEC_OK;
 }
 
-// segmented SUM: chunks of at most 2^16 rows, one block per chunk, then one
-// block per segment over its chunk partials
-static std::string gen_segsum(int P, int S_, int mode)
+int updec_module_cache_stats(int64_t* rtc_compiles, int64_t* memory_hits, int64_t* disk_hits)
 {
-    const int B = compact_bytes(P);
-    const int NW = (B + 3) / 4;
-    const int AW = std::max(1, (8 * B - 1 + 40 + 1 + 31) / 32);
-    std::string t =
-        "#include \"dec_device.h\"\nusing namespace updec;\n"
-        "// segmented SUM of DECIMAL(@P,@S), mode @M\n"
-        "struct SegArgs { const u8* col; usize colbytes; const i32* assoc; const i64* chunk_lo; const i64* chunk_hi;\n"
-        "                 const i32* seg_chunk0; i32 nseg; u32* partials; u8* out; };\n"
-        "extern \"C\" __global__ void __launch_bounds__(256) seg_chunk(SegArgs a) {\n"
-        "  const i64 lo = a.chunk_lo[blockIdx.x], hi = a.chunk_hi[blockIdx.x];\n"
-        "  Mag<@AW> acc = mzero<@AW>();\n"
-        "  for (i64 i = lo + threadIdx.x; i < hi; i += 256) {\n"
-        "    const usize row = a.assoc ? (usize)a.assoc[i] : (usize)i;\n"
-        "    const Dec<@NW> v = load_compact<@B>(a.col, row, a.colbytes);\n"
-        "    acc_add<@AW, @NW>(acc, v);\n  }\n"
-        "  wave_reduce<@AW>(acc);\n"
-        "  __shared__ u32 red[4 * @AW];\n"
-        "  if ((threadIdx.x & 63u) == 0) for (int w = 0; w < @AW; w++) red[(threadIdx.x >> 6) * @AW + w] = acc.w[w];\n"
-        "  __syncthreads();\n"
-        "  if (threadIdx.x == 0) {\n    Mag<@AW> t = mzero<@AW>();\n"
-        "    for (int wv = 0; wv < 4; wv++) { Mag<@AW> o; for (int w = 0; w < @AW; w++) o.w[w] = red[wv * @AW + w]; add_to<@AW>(t, t, o); }\n"
-        "    for (int w = 0; w < @AW; w++) a.partials[(usize)blockIdx.x * @AW + w] = t.w[w];\n  }\n}\n"
-        "extern \"C\" __global__ void __launch_bounds__(64) seg_final(SegArgs a) {\n"
-        "  const i32 g = blockIdx.x * 64 + threadIdx.x;\n  if (g >= a.nseg) return;\n"
-        "  Mag<@AW> t = mzero<@AW>();\n"
-        "  for (i32 c = a.seg_chunk0[g]; c < a.seg_chunk0[g + 1]; c++) { Mag<@AW> o; for (int w = 0; w < @AW; w++) o.w[w] = a.partials[(usize)c * @AW + w]; add_to<@AW>(t, t, o); }\n"
-        "  write_sum_image<@AW>(a.out + (usize)g * 133u, t, @PBu, @SBu, 0u);\n}\n";
-    auto sub = [&](const std::string& key, const std::string& val) {
-        for (size_t pos = t.find(key); pos != std::string::npos; pos = t.find(key, pos + val.size())) t.replace(pos, key.size(), val);
-    };
-    sub("@PB", std::to_string(P & 0xff));
-    sub("@SB", std::to_string(S_ & 0xff));
-    sub("@AW", std::to_string(AW));
-    sub("@NW", std::to_string(NW));
-    sub("@B", std::to_string(B));
-    sub("@P", std::to_string(P));
-    sub("@S", std::to_string(S_));
-    sub("@M", std::to_string(mode));
-    return t;
-}
-
-int updec_segsum_compact(const uint8_t* d_col, int precision, int scale, int64_t rows, const int32_t* d_associated,
-                         const int32_t* d_groups, int32_t ngroups, uint8_t* d_out, int mode, updec_stream_t stream)
-{
-    if (rows < 0 || ngroups < 0 || (ngroups && (!d_groups || !d_out)) || (rows && !d_col) || precision <= 0)
-        return fail(UPDEC_EINVAL, "bad argument");
-    if (ngroups == 0) return UPDEC_OK;
-    hipStream_t st = S(stream);
-    std::vector<int32_t> g(ngroups);
-    HIPCHK(hipMemcpyAsync(g.data(), d_groups, sizeof(int32_t) * ngroups, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    std::vector<int64_t> lo, hi;
-    std::vector<int32_t> c0(ngroups + 1);
-    const int64_t CH = 65536;
-    for (int s = 0; s < ngroups; s++) {
-        int64_t a = g[s], b = s + 1 < ngroups ? g[s + 1] : rows;
-        if (a < 0 || b > rows || a > b) return fail(UPDEC_EINVAL, "segment starts out of order or out of range");
-        c0[s] = (int32_t)lo.size();
-        for (int64_t x = a; x < b; x += CH) { lo.push_back(x); hi.push_back(std::min(b, x + CH)); }
-    }
-    c0[ngroups] = (int32_t)lo.size();
-    std::string err;
-    ModuleP m = ModuleCache::get().compile(gen_segsum(precision, scale, mode), &err);
-    if (!m) return fail(UPDEC_ECOMPILE, err);
-    hipFunction_t fc = m->fn("seg_chunk", &err), ff = fc ? m->fn("seg_final", &err) : nullptr;
-    if (!fc || !ff) return fail(UPDEC_ECOMPILE, err);
-    const int B = compact_bytes(precision);
-    const int AW = std::max(1, (8 * B - 1 + 40 + 1 + 31) / 32);
-    const size_t nch = lo.size();
-    // one device block for the chunk table and partials
-    size_t bytes = nch * 16 + (ngroups + 1) * 4 + std::max<size_t>(nch, 1) * AW * 4 + 64;
-    uint8_t* dmem = nullptr;
-    HIPCHK(hipMallocAsync((void**)&dmem, bytes, st));
-    i64* dlo = reinterpret_cast<i64*>(dmem);
-    i64* dhi = dlo + nch;
-    i32* dc0 = reinterpret_cast<i32*>(dhi + nch);
-    u32* dpart = reinterpret_cast<u32*>(dmem + ((nch * 16 + (ngroups + 1) * 4 + 15) & ~size_t(15)));
-    if (nch) {
-        HIPCHK(hipMemcpyAsync(dlo, lo.data(), nch * 8, hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(dhi, hi.data(), nch * 8, hipMemcpyHostToDevice, st));
-    }
-    HIPCHK(hipMemcpyAsync(dc0, c0.data(), (ngroups + 1) * 4, hipMemcpyHostToDevice, st));
-    struct {
-        const u8* col;
-        usize colbytes;
-        const i32* assoc;
-        const i64* lo;
-        const i64* hi;
-        const i32* c0;
-        i32 nseg;
-        u32* partials;
-        u8* out;
-    } args = {d_col, (usize)rows * (usize)B, d_associated, dlo, dhi, dc0, ngroups, dpart, d_out};
-    void* pm[] = {&args};
-    if (nch) HIPCHK(hipModuleLaunchKernel(fc, (unsigned)nch, 1, 1, 256, 1, 1, 0, st, pm, nullptr));
-    HIPCHK(hipModuleLaunchKernel(ff, (unsigned)((ngroups + 63) / 64), 1, 1, 64, 1, 1, 0, st, pm, nullptr));
-    HIPCHK(hipFreeAsync(dmem, st));
+    if (rtc_compiles) *rtc_compiles = g_rtc_compiles.load();
+    if (memory_hits) *memory_hits = g_mem_hits.load();
+    if (disk_hits) *disk_hits = g_disk_hits.load();
     return UPDEC_OK;
 }
 
