@@ -268,6 +268,21 @@ int updec_div_int64(const uint8_t* d_sums, int64_t stride_bytes, const int64_t* 
  * hipMallocAsync on the stream. */
 int updec_segsum_compact(const uint8_t* d_col, int precision, int scale, int64_t rows, const int32_t* d_associated,
                          const int32_t* d_groups, int32_t ngroups, uint8_t* d_out, int mode, updec_stream_t stream);
+/* updec_segsum_compact for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, results and kernels;
+ * the accumulators live in a fixed 8 MiB device array of the library (one
+ * per GPU, held until the process ends, its address looked up by
+ * updec_init or by the first call on a device), so the call allocates
+ * nothing.  At most updec_segsum_noalloc_max_segments(precision) segments
+ * (ngroups * (W + 2) 64-bit words, W = 2, 4, 8, 16 or 32 limbs by B(P)):
+ * beyond that UPDEC_EINVAL, nothing launched.  Every call on a GPU uses the
+ * same array: calls that overlap on one GPU (different streams or threads)
+ * race and give wrong sums; the array is cleared by each call before use. */
+int updec_segsum_compact_noalloc(const uint8_t* d_col, int precision, int scale, int64_t rows,
+                                 const int32_t* d_associated, const int32_t* d_groups, int32_t ngroups, uint8_t* d_out,
+                                 int mode, updec_stream_t stream);
+int updec_segsum_noalloc_max_segments(int precision, int32_t* max_segments);
 
 /* ---------------- codec: ingest and output of compact columns ----------------
  * Bulk ingest of one text column into DECIMAL(P, S) compact rows: the

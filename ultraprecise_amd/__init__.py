@@ -174,6 +174,9 @@ def _load():
         "updec_agg_free": (None, [P]),
         "updec_div_int64": (ctypes.c_int, [P, I64, P, P, I64, ctypes.c_int, P]),
         "updec_segsum_compact": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I64, P, P, I32, P, ctypes.c_int, P]),
+        "updec_segsum_compact_noalloc": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, I64, P, P, I32, P, ctypes.c_int,
+                                                        P]),
+        "updec_segsum_noalloc_max_segments": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(I32)]),
         "updec_compile": (ctypes.c_int, [ctypes.c_char_p, SZ, ctypes.POINTER(P)]),
         "updec_launch": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_uint32, ctypes.c_uint32, P, SZ, P]),
         "updec_module_free": (None, [P]),
@@ -219,7 +222,8 @@ EXPORTED = ["updec_last_error", "updec_version", "updec_init", "updec_compact_by
             "updec_event_destroy", "updec_event_query", "updec_event_sync", "updec_copy_kernel",
             "updec_expr_compile", "updec_expr_info", "updec_expr_plan", "updec_expr_source", "updec_expr_calc",
             "updec_expr_free", "updec_agg_compile", "updec_agg_run", "updec_agg_last_kernel_ms", "updec_agg_source",
-            "updec_agg_free", "updec_div_int64", "updec_segsum_compact", "updec_compile", "updec_launch",
+            "updec_agg_free", "updec_div_int64", "updec_segsum_compact", "updec_segsum_compact_noalloc",
+            "updec_segsum_noalloc_max_segments", "updec_compile", "updec_launch",
             "updec_module_free", "updec_check_compile", "updec_module_cache_stats", "updec_gen_compact", "updec_gen_int32", "updec_gen_byte",
             "updec_vm_compile", "updec_vm_info", "updec_vm_listing", "updec_vm_free", "updec_batch_create",
             "updec_batch_tuning", "updec_batch_add", "updec_batch_pending", "updec_batch_launch", "updec_batch_free",

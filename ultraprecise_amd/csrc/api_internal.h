@@ -18,5 +18,8 @@ std::vector<ColumnType> api_types(const updec_column_desc* cols, int n, std::vec
 // parse -> AlignOptimize -> plan; mode AUTO picks faithful inside the envelope
 bool api_make_plan(const std::string& text, const std::vector<std::string>& names,
                    const std::vector<ColumnType>& types, int mode, Plan* out, std::string* err);
+// device address of the segmented SUM's fixed scratch on the current device
+// (segsum.hip), looked up once per device
+int segsum_scratch_setup();
 
 }  // namespace updec

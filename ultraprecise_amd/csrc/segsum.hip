@@ -1,5 +1,5 @@
 // segsum.hip -- segmented SUM of a compact decimal column
-// (updec_segsum_compact in include/updec.h).
+// (updec_segsum_compact, updec_segsum_compact_noalloc in include/updec.h).
 //
 // The reference sums a compact column per group with aggregate_column
 // (src/CudaAcc/AriesEngineAlgorithm.h:832-857): rows are visited in
@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "../../include/updec.h"
@@ -215,7 +216,44 @@ int seg_launch(const uint8_t* d_col, int P, int S_, int B, int W, int64_t rows, 
     }
 }
 
+// ------------------------------------------------------------------ fixed scratch of the no-alloc entry point
+
+// Accumulators of updec_segsum_compact_noalloc: 8 MiB on every device the
+// library runs on, held until the process ends.  The array's name is a host
+// stand-in; kernels get the device address from hipGetSymbolAddress.
+constexpr usize kSegScratchWords = (usize)1 << 20;
+__device__ __align__(256) u64 scratch[kSegScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+u64* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(u64** out)
+{
+    int dev = 0;
+    SEGCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return api_fail(UPDEC_EUNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        SEGCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<u64*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return UPDEC_OK;
+}
+
 }  // namespace
+
+namespace updec {
+// set-up of the no-alloc entry point's scratch on the current device (called
+// by updec_init)
+int segsum_scratch_setup()
+{
+    u64* p = nullptr;
+    return scratch_for_current_device(&p);
+}
+}  // namespace updec
 
 extern "C" {
 
@@ -233,6 +271,36 @@ int updec_segsum_compact(const uint8_t* d_col, int precision, int scale, int64_t
     const hipError_t fe = hipFreeAsync(acc, st);
     if (rc != UPDEC_OK) return rc;
     if (fe != hipSuccess) return api_fail(UPDEC_EHIP, std::string("hipFreeAsync: ") + hipGetErrorString(fe));
+    return UPDEC_OK;
+}
+
+int updec_segsum_compact_noalloc(const uint8_t* d_col, int precision, int scale, int64_t rows,
+                                 const int32_t* d_associated, const int32_t* d_groups, int32_t ngroups, uint8_t* d_out,
+                                 int mode, updec_stream_t stream)
+{
+    (void)mode;
+    int B = 0, W = 0;
+    int rc = seg_check(d_col, precision, rows, d_groups, ngroups, d_out, &B, &W);
+    if (rc != UPDEC_OK || ngroups == 0) return rc;
+    const usize need = (usize)ngroups * (W + 2);
+    if (need > kSegScratchWords)
+        return api_fail(UPDEC_EINVAL, "segments need " + std::to_string(need * 8) + " bytes of accumulators; the fixed "
+                                      "scratch holds " + std::to_string(kSegScratchWords * 8) +
+                                      " (at most " + std::to_string(kSegScratchWords / (W + 2)) +
+                                      " segments at this precision)");
+    u64* acc = nullptr;
+    rc = scratch_for_current_device(&acc);
+    if (rc != UPDEC_OK) return rc;
+    return seg_launch(d_col, precision, scale, B, W, rows, d_associated, d_groups, ngroups, d_out, acc,
+                      reinterpret_cast<hipStream_t>(stream));
+}
+
+int updec_segsum_noalloc_max_segments(int precision, int32_t* max_segments)
+{
+    if (precision <= 0 || !max_segments) return api_fail(UPDEC_EINVAL, "bad argument");
+    const int W = seg_class_words(compact_bytes(precision));
+    if (!W) return api_fail(UPDEC_EUNSUPPORTED, "rows wider than 128 bytes (the 32-limb Decimal) are not summed");
+    *max_segments = (int32_t)(kSegScratchWords / (W + 2));
     return UPDEC_OK;
 }
 

@@ -620,7 +620,7 @@ int updec_init(int device)
 {
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipFree(nullptr));
-    return UPDEC_OK;
+    return segsum_scratch_setup();
 }
 
 int updec_malloc(void** d_ptr, size_t bytes)
