@@ -177,9 +177,11 @@ def precision_sweep(U, W, steps, rows_override=None):
         ev1.record()
         ms = ev0.elapsed_ms(ev1) / n
         alg = (4 * U.compact_bytes(P) + e.out_bytes) * rows
+        lp = e.limb_products
         out[f"P{P}"] = {"rows": rows, "kernel_ms": ms, "mrows_per_s": rows / ms / 1e3, "alg_bytes": alg,
                         "achieved_gbps": alg / (ms / 1000.0) / 1e9, "out_type": [e.precision, e.scale],
-                        "faithful": e.faithful, "first_call_s": first_s}
+                        "faithful": e.faithful, "first_call_s": first_s, "limb_products_per_row": lp,
+                        "limb_products_per_s": lp * rows / (ms / 1000.0)}
         del dev, obuf
     return out
 
@@ -207,9 +209,11 @@ def series(U, W, steps, cpu_rows, threads):
     U.device_sync()
     ms = 1000.0 * (time.perf_counter() - t0) / steps
     kernel_ms = sum(kms) / len(kms)
+    lp = U.Expr(W.sin_expr(), cols).limb_products
     res = {"rows": rows, "column_bits": bits, "ms_per_step": ms, "kernel_ms": kernel_ms,
            "mrows_per_s": rows / ms / 1e3, "kernel_mrows_per_s": rows / kernel_ms / 1e3,
-           "first_call_s": first_s, "alg_bytes": rows * U.compact_bytes(300)}
+           "first_call_s": first_s, "alg_bytes": rows * U.compact_bytes(300), "limb_products_per_row": lp,
+           "limb_products_per_s": lp * rows / (kernel_ms / 1000.0)}
     if cpu_rows > 0:
         from oracle import oracle as O
         n = min(cpu_rows, rows)
@@ -361,9 +365,10 @@ def rsa(U, W, steps, cpu_rows, threads):
         ev1.record()
         ms = ev0.elapsed_ms(ev1) / steps
         alg = rows * (U.compact_bytes(P) + e.out_bytes)
+        lp = e.limb_products
         r = {"rows": rows, "kernel_ms": ms, "mrows_per_s": rows / ms / 1e3, "alg_bytes": alg,
              "achieved_gbps": alg / ms / 1e6, "out_type": [e.precision, e.scale], "faithful": e.faithful,
-             "first_call_s": first_s}
+             "first_call_s": first_s, "limb_products_per_row": lp, "limb_products_per_s": lp * rows / (ms / 1000.0)}
         if cpu_rows > 0:
             from oracle import oracle as O
             n = min(cpu_rows, rows)
@@ -377,52 +382,43 @@ def rsa(U, W, steps, cpu_rows, threads):
     return out
 
 
-def serving_bench(U, W, nq, rates, budget_gb, jit_queries=12):
-    """the batched serving workload: nq seeded queries over resident columns,
-    saturated (all queued at once) and at Poisson arrival rates, batched
-    (HIP graph) against one plain launch per query"""
+def serving_bench(U, W, nq, rates, reserve_gb, jit_queries=12):
+    """the batched serving workload: nq seeded queries at the stated
+    distribution (every query as drawn: programs of up to 512 limbs run on
+    the interpreter's multi-lane classes), each on its own fresh columns
+    generated on the device when the query is admitted (admission by free
+    device memory, hipMemGetInfo) and freed when it completes; saturated
+    (all queued at once) and at Poisson arrival rates, batched (HIP graph)
+    against one plain launch per query"""
     import numpy as np
     from ultraprecise_amd import serving
-
-    def ok(text, cols):
-        try:
-            U.VmProgram(text, cols)
-            return True
-        except U.UpdecError as e:
-            if "exceed" in str(e):
-                return False
-            raise
-
     from ultraprecise_amd import autotune
     C = W.CONFIG5
     t0 = time.time()
     tuning = autotune.tune(U, use_cache=False)
     tune_s = time.time() - t0
-    log("serving: autotuned (rows per item, blocks per CU) per width class:", tuning, f"in {tune_s:.1f} s")
+    log("serving: autotuned (rows per item, blocks per CU[, lanes]) per width class:", tuning, f"in {tune_s:.1f} s")
     t0 = time.time()
-    qs = [W.random_query(C["seed"], i, ok) for i in range(nq)]
-    rows_max = int(C["rows"][1])
-    store = serving.ColumnStore(U, budget_gb << 30, seed=C["seed"], rows=rows_max)
-    keys = sorted({k for q in qs for k in q["keys"]})
-    for k in keys:   # resident columns, generated once before the stream starts
-        if store.acquire([k]) is None:
-            raise RuntimeError("the column budget cannot hold the resident set")
-        store.release([k])
-    U.device_sync()
+    qs = [W.random_query(C["seed"], i) for i in range(nq)]
+    progs = [U.VmProgram(q["text"], q["cols"]) for q in qs]
     setup_s = time.time() - t0
     imgs = {m: U.DeviceBuffer(133 * nq).zero() for m in ("batched", "plain")}
     cnts = {m: U.DeviceBuffer(8 * nq).zero() for m in ("batched", "plain")}
     out = {"queries": nq, "rows": sum(q["rows"] for q in qs), "setup_s": setup_s, "tune_s": tune_s,
            "tuning": {str(U.VM_CLASS_WIDTHS[k]): v for k, v in tuning.items()},
-           "columns_resident_gb": store.used / 2 ** 30,
-           "width_classes": {w: sum(1 for q in qs if U.VmProgram(q["text"], q["cols"]).width == w)
-                             for w in U.VM_CLASS_WIDTHS}}
+           "admission": f"fresh columns per query, admitted while they fit in free device memory less "
+                        f"{reserve_gb} GiB (hipMemGetInfo)",
+           "width_classes": {w: sum(1 for p in progs if p.width == w) for w in U.VM_CLASS_WIDTHS},
+           "rows_by_width_class": {w: sum(q["rows"] for q, p in zip(qs, progs) if p.width == w)
+                                   for w in U.VM_CLASS_WIDTHS}}
     zero = np.zeros(nq)
     sat = {}
     for mode in ("plain", "batched"):
+        store = serving.FreshColumns(U, C["seed"], reserve_bytes=reserve_gb << 30)
         srv = serving.Server(U, store, batched=(mode == "batched"), tuning=tuning)
         st = srv.run(qs, zero, imgs[mode], cnts[mode])
         sat[mode] = {k: v for k, v in st.items() if k != "latency_s"}
+        sat[mode]["peak_column_gb"] = store.peak / 2 ** 30
         log(f"serving saturated {mode}:", sat[mode])
     same = bytes(imgs["batched"].numpy()) == bytes(imgs["plain"].numpy())
     # the drop-in JIT path, one query at a time (hipRTC compile of the fused
@@ -432,13 +428,16 @@ def serving_bench(U, W, nq, rates, budget_gb, jit_queries=12):
         got = imgs["batched"].numpy().reshape(nq, 133)
         jimg = U.DeviceBuffer(133)
         jcnt = U.DeviceBuffer(8)
+        store = serving.FreshColumns(U, C["seed"], reserve_bytes=reserve_gb << 30)
         t1 = time.perf_counter()
         jit_same = True
         for i in range(njit):
             q = qs[i]
+            cols = store.acquire_query(q, i)
             agg = U.Agg([q["text"]], q["cols"])
-            agg.run([store.cols[k] for k in q["keys"]], q["rows"], jimg, jcnt, max_groups=1)
+            agg.run(cols, q["rows"], jimg, jcnt, max_groups=1)
             jit_same = jit_same and bytes(jimg.numpy()) == bytes(got[i])
+            store.release_query(q, i)
         jit_s = time.perf_counter() - t1
         jrows = sum(q["rows"] for q in qs[:njit])
         sat["plain_jit"] = {"queries": njit, "rows": jrows, "seconds": jit_s, "mrows_per_s": jrows / jit_s / 1e6,
@@ -454,6 +453,7 @@ def serving_bench(U, W, nq, rates, budget_gb, jit_queries=12):
         arr = serving.poisson_arrivals(nq, rate, seed=C["seed"] + rate)
         pois[rate] = {}
         for mode in ("batched", "plain"):
+            store = serving.FreshColumns(U, C["seed"], reserve_bytes=reserve_gb << 30)
             srv = serving.Server(U, store, batched=(mode == "batched"), tuning=tuning)
             st = srv.run(qs, arr, imgs[mode], cnts[mode])
             pois[rate][mode] = {k: v for k, v in st.items() if k != "latency_s"}
@@ -508,9 +508,10 @@ def main():
     ap.add_argument("--series-cpu-rows", type=int, default=50_000, help="rows of the series CPU sample")
     ap.add_argument("--codec-cpu-rows", type=int, default=1_000_000, help="rows of the text-ingest CPU sample")
     ap.add_argument("--rsa-cpu-rows", type=int, default=200_000, help="rows of the RSA CPU sample (per key length)")
-    ap.add_argument("--serve-queries", type=int, default=2000, help="queries per serving run (workload: 10,000)")
+    ap.add_argument("--serve-queries", type=int, default=10_000, help="queries per serving run")
     ap.add_argument("--serve-rates", default="200,400,800")
-    ap.add_argument("--serve-budget-gb", type=int, default=24, help="device memory for resident columns")
+    ap.add_argument("--serve-reserve-gb", type=int, default=8,
+                    help="device memory kept free by the serving admission (GiB)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: the path runs on one "
@@ -536,6 +537,11 @@ def main():
     if "copy" in want:
         extra["copy"] = copy_bandwidth(U)
         log("copy:", extra["copy"])
+        ms, macs, rate = U.mad_peak()
+        extra["mad_peak"] = {"ms": ms, "multiply_adds": macs, "multiply_adds_per_s": rate,
+                             "note": "32x32->64 multiply-add chains (v_mad_u64_u32), 8 independent per lane: the "
+                                     "measured issue rate the limb_products_per_s figures compare with"}
+        log("mad peak:", extra["mad_peak"])
     if "plain" in want:
         extra["plain_1M"] = plain_method(U, W, args.steps, args.warmup)
         log("plain:", extra["plain_1M"])
@@ -556,7 +562,7 @@ def main():
         log("codec:", extra["codec_10M"])
     if "serving" in want:
         rates = [int(r) for r in args.serve_rates.split(",")]
-        extra["serving"] = serving_bench(U, W, args.serve_queries, rates, args.serve_budget_gb)
+        extra["serving"] = serving_bench(U, W, args.serve_queries, rates, args.serve_reserve_gb)
     value = h["rows"] / h["ms_per_step"] / 1e3  # Mrows/s
     achieved = h["achieved_gbps"]
     cpu = h.get("cpu")

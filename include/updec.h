@@ -57,6 +57,10 @@ int updec_init(int device);
  * managed memory (src/CudaAcc/AriesEngineDef.h:72-122). */
 int updec_malloc(void** d_ptr, size_t bytes);
 int updec_free(void* d_ptr);
+/* stream-ordered allocation (hipMallocAsync / hipFreeAsync): no device-wide
+ * synchronization, for buffers whose lifetime follows one stream's work */
+int updec_malloc_async(void** d_ptr, size_t bytes, updec_stream_t stream);
+int updec_free_async(void* d_ptr, updec_stream_t stream);
 int updec_memcpy_htod(void* d_dst, const void* h_src, size_t bytes, updec_stream_t stream);
 int updec_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes, updec_stream_t stream);
 int updec_memset(void* d_ptr, int value, size_t bytes, updec_stream_t stream);
@@ -66,6 +70,9 @@ int updec_stream_sync(updec_stream_t stream);
 int updec_device_sync(void);
 /* device properties: compute units, clock kHz, HBM bytes */
 int updec_device_info(int32_t* cus, int32_t* clock_khz, int64_t* mem_bytes, char* name, size_t name_cap);
+/* free and total device memory of the current device (hipMemGetInfo), for
+ * admission control */
+int updec_mem_info(int64_t* free_bytes, int64_t* total_bytes);
 /* event timing helpers: record on a stream, elapsed ms between two events */
 int updec_event_create(void** ev);
 int updec_event_record(void* ev, updec_stream_t stream);
@@ -79,6 +86,13 @@ int updec_event_destroy(void* ev);
  * calibrate HBM counters and bandwidth; bytes must be a multiple of 16 and
  * both pointers 16-byte aligned.  No counterpart in the reference. */
 int updec_copy_kernel(void* d_dst, const void* d_src, size_t bytes, updec_stream_t stream);
+
+/* Issue-rate probe of the limb multiply (32 x 32 -> 64 multiply-add,
+ * v_mad_u64_u32): blocks x 256 lanes, 8 independent chains of `iters` steps
+ * per lane; returns the kernel time and the multiply-adds it did
+ * (synchronous).  The measured peak the multiply-bound kernels are compared
+ * with.  No counterpart in the reference. */
+int updec_mad_peak(int32_t blocks, int32_t iters, double* ms_out, double* macs_out);
 
 /* Compact bytes per row of DECIMAL(P, *): GetDecimalRealBytes
  * (src/datatypes/decimal.cu:464-467, table at :92). */
@@ -99,6 +113,10 @@ int updec_expr_compile(const char* text, const updec_column_desc* cols, int ncol
 /* result type of the materialized column, its compact bytes, and whether
  * the faithful contract applies */
 int updec_expr_info(updec_expr_t e, int32_t* precision, int32_t* scale, int32_t* out_bytes, int32_t* faithful);
+/* algorithmic work of one row: 32 x 32 -> 64 limb products at the plan's
+ * static widths (products, the division's dividend scaling and Knuth D,
+ * alignment multiplies; a word division counted as one) */
+int updec_expr_work(updec_expr_t e, int64_t* limb_products_per_row);
 /* canonical text of the rewritten, typed plan (NUL-terminated; returns the
  * needed size in *needed) */
 int updec_expr_plan(updec_expr_t e, char* buf, size_t cap, size_t* needed);
@@ -346,8 +364,11 @@ int updec_module_cache_stats(int64_t* rtc_compiles, int64_t* memory_hits, int64_
  *
  * updec_vm_compile: parse, rewrite and type `text` exactly as
  * updec_expr_compile does, then build a register program.  Every register
- * holds W limbs, W in {2, 4, 8, 16, 32} the smallest that holds every
- * intermediate exactly; UPDEC_EUNSUPPORTED when none does.
+ * holds W limbs, W in {2, 4, 8, 16, 32, 64, 128, 256, 512} the smallest
+ * that holds every intermediate exactly; UPDEC_EUNSUPPORTED when none does.
+ * Up to 32 limbs one lane holds a value; the 64- to 512-limb classes
+ * (extended-mode programs only) spread a value over a group of lanes (the
+ * reference's multi-lane XMP form, AriesExprCalcNode.cpp:1320-1400).
  *
  * A batch collects queries (program, device columns, rows, output image and
  * count) and runs them with one set of launches: persistent kernels per
@@ -368,6 +389,10 @@ int updec_batch_create(int32_t max_queries, int32_t max_items, updec_batch_t* ou
 /* per width class (0..4 for W = 2..32): rows per work item and resident
  * blocks per compute unit of the persistent kernel */
 int updec_batch_tuning(updec_batch_t b, int32_t width_class, int64_t rows_per_item, int32_t blocks_per_cu);
+/* lanes per value of a multi-lane class (width_class 5..8, W = 64 .. 512
+ * limbs): W / 4 (4 limbs per lane, the default) or W / 8 (8 limbs per lane),
+ * at most 64 (the 512-limb class: 64 lanes of 8 limbs) */
+int updec_batch_lanes(updec_batch_t b, int32_t width_class, int32_t lanes);
 /* *index_out (optional): the query's position in the pending batch.
  * UPDEC_EFULL (nothing staged) when the batch's query, work-item or program
  * space would overflow: launch the batch and add the query to the next. */

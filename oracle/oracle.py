@@ -90,6 +90,31 @@ def _load(path=LIB_PATH):
 lib = _load()
 
 
+_wide_lib = None
+
+
+def plan_sum_wide(plan_text, cols, precisions, scales, rows, mode=EXTENDED, threads=8):
+    """plan_sum with 640-limb extended values (liboracle_wide.so, built with
+    OD_MAXW = 640): for plans whose exact intermediates pass 192 limbs"""
+    import numpy as np
+    global _wide_lib
+    if _wide_lib is None:
+        _wide_lib = _load(os.path.join(_HERE, "liboracle_wide.so"))
+    parts = plan_text.split(" ", 4)
+    if parts[0] in ("F", "X") and len(parts) == 5:
+        plan_text = parts[4]
+    h = _wide_lib.od_plan_parse(plan_text.encode())
+    if not h:
+        raise ValueError("unparsable plan: " + plan_text)
+    try:
+        arrs, ptrs, P, S = _colargs(cols, precisions, scales)
+        out = ctypes.create_string_buffer(133)
+        n = _wide_lib.od_plan_sum(h, ptrs, P, S, rows, None, out, mode, threads)
+    finally:
+        _wide_lib.od_plan_free(h)
+    return out.raw, n
+
+
 def use_native():
     """Rebuild the oracle with -O3 -march=native for the host this process
     runs on (make native) and route every call below through it.  Used by

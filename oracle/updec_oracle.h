@@ -23,7 +23,9 @@ extern "C" {
 
 #define OD_FAITHFUL 0
 #define OD_EXTENDED 1
+#ifndef OD_MAXW
 #define OD_MAXW 192
+#endif
 
 typedef struct od_dec {
     int32_t sign;   /* 0 = +, 1 = - (sign-magnitude; -0 is representable) */

@@ -260,3 +260,31 @@ def test_plan_eval_matches_restatement(O):
         q = pyref.div(pyref.neg(a), d)
         r = pyref.add(pyref.mul(pyref.add(a, b), c), q[:3])
         assert bytes(got[i * out_b:(i + 1) * out_b]) == pyref.compact_encode(r, out_b)
+
+
+def test_wide_oracle_sums_exact_products(U, O):
+    """the 640-limb build (liboracle_wide.so) against Python integers on
+    products far past 192 limbs, and against the default build on narrow
+    plans"""
+    import random
+    rng = random.Random(5)
+    P, S = 300, 0
+    B = O.lib.od_compact_bytes(P)
+    rows = 6
+    vals = [rng.randrange(10 ** 299, 10 ** 300) * rng.choice((1, -1)) for _ in range(rows)]
+    col = bytearray()
+    for v in vals:
+        m = abs(v).to_bytes(B, "little")
+        col += m[:-1] + bytes([m[-1] | (0x80 if v < 0 else 0)])
+    import numpy as np
+    arr = np.frombuffer(bytes(col), dtype=np.uint8)
+    e = U.Expr("a*a*a*a*a*a*a*a*a*a*a*a*a*a*a", [("a", P, S)])    # about 15,000 bits per row
+    assert not e.faithful
+    img, n = O.plan_sum_wide(e.plan, [arr], [P], [S], rows)
+    total = sum(v ** 15 for v in vals)
+    assert n == rows
+    assert img[1] == (1 if total < 0 else 0)
+    assert int.from_bytes(img[5:133], "little") == abs(total) % (1 << 1024)
+    e2 = U.Expr("a*a+a", [("a", P, S)])
+    assert O.plan_sum_wide(e2.plan, [arr], [P], [S], rows)[0] == O.plan_sum(O.Plan(e2.plan), [arr], [P], [S], rows,
+                                                                            mode=O.EXTENDED)[0]

@@ -205,7 +205,7 @@ def test_constants_are_kernel_data(U):
     b = U.Expr("p*(3-d)*(7.50+t)", cols)
     assert a.source == b.source
     assert a.plan != b.plan
-    assert "0x7d" not in a.source and "a.kc[" in a.source
+    assert "0x7d" not in a.source and "kld(a.kc, " in a.source
     ga = U.Agg(["p*(1-d)", "d+0.05"], cols, nkeys=1).source(0)
     gb = U.Agg(["p*(2-d)", "d+0.07"], cols, nkeys=1).source(0)
     assert ga == gb

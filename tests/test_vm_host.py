@@ -84,7 +84,7 @@ def test_interpreter_random_serving_queries(U, O, vmh):
             U.VmProgram(text, cols)
             return True
         except U.UpdecError as e:
-            if "exceed" in str(e):
+            if "exceed" in str(e) or "JIT path only" in str(e):
                 return False
             raise
 
@@ -97,4 +97,6 @@ def test_interpreter_random_serving_queries(U, O, vmh):
                                               neg_permille=500), rows) for _, cid in q["keys"]]
         assert vmh(q["text"], q["cols"], data, rows) == oracle_sum(U, O, q["text"], q["cols"], data, rows), q
         widths.add(U.VmProgram(q["text"], q["cols"]).width)
-    assert widths == set(U.VM_CLASS_WIDTHS) - {2} or widths == set(U.VM_CLASS_WIDTHS)
+    # every one-value-per-lane class from 4 limbs and the first multi-lane
+    # class (its programs evaluated here one value per thread)
+    assert {4, 8, 16, 32, 64} <= widths

@@ -63,7 +63,12 @@ extern "C" int vmh_sum(const char* text, const char** names, const int* P, const
     case 4: run<4>(pr, Q, prog, rows, img); break;
     case 8: run<8>(pr, Q, prog, rows, img); break;
     case 16: run<16>(pr, Q, prog, rows, img); break;
-    default: run<32>(pr, Q, prog, rows, img); break;
+    case 32: run<32>(pr, Q, prog, rows, img); break;
+    // the multi-lane classes' programs, evaluated one value per thread at
+    // the same width: the reference their group evaluation must match
+    case 64: run<64>(pr, Q, prog, rows, img); break;
+    case 128: run<128>(pr, Q, prog, rows, img); break;
+    default: run<256>(pr, Q, prog, rows, img); break;
     }
     return 0;
 }

@@ -140,6 +140,8 @@ def _load():
         "updec_copy_kernel": (ctypes.c_int, [P, P, SZ, P]),
         "updec_malloc": (ctypes.c_int, [ctypes.POINTER(P), SZ]),
         "updec_free": (ctypes.c_int, [P]),
+        "updec_malloc_async": (ctypes.c_int, [ctypes.POINTER(P), SZ, P]),
+        "updec_free_async": (ctypes.c_int, [P, P]),
         "updec_memcpy_htod": (ctypes.c_int, [P, P, SZ, P]),
         "updec_memcpy_dtoh": (ctypes.c_int, [P, P, SZ, P]),
         "updec_memset": (ctypes.c_int, [P, ctypes.c_int, SZ, P]),
@@ -147,6 +149,7 @@ def _load():
         "updec_stream_destroy": (ctypes.c_int, [P]),
         "updec_stream_sync": (ctypes.c_int, [P]),
         "updec_device_sync": (ctypes.c_int, []),
+        "updec_mem_info": (ctypes.c_int, [ctypes.POINTER(I64), ctypes.POINTER(I64)]),
         "updec_device_info": (ctypes.c_int, [ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(I64),
                                              ctypes.c_char_p, SZ]),
         "updec_event_create": (ctypes.c_int, [ctypes.POINTER(P)]),
@@ -160,6 +163,8 @@ def _load():
                                               ctypes.POINTER(P)]),
         "updec_expr_info": (ctypes.c_int, [P, ctypes.POINTER(I32), ctypes.POINTER(I32), ctypes.POINTER(I32),
                                            ctypes.POINTER(I32)]),
+        "updec_expr_work": (ctypes.c_int, [P, ctypes.POINTER(I64)]),
+        "updec_mad_peak": (ctypes.c_int, [I32, I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "updec_expr_plan": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
         "updec_expr_source": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
         "updec_expr_source_iter": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
@@ -192,6 +197,7 @@ def _load():
         "updec_vm_free": (None, [P]),
         "updec_batch_create": (ctypes.c_int, [I32, I32, ctypes.POINTER(P)]),
         "updec_batch_tuning": (ctypes.c_int, [P, I32, I64, I32]),
+        "updec_batch_lanes": (ctypes.c_int, [P, I32, I32]),
         "updec_batch_add": (ctypes.c_int, [P, P, ctypes.POINTER(P), I64, P, P, ctypes.POINTER(I32)]),
         "updec_batch_pending": (ctypes.c_int, [P, ctypes.POINTER(I32)]),
         "updec_batch_launch": (ctypes.c_int, [P, ctypes.c_int, P]),
@@ -216,17 +222,17 @@ _lib = _load()
 
 # C ABI symbols declared in include/updec.h (checked by the CPU tests)
 EXPORTED = ["updec_last_error", "updec_version", "updec_init", "updec_compact_bytes", "updec_column_maxbits",
-            "updec_malloc", "updec_free", "updec_memcpy_htod", "updec_memcpy_dtoh", "updec_memset",
+            "updec_malloc", "updec_free", "updec_malloc_async", "updec_free_async", "updec_memcpy_htod", "updec_memcpy_dtoh", "updec_memset",
             "updec_stream_create", "updec_stream_destroy", "updec_stream_sync", "updec_device_sync",
-            "updec_device_info", "updec_event_create", "updec_event_record", "updec_event_elapsed",
+            "updec_device_info", "updec_mem_info", "updec_event_create", "updec_event_record", "updec_event_elapsed",
             "updec_event_destroy", "updec_event_query", "updec_event_sync", "updec_copy_kernel",
-            "updec_expr_compile", "updec_expr_info", "updec_expr_plan", "updec_expr_source", "updec_expr_calc",
+            "updec_expr_compile", "updec_expr_info", "updec_expr_plan", "updec_expr_work", "updec_mad_peak", "updec_expr_source", "updec_expr_calc",
             "updec_expr_free", "updec_agg_compile", "updec_agg_run", "updec_agg_last_kernel_ms", "updec_agg_source",
             "updec_agg_free", "updec_div_int64", "updec_segsum_compact", "updec_segsum_compact_noalloc",
             "updec_segsum_noalloc_max_segments", "updec_compile", "updec_launch",
             "updec_module_free", "updec_check_compile", "updec_module_cache_stats", "updec_gen_compact", "updec_gen_int32", "updec_gen_byte",
             "updec_vm_compile", "updec_vm_info", "updec_vm_listing", "updec_vm_free", "updec_batch_create",
-            "updec_batch_tuning", "updec_batch_add", "updec_batch_pending", "updec_batch_launch", "updec_batch_free",
+            "updec_batch_tuning", "updec_batch_lanes", "updec_batch_add", "updec_batch_pending", "updec_batch_launch", "updec_batch_free",
             "updec_parse_compact", "updec_format_stride", "updec_format_compact", "updec_expr_calc_blocks",
             "updec_agg_run_blocks", "updec_expr_calc_iter", "updec_agg_run_iter", "updec_expr_source_iter",
             "updec_agg_source_iter"]
@@ -279,12 +285,19 @@ class Stream:
 
 
 class DeviceBuffer:
-    """Device memory owned by this object (hipMalloc through the library)."""
+    """Device memory owned by this object (hipMalloc through the library;
+    with ordered=True stream-ordered hipMallocAsync / hipFreeAsync on
+    `stream`, which frees without a device-wide synchronization)."""
 
-    def __init__(self, nbytes):
+    def __init__(self, nbytes, ordered=False, stream=None):
         self.nbytes = int(nbytes)
+        self.ordered = ordered
+        self.stream = stream
         p = ctypes.c_void_p()
-        _check(_lib.updec_malloc(ctypes.byref(p), max(4, self.nbytes)))
+        if ordered:
+            _check(_lib.updec_malloc_async(ctypes.byref(p), max(4, self.nbytes), _stream(stream)))
+        else:
+            _check(_lib.updec_malloc(ctypes.byref(p), max(4, self.nbytes)))
         self.ptr = p.value
 
     @classmethod
@@ -311,7 +324,10 @@ class DeviceBuffer:
 
     def __del__(self):
         if getattr(self, "ptr", None):
-            _lib.updec_free(ctypes.c_void_p(self.ptr))
+            if getattr(self, "ordered", False):
+                _lib.updec_free_async(ctypes.c_void_p(self.ptr), _stream(self.stream))
+            else:
+                _lib.updec_free(ctypes.c_void_p(self.ptr))
             self.ptr = None
 
 
@@ -349,6 +365,22 @@ def copy_kernel(dst, src, nbytes, stream=None):
 
 def device_sync():
     _check(_lib.updec_device_sync())
+
+
+def mad_peak(blocks=None, iters=20000):
+    """measured issue rate of 32 x 32 -> 64 multiply-adds: (ms, MACs, MACs/s)"""
+    if blocks is None:
+        blocks = device_info()["cus"] * 8
+    ms, macs = ctypes.c_double(), ctypes.c_double()
+    _check(_lib.updec_mad_peak(blocks, iters, ctypes.byref(ms), ctypes.byref(macs)))
+    return ms.value, macs.value, macs.value / (ms.value / 1000.0)
+
+
+def mem_info():
+    """(free, total) device memory in bytes (hipMemGetInfo)"""
+    f, t = ctypes.c_int64(), ctypes.c_int64()
+    _check(_lib.updec_mem_info(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
 
 
 def device_info():
@@ -392,6 +424,13 @@ class Expr:
     @property
     def plan(self):
         return _text(_lib.updec_expr_plan, self._h)
+
+    @property
+    def limb_products(self):
+        """algorithmic 32 x 32 -> 64 limb products per row"""
+        v = ctypes.c_int64()
+        _check(_lib.updec_expr_work(self._h, ctypes.byref(v)))
+        return v.value
 
     @property
     def source(self):
@@ -629,7 +668,8 @@ def gen_byte_device(seed, column_id, choices, rows, row0=0, stream=None):
     return out
 
 
-VM_CLASS_WIDTHS = (2, 4, 8, 16, 32)
+VM_CLASS_WIDTHS = (2, 4, 8, 16, 32, 64, 128, 256, 512)
+VM_FIRST_MULTI_LANE = 5   # classes from here spread a value over W / 4 or W / 8 lanes
 EFULL = 5   # UPDEC_EFULL: the batch has no room for the query
 
 
@@ -676,8 +716,10 @@ class Batch:
         self._h = h
         self._keep = []
 
-    def tune(self, width_class, rows_per_item, blocks_per_cu):
+    def tune(self, width_class, rows_per_item, blocks_per_cu, lanes=None):
         _check(_lib.updec_batch_tuning(self._h, width_class, rows_per_item, blocks_per_cu))
+        if lanes is not None and width_class >= VM_FIRST_MULTI_LANE:
+            _check(_lib.updec_batch_lanes(self._h, width_class, lanes))
 
     def add(self, prog, columns, rows, image, count, image_offset=0, count_offset=0):
         """columns: DeviceBuffers in descriptor order; image: 133 bytes at

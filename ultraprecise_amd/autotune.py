@@ -1,15 +1,31 @@
 """Autotuner for the interpreter's persistent kernels.
 
-Per register-width class (W = 2, 4, 8, 16, 32 limbs) it times a
-representative query of that width -- a seeded serving query whose program
-lands in the class -- over every candidate (rows per work item, resident
-blocks per compute unit) and keeps the fastest.  Results are cached per
-device name in a JSON file so a process pays the search once.
+Per register-width class (W = 2 .. 32 limbs one value per lane, W = 64 ..
+512 limbs one value per group of lanes) it times representative queries of
+that width -- seeded serving queries whose programs land in the class --
+over every candidate (rows per work item, resident blocks per compute unit,
+and for the multi-lane classes lanes per value: W / 4 or W / 8, at most 64)
+and keeps the fastest.  Results are cached per device name in a JSON file
+so a process pays the search once.
 """
 import json
 import os
 
 DEFAULT_CANDIDATES = ((1024, 4), (2048, 4), (4096, 2), (4096, 4), (8192, 4), (8192, 8), (16384, 4), (32768, 4))
+MULTI_LANE_ITEMS = ((64, 4), (128, 4), (256, 2), (256, 4), (512, 4))
+
+
+def class_rows(width, rows):
+    """rows timed per query of a class: fewer for the costly wide classes"""
+    return rows if width <= 32 else max(20_000, rows * 32 // (width * 8))
+
+
+def candidates_for(U, cls, candidates):
+    w = U.VM_CLASS_WIDTHS[cls]
+    if cls < U.VM_FIRST_MULTI_LANE:
+        return [(rpi, bpc) for rpi, bpc in candidates]
+    lanes = sorted({min(64, w // 4), w // 8})
+    return [(rpi, bpc, ln) for rpi, bpc in MULTI_LANE_ITEMS for ln in lanes if ln <= 64]
 
 
 def cache_path(device_name):
@@ -18,21 +34,15 @@ def cache_path(device_name):
     return os.path.join(base, f"vm_tuning_{safe}.json")
 
 
-def representative_queries(U, W, per_class=2, seed=77, max_draws=4000):
-    """seeded serving queries, `per_class` of each width class"""
-    def ok(text, cols):
-        try:
-            U.VmProgram(text, cols)
-            return True
-        except U.UpdecError as e:
-            if "exceed" in str(e):
-                return False
-            raise
-
+def representative_queries(U, W, per_class=2, seed=77, max_draws=12000):
+    """seeded serving queries, `per_class` of each width class (as drawn)"""
     found = {c: [] for c in range(len(U.VM_CLASS_WIDTHS))}
     for i in range(max_draws):
-        q = W.random_query(seed, i, ok)
-        cls = U.VmProgram(q["text"], q["cols"]).width_class
+        q = W.random_query(seed, i)
+        try:
+            cls = U.VmProgram(q["text"], q["cols"]).width_class
+        except U.UpdecError:
+            continue
         if len(found[cls]) < per_class:
             found[cls].append(q)
         if all(len(v) >= per_class for v in found.values()):
@@ -41,7 +51,7 @@ def representative_queries(U, W, per_class=2, seed=77, max_draws=4000):
 
 
 def tune(U, rows=2_000_000, candidates=DEFAULT_CANDIDATES, reps=3, use_cache=True):
-    """returns {class: (rows_per_item, blocks_per_cu)}"""
+    """returns {class: (rows_per_item, blocks_per_cu[, lanes per value])}"""
     from . import workloads as W
     name = U.device_info()["name"]
     path = cache_path(name)
@@ -54,20 +64,21 @@ def tune(U, rows=2_000_000, candidates=DEFAULT_CANDIDATES, reps=3, use_cache=Tru
         if not qs:
             continue
         progs, cols = [], []
+        crows = class_rows(U.VM_CLASS_WIDTHS[cls], rows)
         for q in qs:
             progs.append(U.VmProgram(q["text"], q["cols"]))
             cols.append([U.gen_compact_device(U.gen_desc(U.GEN_DIGITS, q["P"], 77, 900 + k, digits=q["P"],
-                                                         neg_permille=500), rows) for k in range(len(q["cols"]))])
+                                                         neg_permille=500), crows) for k in range(len(q["cols"]))])
         img = U.DeviceBuffer(133 * len(qs))
         cnt = U.DeviceBuffer(8 * len(qs))
         best = None
-        for rpi, bpc in candidates:
+        for cand in candidates_for(U, cls, candidates):
             b = U.Batch(max_queries=len(qs), max_items=1 << 16)
-            b.tune(cls, rpi, bpc)
+            b.tune(cls, cand[0], cand[1], lanes=cand[2] if len(cand) > 2 else None)
 
             def launch():
                 for k, (p, c) in enumerate(zip(progs, cols)):
-                    if b.add(p, c, rows, img, cnt, image_offset=133 * k, count_offset=8 * k) is None:
+                    if b.add(p, c, crows, img, cnt, image_offset=133 * k, count_offset=8 * k) is None:
                         raise RuntimeError("tuning batch overflow")
                 b.launch(graph=False)
 
@@ -80,8 +91,8 @@ def tune(U, rows=2_000_000, candidates=DEFAULT_CANDIDATES, reps=3, use_cache=Tru
             e1.record()
             ms = e0.elapsed_ms(e1) / reps
             if best is None or ms < best[0]:
-                best = (ms, rpi, bpc)
-        result[cls] = (best[1], best[2])
+                best = (ms, cand)
+        result[cls] = tuple(best[1])
         del cols
     if use_cache:
         os.makedirs(os.path.dirname(path), exist_ok=True)

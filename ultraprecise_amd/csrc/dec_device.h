@@ -240,9 +240,18 @@ template <int W> UPD_DEV Mag<W> shr_var(const Mag<W>& a, u32 k)
 // Reciprocal of a normalized divisor word d (top bit set):
 // v = floor((2^64 - 1) / d) - 2^32  (Moller & Granlund, "Improved division
 // by invariant integers", IEEE Trans. Computers 60(2), 2011, section 3).
+// A correctly rounded double quotient 2^64 / d (in (2^32, 2^33], relative
+// error below 2^-53, so within 2^-20 of the true value, and 2^64 / d exceeds
+// (2^64 - 1) / d by less than 2^-31) truncates to V - 1, V or V + 1 for
+// V = floor((2^64 - 1) / d); the remainder 2^64 - 1 - c d, formed modulo
+// 2^64, is negative (top bit set: it lies in [-d, 0)) for V + 1 and at least
+// d for V - 1.  This replaces a 64-bit software division.
 UPD_DEV u32 recip_word(u32 d)
 {
-    return (u32)((~0ull) / (u64)d - (1ull << 32));
+    u64 c = (u64)(18446744073709551616.0 / (double)d);
+    const u64 r = ~0ull - c * (u64)d;
+    c = (i64)r < 0 ? c - 1u : (r >= (u64)d ? c + 1u : c);
+    return (u32)(c - (1ull << 32));
 }
 
 // 2-by-1 division with a precomputed reciprocal (Moller & Granlund 2011,
@@ -404,25 +413,6 @@ template <u32 D> UPD_DEV u32 div_word_const(u32& r, u32 w)
     return q;
 }
 
-// one pass: x = floor(x / D)
-template <int W, u32 D> UPD_DEV void div_pass(Mag<W>& x)
-{
-    u32 r = 0;
-#pragma unroll
-    for (int i = W - 1; i >= 0; i--) x.w[i] = div_word_const<D>(r, x.w[i]);
-}
-
-// floor(a / 5^N), a < 2^(32 W).  The full 5^13 passes run as a loop (one
-// copy of the unrolled limb body keeps generated code, and the hipRTC
-// compile time of long expressions, small); the last partial pass follows.
-template <int W, int N> UPD_DEV Mag<W> div_pow5(Mag<W> x)
-{
-#pragma unroll 1
-    for (int p = 0; p < N / 13; p++) div_pass<W, pow5_u32(13)>(x);
-    if constexpr (N % 13 != 0) div_pass<W, pow5_u32(N % 13)>(x);
-    return x;
-}
-
 // floor(a / 2^N) for a static N
 template <int W, int N> UPD_DEV Mag<W> shr_static(const Mag<W>& a)
 {
@@ -437,6 +427,212 @@ template <int W, int N> UPD_DEV Mag<W> shr_static(const Mag<W>& a)
     return r;
 }
 
+// one pass: x = floor(x / D)
+template <int W, u32 D> UPD_DEV void div_pass(Mag<W>& x)
+{
+    u32 r = 0;
+#pragma unroll
+    for (int i = W - 1; i >= 0; i--) x.w[i] = div_word_const<D>(r, x.w[i]);
+}
+
+// floor(a / 5^N), a < 2^(32 W), by passes of one-word division.  The full
+// 5^13 passes run as a loop (one copy of the unrolled limb body keeps
+// generated code, and the hipRTC compile time of long expressions, small);
+// the last partial pass follows.
+template <int W, int N> UPD_DEV Mag<W> div_pow5_passes(Mag<W> x)
+{
+#pragma unroll 1
+    for (int p = 0; p < N / 13; p++) div_pass<W, pow5_u32(13)>(x);
+    if constexpr (N % 13 != 0) div_pass<W, pow5_u32(N % 13)>(x);
+    return x;
+}
+
+// 5^n as 32-bit words, built at compile time (n <= 660)
+struct Pow5Const {
+    u32 w[48];
+    int n;
+    constexpr Pow5Const(int p) : w(), n(0)
+    {
+        w[0] = 1;
+        for (int k = 0; k < p; k++) {
+            u64 c = 0;
+            for (int i = 0; i < 48; i++) {
+                u64 t = (u64)w[i] * 5u + c;
+                w[i] = (u32)t;
+                c = t >> 32;
+            }
+        }
+        int t = 47;
+        while (t > 0 && w[t] == 0) t--;
+        n = t + 1;
+    }
+    constexpr int bits() const
+    {
+        int b = 0;
+        u32 x = w[n - 1];
+        while (x) { b++; x >>= 1; }
+        return 32 * (n - 1) + b;
+    }
+};
+
+template <int N> struct P5 {
+    static constexpr Pow5Const c = Pow5Const(N);
+    static constexpr int W = c.n;
+    static constexpr int BITS = c.bits();
+};
+
+// mu = floor(2^T / 5^N), the Barrett multiplier, by restoring division one
+// bit at a time at compile time (T < 32 * 64)
+struct Pow5Recip {
+    u32 w[64];
+    constexpr Pow5Recip(int T, int N) : w()
+    {
+        const Pow5Const d(N);
+        const int n = d.n + 1;  // limbs of the running remainder (< 2 * 5^N)
+        u32 r[49] = {};
+        for (int i = T; i >= 0; i--) {
+            // r = 2 r + (bit i of 2^T)
+            u32 c = i == T ? 1u : 0u;
+            for (int k = 0; k < n; k++) {
+                const u32 t = (r[k] << 1) | c;
+                c = r[k] >> 31;
+                r[k] = t;
+            }
+            int ge = 1;
+            for (int k = n - 1; k >= 0; k--) {
+                const u32 dk = k < d.n ? d.w[k] : 0u;
+                if (r[k] != dk) {
+                    ge = r[k] > dk;
+                    break;
+                }
+            }
+            if (ge) {
+                u64 b = 0;
+                for (int k = 0; k < n; k++) {
+                    const u64 dk = k < d.n ? d.w[k] : 0u;
+                    const u64 t = (u64)r[k] - dk - b;
+                    r[k] = (u32)t;
+                    b = (t >> 32) & 1u;
+                }
+                w[i >> 5] |= 1u << (i & 31);
+            }
+        }
+    }
+};
+
+template <int T, int N> struct P5Recip {
+    static constexpr Pow5Recip c = Pow5Recip(T, N);
+};
+
+// floor(x / 5^N) for x < 2^BI by Barrett reduction (Barrett, CRYPTO '86;
+// Menezes et al., Handbook of Applied Cryptography, 14.42) with a
+// compile-time multiplier: with 2^S <= 5^N < 2^(S+1), E = BI - S,
+// a = floor(x / 2^S) and mu = floor(2^BI / 5^N), both below 2^E, the estimate
+// q' = floor(a mu / 2^E) satisfies q - 2 <= q' <= q, since
+// a mu > (x / 2^S - 1)(2^BI / 5^N - 1) > 2^E (x / 5^N - 2).  The remainder
+// x - q' 5^N < 3 * 5^N is formed in its low limbs only, and at most two
+// conditional subtractions of 5^N finish the quotient.
+template <int WM, int BI, int N> UPD_DEV Mag<WM> div_pow5_barrett(const Mag<WM>& x)
+{
+    constexpr int S = P5<N>::BITS - 1;
+    if constexpr (BI <= S) {
+        return mzero<WM>();
+    } else {
+        constexpr int E = BI - S;
+        constexpr int LQ = (E + 31) / 32;
+        constexpr Pow5Recip mu = P5Recip<BI, N>::c;
+        constexpr Pow5Const d = P5<N>::c;
+        constexpr int WD = P5<N>::W;
+        const Mag<LQ> a = resize<LQ>(shr_static<WM, S>(x));
+        // a * mu, then the bits from E up
+        Mag<2 * LQ> p = mzero<2 * LQ>();
+#pragma unroll
+        for (int i = 0; i < LQ; i++) {
+            u64 c = 0;
+#pragma unroll
+            for (int j = 0; j < LQ; j++) {
+                const u64 t = (u64)a.w[i] * mu.w[j] + p.w[i + j] + c;
+                p.w[i + j] = (u32)t;
+                c = t >> 32;
+            }
+            p.w[i + LQ] = (u32)c;
+        }
+        Mag<LQ> q = resize<LQ>(shr_static<2 * LQ, E>(p));
+        // remainder in WR limbs (it is below 3 * 5^N < 2^(S + 3))
+        constexpr int WR0 = (S + 3 + 31) / 32;
+        constexpr int WR = WR0 < WM ? WR0 : WM;
+        Mag<WR> qd = mzero<WR>();
+#pragma unroll
+        for (int i = 0; i < LQ; i++) {
+            if (i >= WR) break;
+            u64 c = 0;
+#pragma unroll
+            for (int j = 0; j < WD; j++) {
+                if (i + j < WR) {
+                    const u64 t = (u64)q.w[i] * d.w[j] + qd.w[i + j] + c;
+                    qd.w[i + j] = (u32)t;
+                    c = t >> 32;
+                }
+            }
+            if (i + WD < WR) qd.w[i + WD] = (u32)c;
+        }
+        Mag<WR> r;
+        {
+            u64 b = 0;
+#pragma unroll
+            for (int k = 0; k < WR; k++) {
+                const u64 t = (u64)x.w[k] - qd.w[k] - b;
+                r.w[k] = (u32)t;
+                b = (t >> 32) & 1u;
+            }
+        }
+#pragma unroll
+        for (int pass = 0; pass < 2; pass++) {
+            // r >= 5^N: subtract it and count one more
+            u64 b = 0;
+            Mag<WR> t;
+#pragma unroll
+            for (int k = 0; k < WR; k++) {
+                const u64 v = (u64)r.w[k] - (k < WD ? d.w[k] : 0u) - b;
+                t.w[k] = (u32)v;
+                b = (v >> 32) & 1u;
+            }
+            const bool ge = b == 0;
+#pragma unroll
+            for (int k = 0; k < WR; k++) r.w[k] = ge ? t.w[k] : r.w[k];
+            u32 cy = ge ? 1u : 0u;
+#pragma unroll
+            for (int k = 0; k < LQ; k++) {
+                const u32 s = q.w[k] + cy;
+                cy = s < cy ? 1u : 0u;
+                q.w[k] = s;
+            }
+        }
+        return resize<WM>(q);
+    }
+}
+
+// instructions of the two forms, roughly: a pass costs about 12 per limb; a
+// Barrett reduction about 3 per limb product (LQ^2 for the estimate, about
+// WR * WD for the remainder) plus the corrections
+constexpr bool barrett_pays(int W, int BI, int N)
+{
+    if (N <= 13 || BI >= 32 * 64) return false;
+    const int S = Pow5Const(N).bits() - 1;
+    if (BI <= S) return true;
+    const int LQ = (BI - S + 31) / 32, WD = Pow5Const(N).n;
+    const int WR = (S + 34) / 32 < W ? (S + 34) / 32 : W;
+    return 3 * (LQ * LQ + WR * WD) + 8 * WR + 4 * LQ < 12 * W * ((N + 12) / 13);
+}
+
+// floor(a / 5^N) for a < 2^BI (a held in W limbs): Barrett reduction or
+// passes of one-word division, whichever issues fewer instructions
+template <int W, int BI, int N> UPD_DEV Mag<W> div_pow5(const Mag<W>& x)
+{
+    if constexpr (barrett_pays(W, BI, N)) return div_pow5_barrett<W, BI, N>(x);
+    else return div_pow5_passes<W, N>(x);
+}
+
 // cast to a smaller scale (decimal.cu:471-533, scale-down branch): the
 // magnitude is floor-divided by 10^N and the sign is kept, so a negative
 // value that truncates to zero stays -0
@@ -446,7 +642,7 @@ template <int WO, int WI, int N> UPD_DEV Dec<WO> dshr10(const Dec<WI>& a)
     constexpr int WM = BI <= 32 ? 1 : (BI + 31) / 32;
     const Mag<WM> t = resize<WM>(shr_static<WI, N>(a.m));
     Dec<WO> r;
-    r.m = resize<WO>(div_pow5<WM, N>(t));
+    r.m = resize<WO>(div_pow5<WM, (BI < 1 ? 1 : BI), N>(t));
     r.neg = a.neg;
     return r;
 }

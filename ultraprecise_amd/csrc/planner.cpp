@@ -1216,6 +1216,9 @@ struct Emit {
     // column-iterator inputs (IterArgs): column loads report NULL in a flag
     // and every plan keeps its own NULL flag <prefix>nn
     bool iter = false;
+    // constants read into the kernel's kcv<N> variables before its row loop
+    // (emit_const_prologue); else each use reads the constant array
+    bool kvars = false;
     std::string fresh(const char* p) { return std::string(p) + std::to_string(tmp++); }
 };
 
@@ -1265,9 +1268,12 @@ std::string emit_plan(Emit& E, const Plan& p, const std::vector<ColumnType>& col
         case PK::Const: {
             // sign and limbs from the launch's constant array (kernel data)
             const int kb = p.kbase + x.koff;
+            auto kw = [&](int i) {
+                return E.kvars ? "kcv" + std::to_string(i) : "kld(a.kc, " + std::to_string(i) + ")";
+            };
             E.os << ind << "Dec<" << x.W << "> " << v << ";\n";
-            E.os << ind << v << ".neg = a.kc[" << kb << "];\n";
-            for (int w = 0; w < x.W; w++) E.os << ind << v << ".m.w[" << w << "] = a.kc[" << kb + 1 + w << "];\n";
+            E.os << ind << v << ".neg = " << kw(kb) << ";\n";
+            for (int w = 0; w < x.W; w++) E.os << ind << v << ".m.w[" << w << "] = " << kw(kb + 1 + w) << ";\n";
             break;
         }
         case PK::Neg:
@@ -1388,6 +1394,71 @@ static int plan_ops(const Plan& p)
 
 bool plan_out_of_line(const Plan& p) { return plan_ops(p) > kOutOfLineOps || plan_cost(p) > kOutOfLineCost; }
 
+}  // namespace
+
+// Algorithmic work per row in 32 x 32 -> 64 limb products, at the plan's
+// static widths: a product Wa Wb; a division the scaling of the dividend by
+// 10^S (Wa W(10^S)) and Knuth D's (WX - Wb + 1) Wb; a MOD (Wa - Wb + 1) Wb;
+// an operand aligned up by 10^n W W(10^n); a cast or truncation down by 10^n
+// one word division per limb and 9 digits.
+int64_t plan_limb_products(const Plan& p)
+{
+    int64_t m = 0;
+    auto w10 = [](int n) { return (int64_t)words_for(pow10_bits(n) + 1); };
+    for (const PNode& x : p.n) {
+        const int64_t wa = x.a >= 0 ? p.n[x.a].W : 0, wb = x.b >= 0 ? p.n[x.b].W : 0;
+        switch (x.kind) {
+        case PK::Mul: m += wa * wb; break;
+        case PK::Div: {
+            const int S = p.n[x.b].S + 4;
+            const int64_t wx = words_for(p.n[x.a].bits + pow10_bits(S));
+            m += wa * w10(S) + std::max<int64_t>(1, wx - wb + 1) * wb;
+            break;
+        }
+        case PK::Mod: m += std::max<int64_t>(1, wa - wb + 1) * wb; break;
+        case PK::Add:
+        case PK::Sub: {
+            const PNode& A = p.n[x.a];
+            const PNode& B = p.n[x.b];
+            if (x.S > A.S) m += A.W * w10(x.S - A.S);
+            if (x.S > B.S) m += B.W * w10(x.S - B.S);
+            break;
+        }
+        case PK::Cast:
+        case PK::Trunc: {
+            const PNode& A = p.n[x.a];
+            if (x.S > A.S) m += wa * w10(x.S - A.S);
+            else if (x.S < A.S) m += wa * ((A.S - x.S + 8) / 9);
+            break;
+        }
+        default: break;
+        }
+    }
+    return m;
+}
+
+namespace {
+
+
+// Read the constants of the inline plans into kernel-scope variables
+// kcv<N> before the row loop, when there are few of them (they then stay in
+// scalar registers); a kernel with more reads them at each use.  The reads
+// go through the constant address space (kld), so they are scalar loads.
+constexpr int kMaxHoistedConstWords = 32;
+void emit_const_prologue(Emit& E, const std::vector<Plan>& plans, bool all_inline = false)
+{
+    auto inl = [&](const Plan& p) { return all_inline || !plan_out_of_line(p); };
+    int words = 0;
+    for (const Plan& p : plans)
+        if (inl(p)) words += p.kwords;
+    E.kvars = words <= kMaxHoistedConstWords;
+    if (!E.kvars) return;
+    for (const Plan& p : plans) {
+        if (!inl(p)) continue;
+        for (int w = 0; w < p.kwords; w++)
+            E.os << "  const u32 kcv" << p.kbase + w << " = kld(a.kc, " << p.kbase + w << ");\n";
+    }
+}
 
 // definition of the out-of-line evaluator `fname` (any args struct whose
 // first members are col[] and colbytes[])
@@ -1397,7 +1468,10 @@ void emit_plan_function(Emit& E, const Plan& p, const std::vector<ColumnType>& c
     E.os << "template <class A> __device__ __noinline__ Dec<" << W << "> " << fname << "(const A& a, usize row"
          << (E.iter ? ", u32& nn_out" : "") << ")\n{\n";
     std::vector<std::string> loaded(cols.size());
+    const bool kv = E.kvars;
+    E.kvars = false;   // the kernel's kcv<N> are not visible here
     std::string rv = emit_plan(E, p, cols, loaded, "  ", "");
+    E.kvars = kv;
     if (E.iter) E.os << "  nn_out = nn;\n";
     E.os << "  return " << rv << ";\n}\n";
 }
@@ -1428,6 +1502,10 @@ void emit_plan_functions(Emit& E, const std::vector<Plan>& plans, const std::vec
 // grid-stride step are then contiguous per XCD, and a cache line that
 // straddles two blocks' ranges is fetched into one L2, not two.
 const char* kArgsStruct =
+    // constants are read through the constant address space: the compiler
+    // may then use scalar loads for them
+    "__device__ __forceinline__ u32 kld(const u32* p, int i)\n"
+    "{ return ((const __attribute__((address_space(4))) u32*)p)[i]; }\n"
     "__device__ __forceinline__ usize xcd_block() {\n"
     "  const u32 g = gridDim.x, b = blockIdx.x;\n"
     "  return (g & 7u) ? (usize)b : (usize)((b & 7u) * (g >> 3) + (b >> 3));\n"
@@ -1729,6 +1807,7 @@ std::string gen_calc_kernel(const Plan& p, const std::vector<ColumnType>& cols, 
     E.os << "extern \"C\" __global__ void __launch_bounds__(" << T << ") "
          << (calc_one_wave(p) ? "__attribute__((amdgpu_waves_per_eu(1, 1))) " : "") << name << "("
          << (o.blocks ? "IterArgs" : "ColArgs") << " a)\n{\n";
+    emit_const_prologue(E, {p}, one_row);
     E.os << "  __shared__ u32 tile[(" << T << " * " << (o.blocks ? OB + 1 : OB) << " + 3) / 4];\n";
     E.os << "  u8* t8 = reinterpret_cast<u8*>(tile);\n";
     if (o.blocks) E.os << "  const usize os_ = " << OB << "u + (a.outnull ? 1u : 0u);\n";
@@ -2021,6 +2100,7 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
             "}\n";
     E.os << "extern \"C\" __global__ void __launch_bounds__(" << c.T << ") " << name
          << "_main(AggArgs a, u32* __restrict__ overflow)\n{\n";
+    emit_const_prologue(E, s.plans);
     E.os << "  const u32 tid = threadIdx.x;\n  const u32 wave = tid >> 6, lane = tid & 63u;\n";
     // the one-pass form accumulates with LDS atomics and reads its row
     // quads straight from memory.  Staging whole chunks through LDS (16-byte
@@ -2196,6 +2276,7 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
     E.os << "#define AW1 " << AW1 << "\n";
     E.os << "extern \"C\" __global__ void __launch_bounds__(" << c.T << ") " << name << "_main("
          << (o.blocks ? "IterArgs" : "AggArgs") << " a)\n{\n";
+    emit_const_prologue(E, s.plans);
     E.os << "  const u32 tid = threadIdx.x;\n  const u32 wave = tid >> 6, lane = tid & 63u;\n";
     const StagePlan sp = (reg && !o.blocks && agg_rows_per_step(s, cols, o) == 1) ? stage_plan(s, cols) : StagePlan();
     emit_stage_decl(E, sp, c.T / 64);
@@ -2452,12 +2533,20 @@ struct VmBuilder {
 bool build_vm_program(const Plan& p, const std::vector<ColumnType>& cols, VmProgram* out, std::string* err)
 {
     VmProgram prog;
-    // the register width holds every node's exact bound, so no value wraps
-    // and no intermediate (aligned operand, scaled dividend) is cut
+    // Extended mode: the register width holds every node's exact bound, so
+    // no value wraps and no intermediate (aligned operand, scaled dividend)
+    // is cut.  Faithful mode: the reference's 32-limb values wrap, as the
+    // JIT kernels' widths do (node bounds capped at 1024 bits); a division
+    // whose scaled dividend could pass 1024 bits stays on the JIT path.
+    const bool faithful = p.mode == Mode::Faithful;
     int words = 1;
     for (const PNode& x : p.n) {
-        words = std::max(words, words_for(x.ubits));
+        words = std::max(words, words_for(faithful ? x.bits : x.ubits));
         if (x.S > 65535) { *err = "scale out of the interpreter's range"; return false; }
+        if (faithful && x.kind == PK::Div && p.n[x.a].bits + pow10_bits(p.n[x.b].S + 4) > 1024) {
+            *err = "faithful division whose scaled dividend may pass 1024 bits (JIT path only)";
+            return false;
+        }
     }
     for (const PNode& x : p.n)
         if (x.kind == PK::Col) words = std::max(words, words_for(x.bits));
@@ -2465,11 +2554,12 @@ bool build_vm_program(const Plan& p, const std::vector<ColumnType>& cols, VmProg
     for (int c = 0; c < updec_vm::kNumClasses; c++)
         if (updec_vm::kClassW[c] >= words) { prog.wclass = c; break; }
     if (prog.wclass < 0) {
-        *err = "exact intermediates exceed the interpreter's 32-limb registers";
+        *err = "exact intermediates exceed the interpreter's 512-limb registers (" + std::to_string(words) +
+               " limbs needed)";
         return false;
     }
     prog.W = updec_vm::kClassW[prog.wclass];
-    prog.faithful = p.mode == Mode::Faithful;
+    prog.faithful = faithful;
     prog.outP = p.outP;
     prog.outS = p.outS;
     prog.outB = p.outB;

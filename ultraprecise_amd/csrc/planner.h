@@ -171,6 +171,9 @@ constexpr int kAggOnepassStatWords = 2 + 2048 + 2;
 // rows; the generated accumulator widths rely on it.
 constexpr int kAggRowsPerLaneLog2 = 14;
 
+// algorithmic 32 x 32 -> 64 limb products per row of a plan
+int64_t plan_limb_products(const Plan& p);
+
 // true when the calc kernel of this plan takes one row per thread with no
 // row loop (the launch needs ceil(rows / block) blocks)
 bool calc_one_row(const Plan& p);

@@ -13,10 +13,10 @@
 //   k_seg_main<W> one block per tile of 4096 positions: a tile inside one
 //                 segment (the common case for large groups) is summed in
 //                 registers, reduced over the wave and the block, and added
-//                 to the segment's accumulator once; a tile that crosses
-//                 segment starts is walked per lane, each lane adding its
-//                 running sum to a segment's accumulator when its segment
-//                 changes;
+//                 to the segment's accumulator once; in a tile that
+//                 crosses segment starts each lane walks a run of 16
+//                 consecutive positions, adding its running sum to a
+//                 segment's accumulator when its segment changes;
 //   k_seg_final<W> one lane per segment: carry-normalizes the accumulator
 //                 and writes the image.
 // An accumulator is AW = W + 2 words of 64 bits, one per 32-bit limb of a
@@ -126,14 +126,18 @@ template <int W> __global__ void __launch_bounds__(kSegThreads) k_seg_main(SegAr
         }
         return;
     }
-    // the tile crosses segment starts: every lane walks its positions in
-    // ascending order, so its segment only moves forward
-    i32 s = sf;
+    // the tile crosses segment starts: every lane takes a run of
+    // kSegRowsPerThread consecutive positions, so its segment changes only
+    // at the starts inside its run (a forward walk over the starts) and it
+    // adds its running sum to a segment once per segment it meets
+    const i64 r0 = lo + (i64)tid * kSegRowsPerThread;
+    const i64 r1 = r0 + kSegRowsPerThread < hi ? r0 + kSegRowsPerThread : hi;
+    if (r0 >= hi) return;
+    i32 s = seg_search(a.groups, sf < 0 ? 0 : sf, sl, r0);
     Mag<AW> acc = mzero<AW>();
-    for (i64 i = lo + tid; i < hi; i += kSegThreads) {
-        const i32 from = s < 0 ? 0 : s;
-        i32 si = seg_search(a.groups, from, sl, i);
-        if (si < from) si = s < 0 ? -1 : s;
+    for (i64 i = r0; i < r1; i++) {
+        i32 si = s;
+        while (si < sl && (i64)a.groups[si + 1] <= i) si++;
         if (si != s) {
             if (s >= 0) seg_flush<AW>(a.acc, s, acc);
             acc = mzero<AW>();

@@ -460,6 +460,30 @@ __global__ void __launch_bounds__(256) k_copy16(v4u* __restrict__ dst, const v4u
         __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
+// Issue-rate probe of the limb multiply: every lane runs 8 independent
+// chains of 32 x 32 -> 64 multiply-adds (v_mad_u64_u32), `iters` steps
+// each, and stores the sums so nothing is folded away.  8 * iters * lanes
+// multiply-adds per launch.
+__global__ void __launch_bounds__(256) k_mad_peak(u64* out, u32 iters, u32 seed)
+{
+    const u32 t = blockIdx.x * 256u + threadIdx.x;
+    u32 m[8];
+    u64 acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        m[k] = (t + 1u) * 2654435761u + (u32)k * 40503u + seed;
+        acc[k] = (u64)k;
+    }
+    for (u32 i = 0; i < iters; i++) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) acc[k] = (u64)m[k] * (u32)(acc[k] >> 32 | 1u) + (u32)acc[k];
+    }
+    u64 s = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) s ^= acc[k];
+    out[t] = s;
+}
+
 // largest magnitude bit length of a compact column (sign bit excluded)
 __global__ void __launch_bounds__(256) k_maxbits(const u8* col, int nbytes, int64_t rows, u32* out)
 {
@@ -634,6 +658,17 @@ int updec_free(void* d_ptr)
     if (d_ptr) HIPCHK(hipFree(d_ptr));
     return UPDEC_OK;
 }
+int updec_malloc_async(void** d_ptr, size_t bytes, updec_stream_t stream)
+{
+    if (!d_ptr) return fail(UPDEC_EINVAL, "null pointer");
+    HIPCHK(hipMallocAsync(d_ptr, bytes ? bytes : 4, S(stream)));
+    return UPDEC_OK;
+}
+int updec_free_async(void* d_ptr, updec_stream_t stream)
+{
+    if (d_ptr) HIPCHK(hipFreeAsync(d_ptr, S(stream)));
+    return UPDEC_OK;
+}
 int updec_memcpy_htod(void* d_dst, const void* h_src, size_t bytes, updec_stream_t stream)
 {
     if (bytes) HIPCHK(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, S(stream)));
@@ -685,6 +720,14 @@ int updec_device_info(int32_t* cus, int32_t* clock_khz, int64_t* mem_bytes, char
     if (name && name_cap) {
         snprintf(name, name_cap, "%s %s", p.name, p.gcnArchName);
     }
+    return UPDEC_OK;
+}
+int updec_mem_info(int64_t* free_bytes, int64_t* total_bytes)
+{
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = (int64_t)f;
+    if (total_bytes) *total_bytes = (int64_t)t;
     return UPDEC_OK;
 }
 int updec_event_create(void** ev)
@@ -744,6 +787,29 @@ int updec_copy_kernel(void* d_dst, const void* d_src, size_t bytes, updec_stream
     return UPDEC_OK;
 }
 
+int updec_mad_peak(int32_t blocks, int32_t iters, double* ms_out, double* macs_out)
+{
+    if (blocks < 1 || iters < 1 || !ms_out || !macs_out) return fail(UPDEC_EINVAL, "bad argument");
+    u64* d = nullptr;
+    HIPCHK(hipMalloc(reinterpret_cast<void**>(&d), (size_t)blocks * 256 * sizeof(u64)));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    hipLaunchKernelGGL(k_mad_peak, dim3(blocks), dim3(256), 0, nullptr, d, (u32)iters, 1u);   // warm-up
+    HIPCHK(hipEventRecord(e0, nullptr));
+    hipLaunchKernelGGL(k_mad_peak, dim3(blocks), dim3(256), 0, nullptr, d, (u32)iters, 2u);
+    HIPCHK(hipEventRecord(e1, nullptr));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipFree(d));
+    *ms_out = ms;
+    *macs_out = 8.0 * (double)iters * (double)blocks * 256.0;
+    return UPDEC_OK;
+}
+
 int updec_compact_bytes(int precision) { return compact_bytes(precision); }
 
 int updec_column_maxbits(const uint8_t* d_col, int precision, int64_t rows, int32_t* bits_out, updec_stream_t stream)
@@ -792,6 +858,13 @@ int updec_expr_info(updec_expr_t e, int32_t* P, int32_t* S_, int32_t* out_bytes,
     if (S_) *S_ = e->plan.outS;
     if (out_bytes) *out_bytes = e->plan.outB;
     if (faithful) *faithful = e->plan.mode == Mode::Faithful ? 1 : 0;
+    return UPDEC_OK;
+}
+
+int updec_expr_work(updec_expr_t e, int64_t* limb_products_per_row)
+{
+    if (!e || !limb_products_per_row) return fail(UPDEC_EINVAL, "bad argument");
+    *limb_products_per_row = plan_limb_products(e->plan);
     return UPDEC_OK;
 }
 

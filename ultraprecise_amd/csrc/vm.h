@@ -35,12 +35,17 @@ constexpr uint8_t kNoStore = 0xff;
 constexpr int kMaxRegs = 8;
 constexpr int kMaxCols = 8;
 constexpr int kMaxIns = 512;
-constexpr int kNumClasses = 5;                  // W = 2, 4, 8, 16, 32
-constexpr int kClassW[kNumClasses] = {2, 4, 8, 16, 32};
-constexpr int kMaxAccWords = 34;                // accumulator limbs of the widest class (W + 2)
+// W = 2 .. 32: one value per lane; W = 64 .. 512: one value per group of
+// lanes (dec_mlane.h), L limbs per lane, for extended-mode programs whose
+// exact intermediates need more than 32 limbs
+constexpr int kNumClasses = 9;
+constexpr int kFirstMultiLane = 5;
+constexpr int kClassW[kNumClasses] = {2, 4, 8, 16, 32, 64, 128, 256, 512};
+constexpr int kMaxAccWords = 514;               // accumulator limbs of the widest class (W + 2)
 // registers per class: the register file is held in LDS, so the wide
-// classes get fewer registers to keep four blocks resident per CU
-constexpr int kClassRegs[kNumClasses] = {8, 8, 8, 6, 4};
+// one-value-per-lane classes get fewer registers to keep four blocks
+// resident per CU; a multi-lane register is L + 1 words per lane
+constexpr int kClassRegs[kNumClasses] = {8, 8, 8, 6, 4, 8, 8, 8, 8};
 
 struct Operand {
     uint8_t kind;    // Kind

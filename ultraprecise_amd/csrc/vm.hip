@@ -22,6 +22,7 @@
 #include "../../include/updec.h"
 #include "api_internal.h"
 #include "dec_device.h"
+#include "dec_mlane.h"
 #include "planner.h"
 #include "vm.h"
 #include "vm_device.h"
@@ -37,9 +38,12 @@ namespace {
         if (e_ != hipSuccess) return api_fail(UPDEC_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kBlock[kNumClasses] = {256, 256, 128, 64, 64};
+constexpr int kBlock[kNumClasses] = {256, 256, 128, 64, 64, 128, 128, 128, 128};
 
-template <int W> constexpr int class_of() { return W == 2 ? 0 : W == 4 ? 1 : W == 8 ? 2 : W == 16 ? 3 : 4; }
+template <int W> constexpr int class_of()
+{
+    return W == 2 ? 0 : W == 4 ? 1 : W == 8 ? 2 : W == 16 ? 3 : W == 32 ? 4 : W == 64 ? 5 : W == 128 ? 6 : W == 256 ? 7 : 8;
+}
 
 template <int W, int T>
 __global__ void __launch_bounds__(T)
@@ -100,6 +104,208 @@ __global__ void __launch_bounds__(T)
     }
 }
 
+// ---------------------------------------------------------------- multi-lane classes
+
+// the lane's slice of register i: L limbs and the sign, in LDS at
+// [register][word][lane] like RegFile
+template <int L, int STRIDE> struct MLRegFile {
+    u32* base;
+    UPD_DEV ml::MDec<L> get(u32 i) const
+    {
+        const u32* p = base + i * (u32)((L + 1) * STRIDE);
+        ml::MDec<L> r;
+#pragma unroll
+        for (int w = 0; w < L; w++) r.m.w[w] = p[w * STRIDE];
+        r.neg = p[L * STRIDE];
+        return r;
+    }
+    UPD_DEV void set(u32 i, const ml::MDec<L>& v)
+    {
+        u32* p = base + i * (u32)((L + 1) * STRIDE);
+#pragma unroll
+        for (int w = 0; w < L; w++) p[w * STRIDE] = v.m.w[w];
+        p[L * STRIDE] = v.neg;
+    }
+};
+
+template <int L, int TPI, class RF>
+UPD_DEV ml::MDec<L> fetch_ml(u32 ow, const Query* q, const u32* cst, const RF& R, const ml::MDec<L>& last, usize row,
+                             const ml::Grp<TPI>& g)
+{
+    constexpr int W = L * TPI;
+    const u32 kind = ow & 0xffu, idx = (ow >> 8) & 0xffu, shift = ow >> 16;
+    ml::MDec<L> v;
+    if (kind == K_LAST) {
+        v = last;
+    } else if (kind == K_REG) {
+        v = R.get(idx);
+    } else if (kind == K_COL) {
+        const u32 B = q->colbytes[idx];
+        v = ml::load_compact<L, TPI>(q->col[idx], row, B, (usize)q->rows * B, g);
+    } else {
+        // constants at the class width: {sign, W limbs}
+        const u32* k = cst + idx * (u32)(W + 1);
+        v.neg = k[0];
+#pragma unroll
+        for (int w = 0; w < L; w++) v.m.w[w] = k[1 + g.rank * L + w];
+    }
+    if (shift) ml::mul_p10<L, TPI>(v.m, shift, g);
+    return v;
+}
+
+// the register program of one row, evaluated by the row's group of TPI lanes
+// (extended mode: multi-lane classes hold no faithful programs)
+template <int L, int TPI, class RF>
+UPD_DEV ml::MDec<L> vm_eval_ml(const Query* q, const u32* prog, const u32* cst, usize row, RF& R, const ml::Grp<TPI>& g)
+{
+    const u32 n = q->nins;
+    ml::MDec<L> last;
+    last.m = ml::mzero<L>();
+    last.neg = 0u;
+    for (u32 pc = 0; pc < n; pc++) {
+        const u32 w0 = prog[3 * pc], w1 = prog[3 * pc + 1], w2 = prog[3 * pc + 2];
+        const u32 op = w0 & 0xffu, dst = (w0 >> 8) & 0xffu, arg = w0 >> 16;
+        const ml::MDec<L> a = fetch_ml<L, TPI, RF>(w1, q, cst, R, last, row, g);
+        ml::MDec<L> r;
+        if (op == OP_MOV) {
+            r = a;
+        } else if (op == OP_NEG) {
+            r = a;
+            r.neg ^= 1u;
+        } else if (op == OP_UP) {
+            r = a;
+            ml::mul_p10<L, TPI>(r.m, arg, g);
+        } else if (op == OP_DOWN) {
+            r = a;
+            ml::div_p10<L, TPI>(r.m, arg, g);
+        } else {
+            const ml::MDec<L> b = fetch_ml<L, TPI, RF>(w2, q, cst, R, last, row, g);
+            if (op == OP_ADD) {
+                r = ml::add<L, TPI>(a, b, g);
+            } else if (op == OP_SUB) {
+                r = ml::sub<L, TPI>(a, b, g);
+            } else if (op == OP_MUL) {
+                r.m = ml::mul<L, TPI>(a.m, b.m, g);
+                r.neg = a.neg ^ b.neg;
+            } else {
+                r = ml::div<L, TPI>(a, b, arg, g);
+            }
+        }
+        if (dst != kNoStore) R.set(dst, r);
+        last = r;
+    }
+    return q->root == kNoStore ? last : R.get(q->root);
+}
+
+// Persistent kernel of a multi-lane class: blocks take work items as
+// k_vm_sum does; within an item the block's T / TPI groups take rows in
+// turn, each group evaluating its row's program and folding the stored
+// value into its two's complement accumulator; the block then adds its
+// groups' accumulators column by column and writes the item's partial.
+template <int L, int TPI, int T>
+__global__ void __launch_bounds__(T)
+    k_vm_sum_ml(Header* h, const Query* qs, const u32* prog, const u32* cst, const Item* items, u32* partials)
+{
+    constexpr int W = L * TPI;
+    constexpr int CLS = class_of<W>();
+    constexpr int AW = W + 2;
+    constexpr int NG = T / TPI;
+    __shared__ u32 s_item;
+    __shared__ u32 regs[kClassRegs[CLS] * (L + 1) * T];
+    __shared__ u32 gacc[NG * AW];
+    __shared__ u64 csum[AW];
+    const u32 tid = threadIdx.x;
+    const ml::Grp<TPI> g;
+    const u32 grp = tid / (u32)TPI;
+    MLRegFile<L, T> R{regs + tid};
+    const u32 n = h->class_nitems[CLS];
+    const u32 i0 = h->class_item0[CLS];
+    for (;;) {
+        // as in k_vm_sum: the barrier keeps thread 0's item fetch apart from
+        // its partial write of the previous iteration
+        __syncthreads();
+        if (tid == 0) s_item = atomicAdd(&h->counter[CLS], 1u);
+        __syncthreads();
+        const u32 it = __builtin_amdgcn_readfirstlane(s_item);
+        if (it >= n) break;
+        const Item I = items[i0 + it];
+        const Query* q = qs + I.query;
+        const u32* qp = prog + 3u * q->ins_off;
+        const u32* qc = cst + q->const_off;
+        const u32 ob = q->outB;
+        ml::Acc<L> acc = ml::acc_zero<L>();
+        for (usize row = I.row0 + grp; row < I.row1; row += NG) {
+            const ml::MDec<L> v = vm_eval_ml<L, TPI>(q, qp, qc, row, R, g);
+            ml::acc_add<L, TPI>(acc, ml::stored<L, TPI>(v, ob, g), g);
+        }
+#pragma unroll
+        for (int w = 0; w < L; w++) gacc[grp * AW + g.rank * L + w] = acc.m.w[w];
+        if (g.rank == TPI - 1) {
+            gacc[grp * AW + W] = acc.x0;
+            gacc[grp * AW + W + 1] = acc.x1;
+        }
+        __syncthreads();
+        for (u32 w = tid; w < (u32)AW; w += T) {
+            u64 s = 0;
+            for (u32 k = 0; k < (u32)NG; k++) s += gacc[k * AW + w];
+            csum[w] = s;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            u32* dst = partials + (usize)(i0 + it) * kMaxAccWords;
+            u64 c = 0;
+            for (int w = 0; w < AW; w++) {
+                const u64 t = csum[w] + c;
+                dst[w] = (u32)t;
+                c = t >> 32;
+            }
+        }
+    }
+}
+
+// SUM image and COUNT of a query of a multi-lane class: its items'
+// partials added column by column (64-bit words, no carries), then one
+// carry pass; the image keeps the low 32 limbs of the magnitude
+template <int AW> __device__ void final_query_wide(const Query* q, const u32* partials)
+{
+    __shared__ u64 cs[AW];
+    __shared__ u32 mag[AW];
+    for (u32 w = threadIdx.x; w < (u32)AW; w += 64) {
+        u64 s = 0;
+        for (u32 k = 0; k < q->nitems; k++) s += partials[(usize)(q->item0 + k) * kMaxAccWords + w];
+        cs[w] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        u64 c = 0;
+        for (int w = 0; w < AW; w++) {
+            const u64 t = cs[w] + c;
+            mag[w] = (u32)t;
+            c = t >> 32;
+        }
+        const bool neg = (mag[AW - 1] >> 31) != 0;
+        // magnitude of the two's complement value
+        u64 cc = neg ? 1u : 0u;
+        const u32 fm = neg ? 0xffffffffu : 0u;
+        u8* o = q->image;
+        o[0] = 1;
+        o[1] = neg ? 1 : 0;
+        o[2] = (u8)(q->outP & 0xffu);
+        o[3] = (u8)(q->outS & 0xffu);
+        o[4] = 0;
+        for (int w = 0; w < 32; w++) {
+            const u64 t = (u64)(mag[w] ^ fm) + cc;
+            const u32 x = (u32)t;
+            cc = t >> 32;
+            o[5 + 4 * w + 0] = (u8)x;
+            o[5 + 4 * w + 1] = (u8)(x >> 8);
+            o[5 + 4 * w + 2] = (u8)(x >> 16);
+            o[5 + 4 * w + 3] = (u8)(x >> 24);
+        }
+        *q->count = (int64_t)q->rows;
+    }
+}
+
 template <int AW> __device__ void final_query(const Query* q, const u32* partials)
 {
     Mag<AW> acc = mzero<AW>();
@@ -127,7 +333,11 @@ __global__ void __launch_bounds__(64) k_vm_final(const Header* h, const Query* q
     case 1: final_query<6>(q, partials); break;
     case 2: final_query<10>(q, partials); break;
     case 3: final_query<18>(q, partials); break;
-    default: final_query<34>(q, partials); break;
+    case 4: final_query<34>(q, partials); break;
+    case 5: final_query_wide<66>(q, partials); break;
+    case 6: final_query_wide<130>(q, partials); break;
+    case 7: final_query_wide<258>(q, partials); break;
+    default: final_query_wide<514>(q, partials); break;
     }
 }
 
@@ -138,6 +348,7 @@ struct Pending {
     const uint8_t* cols[kMaxCols];
     uint32_t colbytes[kMaxCols];
     int64_t rows;
+    int64_t chunk;  // rows per work item of this query
     uint8_t* image;
     int64_t* count;
 };
@@ -163,8 +374,10 @@ struct updec_vm_s {
 struct updec_batch_s {
     int max_queries = 0, max_items = 0, max_prog_words = 0, max_const_words = 0;
     // defaults sized so one 10^7-row query alone makes >= 4 items per CU
-    int64_t rows_per_item[kNumClasses] = {8192, 8192, 4096, 2048, 1024};
-    int blocks_per_cu[kNumClasses] = {4, 4, 4, 4, 4};
+    int64_t rows_per_item[kNumClasses] = {8192, 8192, 4096, 2048, 1024, 512, 256, 128, 64};
+    int blocks_per_cu[kNumClasses] = {4, 4, 4, 4, 4, 4, 4, 4, 4};
+    // lanes per value of the multi-lane classes: W / 4 (4 limbs per lane) or W / 8 (8 limbs)
+    int lanes[kNumClasses] = {1, 1, 1, 1, 1, 16, 32, 64, 64};
     Header* h_hdr = nullptr;
     Query* h_q = nullptr;
     Item* h_items = nullptr;
@@ -233,7 +446,13 @@ struct updec_batch_s {
             case 1: hipLaunchKernelGGL((k_vm_sum<4, kBlock[1]>), dim3(g), dim3(kBlock[1]), 0, st, d_hdr, d_q, d_prog, d_cst, d_items, d_partials); break;
             case 2: hipLaunchKernelGGL((k_vm_sum<8, kBlock[2]>), dim3(g), dim3(kBlock[2]), 0, st, d_hdr, d_q, d_prog, d_cst, d_items, d_partials); break;
             case 3: hipLaunchKernelGGL((k_vm_sum<16, kBlock[3]>), dim3(g), dim3(kBlock[3]), 0, st, d_hdr, d_q, d_prog, d_cst, d_items, d_partials); break;
-            default: hipLaunchKernelGGL((k_vm_sum<32, kBlock[4]>), dim3(g), dim3(kBlock[4]), 0, st, d_hdr, d_q, d_prog, d_cst, d_items, d_partials); break;
+            case 4: hipLaunchKernelGGL((k_vm_sum<32, kBlock[4]>), dim3(g), dim3(kBlock[4]), 0, st, d_hdr, d_q, d_prog, d_cst, d_items, d_partials); break;
+#define UPD_ML(Lx, TPIx, C) hipLaunchKernelGGL((k_vm_sum_ml<Lx, TPIx, kBlock[C]>), dim3(g), dim3(kBlock[C]), 0, st, d_hdr, d_q, d_prog, d_cst, d_items, d_partials)
+            case 5: if (lanes[5] == 8) UPD_ML(8, 8, 5); else UPD_ML(4, 16, 5); break;
+            case 6: if (lanes[6] == 16) UPD_ML(8, 16, 6); else UPD_ML(4, 32, 6); break;
+            case 7: if (lanes[7] == 32) UPD_ML(8, 32, 7); else UPD_ML(4, 64, 7); break;
+            default: UPD_ML(8, 64, 8); break;   // 512 limbs: a whole wave, 8 limbs per lane
+#undef UPD_ML
             }
             VMCHK(hipGetLastError());
             return UPDEC_OK;
@@ -318,7 +537,7 @@ int updec_batch_create(int32_t max_queries, int32_t max_items, updec_batch_t* ou
     b->max_queries = max_queries;
     b->max_items = max_items;
     b->max_prog_words = max_queries * 3 * 64;
-    b->max_const_words = max_queries * 8 * (32 + 1);
+    b->max_const_words = max_queries * 8 * (512 + 1);
     int rc;
     if ((rc = host_alloc(&b->h_hdr, 1)) || (rc = host_alloc(&b->h_q, max_queries)) || (rc = host_alloc(&b->h_items, max_items)) ||
         (rc = host_alloc(&b->h_prog, b->max_prog_words)) || (rc = host_alloc(&b->h_cst, b->max_const_words)) ||
@@ -353,6 +572,23 @@ int updec_batch_tuning(updec_batch_t b, int32_t width_class, int64_t rows_per_it
     return UPDEC_OK;
 }
 
+int updec_batch_lanes(updec_batch_t b, int32_t width_class, int32_t lanes)
+{
+    if (!b || width_class < kFirstMultiLane || width_class >= kNumClasses)
+        return api_fail(UPDEC_EINVAL, "lanes per value apply to the multi-lane classes (W = 64 .. 512)");
+    const int W = kClassW[width_class];
+    if ((lanes != W / 4 && lanes != W / 8) || lanes > 64)
+        return api_fail(UPDEC_EINVAL, "lanes per value must be W / 4 or W / 8, at most 64");
+    std::lock_guard<std::mutex> g(b->mu);
+    if (!b->pending.empty()) return api_fail(UPDEC_EINVAL, "tuning changes while queries are staged");
+    b->lanes[width_class] = lanes;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return UPDEC_OK;
+}
+
 int updec_batch_add(updec_batch_t b, updec_vm_t v, const uint8_t* const* d_cols, int64_t rows, uint8_t* d_image,
                     int64_t* d_count, int32_t* index_out)
 {
@@ -360,7 +596,10 @@ int updec_batch_add(updec_batch_t b, updec_vm_t v, const uint8_t* const* d_cols,
         return api_fail(UPDEC_EINVAL, "bad argument");
     std::lock_guard<std::mutex> g(b->mu);
     const VmProgram& pr = *v->prog;
-    const int64_t ch = b->rows_per_item[pr.wclass];
+    // the class's tuned rows per item, raised (in steps of 64 rows) for a
+    // query that would otherwise need more items than the whole batch holds
+    int64_t ch = b->rows_per_item[pr.wclass];
+    if ((rows + ch - 1) / ch > b->max_items) ch = ((rows + b->max_items - 1) / b->max_items + 63) / 64 * 64;
     const uint32_t items = (uint32_t)std::max<int64_t>(1, (rows + ch - 1) / ch);
     const bool new_prog = b->pend_programs.find(&pr) == b->pend_programs.end();
     const uint32_t pw = new_prog ? 3u * (uint32_t)pr.ins.size() : 0u, cw = new_prog ? (uint32_t)pr.consts.size() : 0u;
@@ -377,6 +616,7 @@ int updec_batch_add(updec_batch_t b, updec_vm_t v, const uint8_t* const* d_cols,
         p.colbytes[c] = v->colbytes[c];
     }
     p.rows = rows;
+    p.chunk = ch;
     p.image = d_image;
     p.count = d_count;
     b->pending.push_back(p);
@@ -411,7 +651,7 @@ int updec_batch_launch(updec_batch_t b, int use_graph, updec_stream_t stream)
     uint32_t nitems[kNumClasses] = {};
     for (const Pending& p : b->pending) {
         const int c = p.prog->wclass;
-        nitems[c] += (uint32_t)std::max<int64_t>(1, (p.rows + b->rows_per_item[c] - 1) / b->rows_per_item[c]);
+        nitems[c] += (uint32_t)std::max<int64_t>(1, (p.rows + p.chunk - 1) / p.chunk);
     }
     uint32_t total = 0, first[kNumClasses];
     for (int c = 0; c < kNumClasses; c++) {
@@ -473,7 +713,7 @@ int updec_batch_launch(updec_batch_t b, int use_graph, updec_stream_t stream)
         Q.image = p.image;
         Q.count = p.count;
         const int c = pr.wclass;
-        const int64_t ch = b->rows_per_item[c];
+        const int64_t ch = p.chunk;
         const uint32_t n = (uint32_t)std::max<int64_t>(1, (p.rows + ch - 1) / ch);
         Q.item0 = fill[c];
         Q.nitems = n;

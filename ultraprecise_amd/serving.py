@@ -1,11 +1,12 @@
 """Serving loop for decimal queries on one GPU.
 
 Queries (an expression over 2..4 compact columns, then SUM) arrive over
-time.  Each query is admitted when its columns are resident in device
-memory (a column store with a byte budget: columns are generated on the
-device at first use and the least recently used ones not held by a running
-query are evicted to make room), compiled for the precompiled interpreter
-(host only, no hipRTC), and then either
+time.  Each query is admitted when its columns fit in device memory --
+FreshColumns: the query's own columns are generated on the device when the
+free device memory (hipMemGetInfo) minus a reserve holds them, and freed
+when it completes; ColumnStore: a resident column pool with a byte budget,
+least recently used columns not held by a running query evicted -- then
+compiled for the precompiled interpreter (host only, no hipRTC), and either
 
   * batched: every admitted query not yet running goes into the next batch,
     launched as one HIP graph of persistent kernels, with up to two batches
@@ -78,6 +79,62 @@ class ColumnStore:
         for k in keys:
             self.pins[k] -= 1
 
+    def acquire_query(self, q, i):
+        return self.acquire(q["keys"])
+
+    def release_query(self, q, i):
+        self.release(q["keys"])
+
+
+class FreshColumns:
+    """Every query's own columns (the generator streams of the query,
+    workloads.query_column_desc), generated on the device at admission and
+    freed at completion.  A query is admitted while its columns fit in the
+    free device memory less `reserve_bytes` (and, with `cap_bytes`, while
+    the live columns stay under that cap)."""
+
+    def __init__(self, U, seed, reserve_bytes=4 << 30, cap_bytes=None, stream=None):
+        from . import workloads as W
+        self.U = U
+        self.W = W
+        self.seed = seed
+        self.reserve = int(reserve_bytes)
+        self.cap = cap_bytes
+        self.stream = stream
+        self.live = {}
+        self.used = 0
+        self.peak = 0
+        self.generated = 0
+        self.refused = 0
+
+    def nbytes(self, q):
+        return len(q["cols"]) * (q["rows"] * self.U.compact_bytes(q["P"]) + 16)
+
+    def acquire_query(self, q, i):
+        need = self.nbytes(q)
+        if self.cap is not None and self.used + need > self.cap:
+            self.refused += 1
+            return None
+        free, _ = self.U.mem_info()
+        if need > free - self.reserve:
+            self.refused += 1
+            return None
+        bufs = []
+        for k in range(len(q["cols"])):
+            b = self.U.DeviceBuffer(q["rows"] * self.U.compact_bytes(q["P"]) + 16, ordered=True, stream=self.stream)
+            self.U.gen_compact_into(self.W.query_column_desc(self.U, self.seed, q, k), q["rows"], b, stream=self.stream)
+            bufs.append(b)
+        self.live[i] = bufs
+        self.used += need
+        self.peak = max(self.peak, self.used)
+        self.generated += len(bufs)
+        return bufs
+
+    def release_query(self, q, i):
+        bufs = self.live.pop(i)
+        self.used -= self.nbytes(q)
+        del bufs
+
 
 class Server:
     def __init__(self, U, store, batched=True, graph=True, max_batch=512, max_items=1 << 16, stream=None,
@@ -91,8 +148,8 @@ class Server:
         self.stream = stream
         self.batches = [U.Batch(max_queries=self.max_batch, max_items=max_items) for _ in range(self.inflight_max)]
         for b in self.batches:
-            for cls, (rows_per_item, blocks) in (tuning or {}).items():
-                b.tune(cls, rows_per_item, blocks)
+            for cls, t in (tuning or {}).items():
+                b.tune(cls, t[0], t[1], lanes=t[2] if len(t) > 2 else None)
         self.programs = {}
 
     def program(self, q):
@@ -129,7 +186,7 @@ class Server:
             while waiting:
                 i = waiting[0]
                 q = queries[i]
-                cols = self.store.acquire(q["keys"])
+                cols = self.store.acquire_query(q, i)
                 if cols is None:
                     if not inflight and not ready:
                         raise RuntimeError("a query's columns exceed the device-memory budget on their own")
@@ -158,7 +215,7 @@ class Server:
                 for i, cols, prog in take:
                     fin[i] = t
                     lat[i] = t - arrivals[i]
-                    self.store.release(queries[i]["keys"])
+                    self.store.release_query(queries[i], i)
                 done += len(take)
                 free_batches.append(b)
                 progressed = True
