@@ -131,6 +131,47 @@ def headline(U, W, steps, warmup, cpu_rows, threads):
     return res
 
 
+def headline_general(U, W, steps, warmup):
+    """the headline query without column statistics: every value at the
+    width its declared precision allows (a caller that passes bits = 0);
+    results compared with the statistics form's"""
+    C = W.CONFIG3
+    rows = C["rows"]
+    seed = C["seed"]
+    dev = [U.gen_compact_device(W.gen_desc(U, seed, i, c), rows) for i, c in enumerate(C["cols"])]
+    f1 = U.gen_byte_device(seed, W.COL_FLAG1, C["flag1"], rows)
+    f2 = U.gen_byte_device(seed, W.COL_FLAG2, C["flag2"], rows)
+    ship = U.gen_int32_device(seed, W.COL_SHIP, C["ship"][0], C["ship"][1], rows)
+    bits = [U.column_maxbits(t, c[1], rows) for t, c in zip(dev, C["cols"])]
+    ne, maxg = len(C["exprs"]), 16
+    out = {}
+    for name, cols in (("statistics", [(c[0], c[1], c[2], b) for c, b in zip(C["cols"], bits)]),
+                       ("general", [c[:3] for c in C["cols"]])):
+        agg = U.Agg(C["exprs"], cols, filter=C["filter"], nkeys=2, avg=(0, 1, 4))
+        img = U.DeviceBuffer(maxg * (ne + 3) * 133).zero()
+        cnt = U.DeviceBuffer(maxg * 8).zero()
+
+        def step():
+            return agg.run(dev, rows, img, cnt, filter_col=ship, key0=f1, key1=f2, max_groups=maxg)[0]
+
+        for _ in range(warmup + 1):
+            step()
+        U.device_sync()
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+            kms.append(agg.last_kernel_ms())
+        U.device_sync()
+        ms = 1000.0 * (time.perf_counter() - t0) / steps
+        km = sum(kms) / len(kms)
+        out[name] = {"ms_per_step": ms, "kernel_ms": km, "mrows_per_s": rows / ms / 1e3,
+                     "achieved_gbps": 34 * rows / (km / 1000.0) / 1e9, "images": bytes(img.numpy()) + bytes(cnt.numpy())}
+    same = out["general"].pop("images") == out["statistics"].pop("images")
+    r = dict(out["general"], column_bits=0, identical_results_to_statistics_form=same)
+    return r
+
+
 def plain_method(U, W, steps, warmup):
     C = W.CONFIG1
     rows = C["rows"]
@@ -151,37 +192,73 @@ def plain_method(U, W, steps, warmup):
     U.device_sync()
     ms = 1000.0 * (time.perf_counter() - t0) / steps
     kernel_ms = sum(kms) / len(kms)
+    fx = fixtures()["plain"]["sums"]
+    got = images.numpy().reshape(-1, 133)
+    same = all(bytes(got[k]).hex() == fx[t]["image"] for k, t in enumerate(C["exprs"]))
     return {"rows": rows, "ms_per_step": ms, "kernel_ms": kernel_ms, "mrows_per_s": rows / ms / 1e3,
             "kernel_mrows_per_s": rows / kernel_ms / 1e3, "alg_bytes": 21 * rows,
-            "achieved_gbps": 21 * rows / (kernel_ms / 1000.0) / 1e9}
+            "achieved_gbps": 21 * rows / (kernel_ms / 1000.0) / 1e9, "images_equal_oracle": same}
+
+
+def fixtures():
+    """the oracle's full-size results, computed in advance by
+    tests/golden/make_fullsize.py (SUM images, sweep output hashes)"""
+    with open(os.path.join(ROOT, "tests", "golden", "fullsize.json")) as f:
+        return json.load(f)
 
 
 def precision_sweep(U, W, steps, rows_override=None):
+    import hashlib
     C = W.CONFIG2
     rows = rows_override or C["rows"]
+    fx = fixtures()["sweep"]["precisions"] if rows == C["rows"] else {}
     out = {}
     for P, S in C["precisions"]:
         cols = [(n, P, S) for n in "abcd"]
         dev = [U.gen_compact_device(W.digits_desc(U, C["seed"], i, P), rows) for i in range(4)]
         e = U.Expr(C["expr"], cols)
         obuf = U.DeviceBuffer(rows * e.out_bytes + 16)
-        t0 = time.time()
-        e.calc(dev, rows, obuf)
-        U.device_sync()
-        first_s = time.time() - t0
-        ev0, ev1 = U.Event(), U.Event()
         n = steps if P <= 76 else max(1, steps // 3)
-        ev0.record()
-        for _ in range(n):
-            e.calc(dev, rows, obuf)
-        ev1.record()
-        ms = ev0.elapsed_ms(ev1) / n
         alg = (4 * U.compact_bytes(P) + e.out_bytes) * rows
         lp = e.limb_products
-        out[f"P{P}"] = {"rows": rows, "kernel_ms": ms, "mrows_per_s": rows / ms / 1e3, "alg_bytes": alg,
-                        "achieved_gbps": alg / (ms / 1000.0) / 1e9, "out_type": [e.precision, e.scale],
-                        "faithful": e.faithful, "first_call_s": first_s, "limb_products_per_row": lp,
-                        "limb_products_per_s": lp * rows / (ms / 1000.0)}
+
+        def timed(lanes):
+            e.set_lanes(lanes)
+            t0 = time.time()
+            e.calc(dev, rows, obuf)
+            U.device_sync()
+            first_s = time.time() - t0
+            ev0, ev1 = U.Event(), U.Event()
+            ev0.record()
+            for _ in range(n):
+                e.calc(dev, rows, obuf)
+            ev1.record()
+            ms = ev0.elapsed_ms(ev1) / n
+            r = {"kernel_ms": ms, "mrows_per_s": rows / ms / 1e3, "achieved_gbps": alg / (ms / 1000.0) / 1e9,
+                 "first_call_s": first_s, "limb_products_per_s": lp * rows / (ms / 1000.0)}
+            if str(P) in fx:
+                digest = hashlib.sha256(obuf.numpy(count=rows * e.out_bytes).tobytes()).hexdigest()
+                r["output_sha256_equals_oracle"] = digest == fx[str(P)]["sha256"]
+            return r
+
+        r = timed(1)
+        r.update({"rows": rows, "alg_bytes": alg, "out_type": [e.precision, e.scale], "faithful": e.faithful,
+                  "limb_products_per_row": lp, "lanes_per_row": 1})
+        w = e.multi_lane_width
+        if w:
+            # the multi-lane form: one row over W/4 or W/8 lanes
+            r["multi_lane"] = {}
+            for lanes in (w // 4, w // 8):
+                if lanes <= 64:
+                    r["multi_lane"][str(lanes)] = timed(lanes)
+                    log(f"sweep P={P} lanes {lanes}:", r["multi_lane"][str(lanes)])
+            e.set_lanes(1)
+            opts = {"1": r["kernel_ms"], **{k: v["kernel_ms"] for k, v in r["multi_lane"].items()}}
+            best = min(opts, key=opts.get)
+            r["fastest_lanes_per_row"] = int(best)
+            r["fastest_kernel_ms"] = opts[best]
+            r["fastest_mrows_per_s"] = rows / opts[best] / 1e3
+        out[f"P{P}"] = r
         del dev, obuf
     return out
 
@@ -213,7 +290,8 @@ def series(U, W, steps, cpu_rows, threads):
     res = {"rows": rows, "column_bits": bits, "ms_per_step": ms, "kernel_ms": kernel_ms,
            "mrows_per_s": rows / ms / 1e3, "kernel_mrows_per_s": rows / kernel_ms / 1e3,
            "first_call_s": first_s, "alg_bytes": rows * U.compact_bytes(300), "limb_products_per_row": lp,
-           "limb_products_per_s": lp * rows / (kernel_ms / 1000.0)}
+           "limb_products_per_s": lp * rows / (kernel_ms / 1000.0),
+           "image_equals_oracle": bytes(images.numpy()).hex() == fixtures()["series"]["image"]}
     if cpu_rows > 0:
         from oracle import oracle as O
         n = min(cpu_rows, rows)
@@ -404,7 +482,8 @@ def serving_bench(U, W, nq, rates, reserve_gb, jit_queries=12):
     setup_s = time.time() - t0
     imgs = {m: U.DeviceBuffer(133 * nq).zero() for m in ("batched", "plain")}
     cnts = {m: U.DeviceBuffer(8 * nq).zero() for m in ("batched", "plain")}
-    out = {"queries": nq, "rows": sum(q["rows"] for q in qs), "setup_s": setup_s, "tune_s": tune_s,
+    out = {"queries": nq, "stream": f"the first {nq:,} queries of the seeded {C['queries']:,}-query stream",
+           "rows": sum(q["rows"] for q in qs), "setup_s": setup_s, "tune_s": tune_s,
            "tuning": {str(U.VM_CLASS_WIDTHS[k]): v for k, v in tuning.items()},
            "admission": f"fresh columns per query, admitted while they fit in free device memory less "
                         f"{reserve_gb} GiB (hipMemGetInfo)",
@@ -502,13 +581,15 @@ def main():
                     help="rows of the CPU baseline (all 60,000,000 by default; then also compared with the GPU)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="headline only")
-    ap.add_argument("--extra", default="copy,plain,sweep,series,serving,codec,rsa,segsum",
+    ap.add_argument("--extra", default="general,copy,plain,sweep,series,serving,codec,rsa,segsum",
                     help="comma list of extra workloads")
     ap.add_argument("--sweep-rows", type=int, default=0)
     ap.add_argument("--series-cpu-rows", type=int, default=50_000, help="rows of the series CPU sample")
     ap.add_argument("--codec-cpu-rows", type=int, default=1_000_000, help="rows of the text-ingest CPU sample")
     ap.add_argument("--rsa-cpu-rows", type=int, default=200_000, help="rows of the RSA CPU sample (per key length)")
-    ap.add_argument("--serve-queries", type=int, default=10_000, help="queries per serving run")
+    ap.add_argument("--serve-queries", type=int, default=1_000,
+                    help="queries per serving run: the first N of the seeded 10,000-query stream (the whole "
+                         "stream takes about 150 s per run on one MI355X)")
     ap.add_argument("--serve-rates", default="200,400,800")
     ap.add_argument("--serve-reserve-gb", type=int, default=8,
                     help="device memory kept free by the serving admission (GiB)")
@@ -534,6 +615,9 @@ def main():
     log("headline:", h)
     extra = {}
     want = set() if args.no_extra else set(args.extra.split(","))
+    if "general" in want:
+        extra["headline_without_column_statistics"] = headline_general(U, W, max(5, args.steps // 2), args.warmup)
+        log("headline without column statistics:", extra["headline_without_column_statistics"])
     if "copy" in want:
         extra["copy"] = copy_bandwidth(U)
         log("copy:", extra["copy"])

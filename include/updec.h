@@ -130,6 +130,17 @@ int updec_expr_source_iter(updec_expr_t e, char* buf, size_t cap, size_t* needed
  * d_cols[i] is column i of the compile-time descriptor list; d_out must be
  * 4-byte aligned. */
 int updec_expr_calc(updec_expr_t e, const uint8_t* const* d_cols, int64_t rows, uint8_t* d_out, updec_stream_t stream);
+/* Lanes per row of updec_expr_calc: the reference's multi-lane (XMP) form of
+ * calc_expr_<id>, one value spread over a group of lanes
+ * (src/AriesEngine/AriesExprCalcNode.cpp:1320-1400, the TPI choice of
+ * AriesCalcTreeGenerator.cpp:40-46).  An extended-mode expression whose
+ * exact intermediates need W >= 64 limbs has it, with W / 4 or W / 8 lanes
+ * per row (at most 64); 1 selects one row per lane (the default).
+ * *width_out (optional) receives W, or 0 when there is no multi-lane form.
+ * lanes_per_row = 0 only queries.  Results are identical in every form. */
+int updec_expr_lanes(updec_expr_t e, int32_t lanes_per_row, int32_t* width_out);
+/* source of the multi-lane calc kernel at lanes_per_row lanes per row */
+int updec_expr_source_ml(updec_expr_t e, int32_t lanes_per_row, char* buf, size_t cap, size_t* needed);
 void updec_expr_free(updec_expr_t e);
 
 /* Block-column inputs: AriesColumnDataIterator (src/AriesColumnDataIterator.hxx:8-41,

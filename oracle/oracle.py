@@ -115,6 +115,27 @@ def plan_sum_wide(plan_text, cols, precisions, scales, rows, mode=EXTENDED, thre
     return out.raw, n
 
 
+def plan_eval_wide(plan_text, cols, precisions, scales, rows, out_bytes, mode=EXTENDED, threads=8):
+    """plan_eval with 640-limb extended values (liboracle_wide.so)"""
+    import numpy as np
+    global _wide_lib
+    if _wide_lib is None:
+        _wide_lib = _load(os.path.join(_HERE, "liboracle_wide.so"))
+    parts = plan_text.split(" ", 4)
+    if parts[0] in ("F", "X") and len(parts) == 5:
+        plan_text = parts[4]
+    h = _wide_lib.od_plan_parse(plan_text.encode())
+    if not h:
+        raise ValueError("unparsable plan: " + plan_text)
+    try:
+        arrs, ptrs, P, S = _colargs(cols, precisions, scales)
+        out = np.zeros(rows * out_bytes, dtype=np.uint8)
+        _wide_lib.od_plan_eval(h, ptrs, P, S, rows, out.ctypes.data, out_bytes, mode, threads)
+    finally:
+        _wide_lib.od_plan_free(h)
+    return out
+
+
 def use_native():
     """Rebuild the oracle with -O3 -march=native for the host this process
     runs on (make native) and route every call below through it.  Used by

@@ -228,3 +228,23 @@ def test_iterator_kernels_compile(U):
     src = a.source(6, iterators=True)
     assert "col_load<7>(a, 1, row, nc1)" in src and "p2_nn" in src
     U.check_compile(src)
+
+
+def test_multi_lane_calc_kernel_compiles(U):
+    """an extended-mode plan of 64+ limbs has the multi-lane calc form at
+    W/4 and W/8 lanes per row; both sources compile for gfx950, and lane
+    counts other than 1, W/4 and W/8 are refused"""
+    e = U.Expr("(a+b)*c - a/d", [(n, 300, 100) for n in "abcd"])
+    assert e.multi_lane_width == 64
+    for lanes in (16, 8):
+        src = e.ml_source(lanes)
+        assert f"#define TPI_ {lanes}" in src and "ml::div<L_, TPI_>(A, B, 104u, g)" in src
+        U.check_compile(src)
+    with pytest.raises(U.UpdecError):
+        e.set_lanes(32)
+    e.set_lanes(8)
+    e.set_lanes(1)
+    narrow = U.Expr("(a+b)*c - a/d", [(n, 38, 10) for n in "abcd"])
+    assert narrow.multi_lane_width == 0
+    with pytest.raises(U.UpdecError):
+        narrow.set_lanes(8)

@@ -168,6 +168,8 @@ def _load():
         "updec_expr_plan": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
         "updec_expr_source": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
         "updec_expr_source_iter": (ctypes.c_int, [P, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
+        "updec_expr_lanes": (ctypes.c_int, [P, I32, ctypes.POINTER(I32)]),
+        "updec_expr_source_ml": (ctypes.c_int, [P, I32, ctypes.c_char_p, SZ, ctypes.POINTER(SZ)]),
         "updec_expr_calc": (ctypes.c_int, [P, ctypes.POINTER(P), I64, P, P]),
         "updec_expr_free": (None, [P]),
         "updec_agg_compile": (ctypes.c_int, [ctypes.POINTER(AggDesc), ctypes.POINTER(P)]),
@@ -235,7 +237,7 @@ EXPORTED = ["updec_last_error", "updec_version", "updec_init", "updec_compact_by
             "updec_batch_tuning", "updec_batch_lanes", "updec_batch_add", "updec_batch_pending", "updec_batch_launch", "updec_batch_free",
             "updec_parse_compact", "updec_format_stride", "updec_format_compact", "updec_expr_calc_blocks",
             "updec_agg_run_blocks", "updec_expr_calc_iter", "updec_agg_run_iter", "updec_expr_source_iter",
-            "updec_agg_source_iter"]
+            "updec_agg_source_iter", "updec_expr_lanes", "updec_expr_source_ml"]
 
 
 def lib():
@@ -435,6 +437,21 @@ class Expr:
     @property
     def source(self):
         return _text(_lib.updec_expr_source, self._h)
+
+    @property
+    def multi_lane_width(self):
+        """limbs W of the multi-lane form (0: one row per lane only)"""
+        w = ctypes.c_int32()
+        _check(_lib.updec_expr_lanes(self._h, 0, ctypes.byref(w)))
+        return w.value
+
+    def set_lanes(self, lanes_per_row):
+        """lanes per row of calc(): 1, or W/4 or W/8 (at most 64) for an
+        expression with a multi-lane form"""
+        _check(_lib.updec_expr_lanes(self._h, int(lanes_per_row), None))
+
+    def ml_source(self, lanes_per_row):
+        return _text(_lib.updec_expr_source_ml, self._h, ctypes.c_int32(int(lanes_per_row)))
 
     def iter_source(self):
         """source of the column-iterator variant (calc_iter)"""

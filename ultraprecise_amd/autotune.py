@@ -12,7 +12,7 @@ import json
 import os
 
 DEFAULT_CANDIDATES = ((1024, 4), (2048, 4), (4096, 2), (4096, 4), (8192, 4), (8192, 8), (16384, 4), (32768, 4))
-MULTI_LANE_ITEMS = ((64, 4), (128, 4), (256, 2), (256, 4), (512, 4))
+MULTI_LANE_ITEMS = ((128, 4), (256, 4), (512, 4), (512, 8), (1024, 4), (1024, 8), (2048, 4))
 
 
 def class_rows(width, rows):
@@ -99,3 +99,32 @@ def tune(U, rows=2_000_000, candidates=DEFAULT_CANDIDATES, reps=3, use_cache=Tru
         with open(path, "w") as f:
             json.dump({str(k): list(v) for k, v in result.items()}, f)
     return result
+
+
+def tune_expr_lanes(U, expr, columns, rows, out, reps=3, sample_rows=1_000_000):
+    """Lanes per row of a JIT calc expression with a multi-lane form: times
+    calc over the first `sample_rows` rows at one row per lane and at W/4
+    and W/8 lanes per row (at most 64), keeps the fastest on `expr` and
+    returns {lanes: ms}.  An expression without the form keeps one row per
+    lane."""
+    w = expr.multi_lane_width
+    if not w:
+        return {1: None}
+    n = min(rows, sample_rows)
+    times = {}
+    for lanes in sorted({1, w // 4, w // 8}):
+        if lanes > 64:
+            continue
+        expr.set_lanes(lanes)
+        expr.calc(columns, n, out)      # compile and warm up
+        U.device_sync()
+        ev0, ev1 = U.Event(), U.Event()
+        ev0.record()
+        for _ in range(reps):
+            expr.calc(columns, n, out)
+        ev1.record()
+        ev1.sync()
+        times[lanes] = ev0.elapsed_ms(ev1) / reps
+    best = min(times, key=times.get)
+    expr.set_lanes(best)
+    return times

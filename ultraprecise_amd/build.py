@@ -1,4 +1,4 @@
-"""Build step for the HIP library: embeds dec_device.h for hipRTC and
+"""Build step for the HIP library: embeds the device headers for hipRTC and
 compiles libupdec.so for gfx950 in-tree (the built library travels with the
 source tree; nothing is installed)."""
 import os
@@ -20,11 +20,14 @@ SOURCES = ("updec_api.hip", "vm.hip", "codec.hip", "segsum.hip", "planner.cpp")
 
 
 def embed_device_header():
-    src = open(os.path.join(CSRC, "dec_device.h")).read()
+    """dec_device.h and dec_mlane.h as string constants, the headers hipRTC
+    is given for every generated kernel"""
     delim = "UPDEC_EMBED"
-    assert ")" + delim + '"' not in src
-    out = ("// generated from dec_device.h by build.py; do not edit\n"
-           "static const char kDecDeviceSource[] = R\"" + delim + "(" + src + ")" + delim + "\";\n")
+    out = "// generated from dec_device.h and dec_mlane.h by build.py; do not edit\n"
+    for header, name in (("dec_device.h", "kDecDeviceSource"), ("dec_mlane.h", "kDecMlaneSource")):
+        src = open(os.path.join(CSRC, header)).read()
+        assert ")" + delim + '"' not in src
+        out += "static const char " + name + "[] = R\"" + delim + "(" + src + ")" + delim + "\";\n"
     path = os.path.join(CSRC, "dec_device_embed.inc")
     if not os.path.exists(path) or open(path).read() != out:
         with open(path, "w") as f:

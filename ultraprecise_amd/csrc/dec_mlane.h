@@ -542,6 +542,57 @@ UPD_DEV MDec<L> load_compact(const u8* __restrict__ col, usize row, u32 B, usize
     return r;
 }
 
+// load_compact with the row width B fixed at compile time (the generated
+// multi-lane kernels): rows of a whole number of dwords are read as aligned
+// dword loads (the column base is 4-byte aligned), others byte by byte
+template <int L, int TPI, int B>
+UPD_DEV MDec<L> load_compact_b(const u8* __restrict__ col, usize row, const Grp<TPI>& g)
+{
+    MDec<L> r;
+    if constexpr (B % 4 == 0) {
+        const u32* p = reinterpret_cast<const u32*>(col + row * (usize)B);
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+            const u32 k = g.rank * (u32)L + (u32)i;
+            r.m.w[i] = k < (u32)(B / 4) ? p[k] : 0u;
+        }
+        constexpr u32 sb = 8u * (B - 1u) + 7u;
+        constexpr u32 owner = sb / (32u * (u32)L), loc = (sb / 32u) % (u32)L;
+        r.neg = (g.from(r.m.w[loc], owner) >> (sb & 31u)) & 1u;
+        if (g.rank == owner) r.m.w[loc] &= ~(1u << (sb & 31u));
+    } else {
+        r = load_compact<L, TPI>(col, row, (u32)B, (row + 1) * (usize)B, g);
+    }
+    return r;
+}
+
+// a compact output row of B bytes (encode_compact): the magnitude's low B
+// bytes, the sign ORed into bit 7 of byte B - 1; the lane of rank r writes
+// the bytes of its limbs (whole dwords when B is a multiple of 4)
+template <int L, int TPI, int B>
+UPD_DEV void store_compact_b(u8* __restrict__ out, usize row, const MDec<L>& v, const Grp<TPI>& g)
+{
+    constexpr u32 sb = 8u * (B - 1u) + 7u;
+    if constexpr (B % 4 == 0) {
+        u32* p = reinterpret_cast<u32*>(out + row * (usize)B);
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+            const u32 k = g.rank * (u32)L + (u32)i;
+            if (k < (u32)(B / 4)) p[k] = v.m.w[i] | (k == sb / 32u ? v.neg << (sb & 31u) : 0u);
+        }
+    } else {
+        u8* o = out + row * (usize)B;
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const u32 bi = (g.rank * (u32)L + (u32)i) * 4u + (u32)k;
+                if (bi < (u32)B) o[bi] = (u8)(((v.m.w[i] >> (8 * k)) & 0xffu) | (bi == (u32)B - 1u ? v.neg << 7 : 0u));
+            }
+        }
+    }
+}
+
 // the compact round trip of a SUM input (stored_rt): keep 8 B - 1 magnitude
 // bits, OR the bit at 8 B - 1 into the sign
 template <int L, int TPI> UPD_DEV MDec<L> stored(const MDec<L>& v, u32 B, const Grp<TPI>& g)

@@ -2608,6 +2608,85 @@ bool build_vm_program(const Plan& p, const std::vector<ColumnType>& cols, VmProg
     return true;
 }
 
+// Multi-lane calc kernel (the reference's XMP form of calc_expr_<id>,
+// AriesExprCalcNode.cpp:1320-1400, re-designed for wave64): the register
+// program of an extended-mode plan whose exact intermediates need 64 or
+// more limbs, emitted straight-line with every shift count, scale and row
+// width as a compile-time constant.  A group of W / L lanes evaluates one
+// row, L limbs per lane (dec_mlane.h); the groups of a block take
+// consecutive rows.  Each column the program reads is loaded once per row;
+// constants ({sign, W limbs} each, in a.kc) are loaded once per thread.
+std::string gen_calc_ml(const VmProgram& vp, const std::vector<ColumnType>& cols, int L, const std::string& name)
+{
+    using namespace updec_vm;
+    const int W = vp.W, TPI = W / L;
+    std::ostringstream os;
+    os << "#include \"dec_mlane.h\"\nusing namespace updec;\n";
+    os << "struct MlArgs { const u8* col[" << kMaxCols << "]; usize rows; u8* out; const u32* kc; };\n";
+    os << "// program (" << vp.ins.size() << " instructions, " << W << " limbs, " << TPI << " lanes of " << L
+       << " limbs per row)\n";
+    {
+        std::istringstream lines(vp.text());
+        std::string ln;
+        while (std::getline(lines, ln)) os << "//   " << ln << "\n";
+    }
+    os << "#define L_ " << L << "\n#define TPI_ " << TPI << "\n#define NG_ (256 / TPI_)\n";
+    os << "extern \"C\" __global__ void __launch_bounds__(256) " << name << "(MlArgs a)\n{\n";
+    os << "  const ml::Grp<TPI_> g;\n";
+    std::vector<bool> used_col(cols.size(), false);
+    int nconst = 0;
+    auto note = [&](const Operand& o) {
+        if (o.kind == K_COL) used_col[o.idx] = true;
+        if (o.kind == K_CONST) nconst = std::max(nconst, o.idx + 1);
+    };
+    for (const Ins& I : vp.ins) {
+        note(I.a);
+        if (I.op >= OP_ADD && I.op <= OP_DIV) note(I.b);
+    }
+    for (int k = 0; k < nconst; k++) {
+        os << "  ml::MDec<L_> k" << k << ";\n  k" << k << ".neg = a.kc[" << k * (W + 1) << "];\n";
+        os << "#pragma unroll\n  for (int w = 0; w < L_; w++) k" << k << ".m.w[w] = a.kc[" << k * (W + 1)
+           << " + 1 + g.rank * L_ + w];\n";
+    }
+    os << "  const usize step = (usize)gridDim.x * NG_;\n";
+    os << "  for (usize row = (usize)blockIdx.x * NG_ + threadIdx.x / TPI_; row < a.rows; row += step) {\n";
+    for (size_t c = 0; c < cols.size(); c++)
+        if (used_col[c])
+            os << "    const ml::MDec<L_> c" << c << " = ml::load_compact_b<L_, TPI_, " << compact_bytes(cols[c].P)
+               << ">(a.col[" << c << "], row, g);\n";
+    os << "    ml::MDec<L_> last";
+    for (int r = 0; r < vp.nregs; r++) os << ", r" << r;
+    os << ";\n";
+    auto fetch = [&](const Operand& o, const char* v) {
+        std::string src = o.kind == K_LAST ? "last"
+                          : (o.kind == K_REG ? "r" : o.kind == K_COL ? "c" : "k") + std::to_string(o.idx);
+        os << "      ml::MDec<L_> " << v << " = " << src << ";\n";
+        if (o.shift) os << "      ml::mul_p10<L_, TPI_>(" << v << ".m, " << o.shift << "u, g);\n";
+    };
+    for (const Ins& I : vp.ins) {
+        os << "    {\n";
+        fetch(I.a, "A");
+        if (I.op >= OP_ADD && I.op <= OP_DIV) fetch(I.b, "B");
+        os << "      ml::MDec<L_> R;\n";
+        switch (I.op) {
+        case OP_MOV: os << "      R = A;\n"; break;
+        case OP_NEG: os << "      R = A;\n      R.neg ^= 1u;\n"; break;
+        case OP_UP: os << "      R = A;\n      ml::mul_p10<L_, TPI_>(R.m, " << I.arg << "u, g);\n"; break;
+        case OP_DOWN: os << "      R = A;\n      ml::div_p10<L_, TPI_>(R.m, " << I.arg << "u, g);\n"; break;
+        case OP_ADD: os << "      R = ml::add<L_, TPI_>(A, B, g);\n"; break;
+        case OP_SUB: os << "      R = ml::sub<L_, TPI_>(A, B, g);\n"; break;
+        case OP_MUL: os << "      R.m = ml::mul<L_, TPI_>(A.m, B.m, g);\n      R.neg = A.neg ^ B.neg;\n"; break;
+        default: os << "      R = ml::div<L_, TPI_>(A, B, " << I.arg << "u, g);\n"; break;
+        }
+        if (I.dst != kNoStore) os << "      r" << (int)I.dst << " = R;\n";
+        os << "      last = R;\n    }\n";
+    }
+    os << "    ml::store_compact_b<L_, TPI_, " << vp.outB << ">(a.out, row, "
+       << (vp.root == kNoStore ? std::string("last") : "r" + std::to_string(vp.root)) << ", g);\n";
+    os << "  }\n}\n";
+    return os.str();
+}
+
 std::string VmProgram::text() const
 {
     static const char* names[] = {"MOV", "NEG", "ADD", "SUB", "MUL", "DIV", "UP", "DOWN"};

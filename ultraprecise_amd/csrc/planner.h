@@ -235,5 +235,7 @@ struct VmProgram {
     std::string text() const;       // one line per instruction
 };
 bool build_vm_program(const Plan& p, const std::vector<ColumnType>& cols, VmProgram* out, std::string* err);
+// multi-lane calc kernel `name` of a program of 64 or more limbs, L limbs per lane
+std::string gen_calc_ml(const VmProgram& vp, const std::vector<ColumnType>& cols, int L, const std::string& name);
 
 }  // namespace updec

@@ -101,7 +101,8 @@ class DiskCache {
         } else if (const char* h = getenv("HOME"); h && *h) {
             dir_ = std::string(h) + "/.cache/updec";
         }
-        lib_hash_ = fnv1a(kDecDeviceSource, sizeof(kDecDeviceSource) - 1);
+        lib_hash_ = fnv1a(kDecMlaneSource, sizeof(kDecMlaneSource) - 1,
+                          fnv1a(kDecDeviceSource, sizeof(kDecDeviceSource) - 1));
     }
     bool load(const std::string& src, std::vector<char>* code)
     {
@@ -208,9 +209,9 @@ class ModuleCache {
     static bool rtc(const std::string& src, std::vector<char>* code, std::string* err)
     {
         hiprtcProgram prog;
-        const char* hdr_src[] = {kDecDeviceSource};
-        const char* hdr_name[] = {"dec_device.h"};
-        if (hiprtcCreateProgram(&prog, src.c_str(), "updec_kernel.hip", 1, hdr_src, hdr_name) != HIPRTC_SUCCESS) {
+        const char* hdr_src[] = {kDecDeviceSource, kDecMlaneSource};
+        const char* hdr_name[] = {"dec_device.h", "dec_mlane.h"};
+        if (hiprtcCreateProgram(&prog, src.c_str(), "updec_kernel.hip", 2, hdr_src, hdr_name) != HIPRTC_SUCCESS) {
             *err = "hiprtcCreateProgram failed";
             return false;
         }
@@ -579,6 +580,13 @@ struct updec_expr_s {
     // the block-column variant, generated and compiled at first use
     ModuleP bmod;
     hipFunction_t bfn = nullptr;
+    // multi-lane form (extended-mode plans whose exact intermediates need 64
+    // or more limbs): the register program, lanes per row (1 = one row per
+    // lane, the kernel above), the kernels compiled per lane count
+    std::unique_ptr<VmProgram> vp;
+    int lanes = 1;
+    ConstBlock ml_consts;
+    std::map<int, std::pair<ModuleP, hipFunction_t>> ml_fn;
 };
 
 struct updec_agg_s {
@@ -847,8 +855,41 @@ int updec_expr_compile(const char* text, const updec_column_desc* cols, int ncol
     GenOptions o;
     o.block = calc_block(e->plan);
     e->src = gen_calc_kernel(e->plan, types, "calc_expr", o);
+    // a plan too wide for one row per lane also gets the multi-lane form
+    // (updec_expr_lanes selects it)
+    if (e->plan.mode != Mode::Faithful && ncols <= updec_vm::kMaxCols) {
+        std::unique_ptr<VmProgram> vp(new VmProgram());
+        std::string verr;
+        if (build_vm_program(e->plan, types, vp.get(), &verr) && vp->wclass >= updec_vm::kFirstMultiLane &&
+            vp->outB == e->plan.outB) {
+            e->ml_consts.host = vp->consts;
+            e->vp = std::move(vp);
+        }
+    }
     *out = e.release();
     return UPDEC_OK;
+}
+
+int updec_expr_lanes(updec_expr_t e, int32_t lanes_per_row, int32_t* width_out)
+{
+    if (!e) return fail(UPDEC_EINVAL, "null expression");
+    const int W = e->vp ? e->vp->W : 0;
+    if (width_out) *width_out = W;
+    if (lanes_per_row == 0) return UPDEC_OK;
+    if (lanes_per_row == 1 || (W && (lanes_per_row == W / 4 || lanes_per_row == W / 8) && lanes_per_row <= 64)) {
+        e->lanes = lanes_per_row;
+        return UPDEC_OK;
+    }
+    return fail(UPDEC_EUNSUPPORTED, W ? "lanes per row must be 1, W/4 or W/8 (at most 64)"
+                                      : "this expression has only the one-row-per-lane form");
+}
+
+int updec_expr_source_ml(updec_expr_t e, int32_t lanes_per_row, char* buf, size_t cap, size_t* needed)
+{
+    if (!e) return fail(UPDEC_EINVAL, "null expression");
+    if (!e->vp || (lanes_per_row != e->vp->W / 4 && lanes_per_row != e->vp->W / 8))
+        return fail(UPDEC_EUNSUPPORTED, "no multi-lane form at this lane count");
+    return copy_text(gen_calc_ml(*e->vp, e->types, e->vp->W / lanes_per_row, "calc_ml"), buf, cap, needed);
 }
 
 int updec_expr_info(updec_expr_t e, int32_t* P, int32_t* S_, int32_t* out_bytes, int32_t* faithful)
@@ -910,6 +951,33 @@ int updec_expr_calc(updec_expr_t e, const uint8_t* const* d_cols, int64_t rows, 
     if (reinterpret_cast<uintptr_t>(d_out) & 3u) return fail(UPDEC_EINVAL, "output must be 4-byte aligned");
     if (rows == 0) return UPDEC_OK;
     std::string err;
+    if (e->lanes > 1) {
+        // multi-lane form: a group of `lanes` lanes per row, 256-thread
+        // blocks striding over the rows
+        auto& mf = e->ml_fn[e->lanes];
+        if (!mf.first) {
+            mf.first = ModuleCache::get().compile(gen_calc_ml(*e->vp, e->types, e->vp->W / e->lanes, "calc_ml"), &err);
+            if (!mf.first) return fail(UPDEC_ECOMPILE, err);
+            mf.second = mf.first->fn("calc_ml", &err);
+            if (!mf.second) return fail(UPDEC_ECOMPILE, err);
+        }
+        if (int rc = e->ml_consts.upload(S(stream))) return rc;
+        struct MlArgs {
+            const uint8_t* col[updec_vm::kMaxCols];
+            usize rows;
+            uint8_t* out;
+            const u32* kc;
+        } a{};
+        for (size_t i = 0; i < e->types.size(); i++) a.col[i] = d_cols[i];
+        a.rows = (usize)rows;
+        a.out = d_out;
+        a.kc = e->ml_consts.dev;
+        const int64_t per_block = 256 / e->lanes;
+        const int64_t blocks = std::min<int64_t>((rows + per_block - 1) / per_block, (int64_t)device_cus() * 16);
+        void* params[] = {&a};
+        HIPCHK(hipModuleLaunchKernel(mf.second, (unsigned)blocks, 1, 1, 256, 1, 1, 0, S(stream), params, nullptr));
+        return UPDEC_OK;
+    }
     if (!e->mod) {
         e->mod = ModuleCache::get().compile(e->src, &err);
         if (!e->mod) return fail(UPDEC_ECOMPILE, err);
