@@ -460,7 +460,7 @@ def rsa(U, W, steps, cpu_rows, threads):
     return out
 
 
-def serving_bench(U, W, nq, rates, reserve_gb, jit_queries=12):
+def serving_bench(U, W, nq, rates, reserve_gb, jit_queries=4):
     """the batched serving workload: nq seeded queries at the stated
     distribution (every query as drawn: programs of up to 512 limbs run on
     the interpreter's multi-lane classes), each on its own fresh columns
