@@ -460,7 +460,7 @@ def rsa(U, W, steps, cpu_rows, threads):
     return out
 
 
-def serving_bench(U, W, nq, rates, reserve_gb, jit_queries=4):
+def serving_bench(U, W, nq, rates, reserve_gb, jit_queries=4, saturated=True):
     """the batched serving workload: nq seeded queries at the stated
     distribution (every query as drawn: programs of up to 512 limbs run on
     the interpreter's multi-lane classes), each on its own fresh columns
@@ -492,17 +492,17 @@ def serving_bench(U, W, nq, rates, reserve_gb, jit_queries=4):
                                    for w in U.VM_CLASS_WIDTHS}}
     zero = np.zeros(nq)
     sat = {}
-    for mode in ("plain", "batched"):
+    for mode in (("plain", "batched") if saturated else ()):
         store = serving.FreshColumns(U, C["seed"], reserve_bytes=reserve_gb << 30)
         srv = serving.Server(U, store, batched=(mode == "batched"), tuning=tuning)
         st = srv.run(qs, zero, imgs[mode], cnts[mode])
         sat[mode] = {k: v for k, v in st.items() if k != "latency_s"}
         sat[mode]["peak_column_gb"] = store.peak / 2 ** 30
         log(f"serving saturated {mode}:", sat[mode])
-    same = bytes(imgs["batched"].numpy()) == bytes(imgs["plain"].numpy())
+    same = bytes(imgs["batched"].numpy()) == bytes(imgs["plain"].numpy()) if saturated else None
     # the drop-in JIT path, one query at a time (hipRTC compile of the fused
     # SUM kernel, then one launch), on the first queries of the stream
-    njit = min(jit_queries, nq)
+    njit = min(jit_queries, nq) if saturated else 0
     if njit:
         got = imgs["batched"].numpy().reshape(nq, 133)
         jimg = U.DeviceBuffer(133)
@@ -523,7 +523,8 @@ def serving_bench(U, W, nq, rates, reserve_gb, jit_queries=4):
                             "identical_results": jit_same}
         log("serving saturated plain_jit (first queries):", sat["plain_jit"])
     out["saturated"] = sat
-    out["saturated_speedup_vs_plain_interpreter"] = sat["batched"]["mrows_per_s"] / sat["plain"]["mrows_per_s"]
+    if saturated:
+        out["saturated_speedup_vs_plain_interpreter"] = sat["batched"]["mrows_per_s"] / sat["plain"]["mrows_per_s"]
     if "plain_jit" in sat:
         out["saturated_speedup_vs_plain_jit"] = sat["batched"]["mrows_per_s"] / sat["plain_jit"]["mrows_per_s"]
     out["identical_results"] = same
@@ -591,6 +592,7 @@ def main():
                     help="queries per serving run: the first N of the seeded 10,000-query stream (the whole "
                          "stream takes about 150 s per run on one MI355X)")
     ap.add_argument("--serve-rates", default="200,400,800")
+    ap.add_argument("--serve-saturated", type=int, default=1, help="0: only the Poisson-rate runs")
     ap.add_argument("--serve-reserve-gb", type=int, default=8,
                     help="device memory kept free by the serving admission (GiB)")
     args = ap.parse_args()
@@ -646,7 +648,8 @@ def main():
         log("codec:", extra["codec_10M"])
     if "serving" in want:
         rates = [int(r) for r in args.serve_rates.split(",")]
-        extra["serving"] = serving_bench(U, W, args.serve_queries, rates, args.serve_reserve_gb)
+        extra["serving"] = serving_bench(U, W, args.serve_queries, rates, args.serve_reserve_gb,
+                                         saturated=bool(args.serve_saturated))
     value = h["rows"] / h["ms_per_step"] / 1e3  # Mrows/s
     achieved = h["achieved_gbps"]
     cpu = h.get("cpu")
