@@ -83,6 +83,17 @@ def headline(U, W, steps, warmup, cpu_rows, threads):
         "alg_bytes": alg_bytes, "achieved_gbps": achieved, "compile_s": compile_s,
         "column_bits": bits,
     }
+    # the timed run's group images, keys and counts against the oracle's
+    # full-size results committed with the tests (outside the timed region)
+    G = fixtures()["q1"]
+    got = images.numpy(count=maxg * (ne + 3) * 133).reshape(-1, 133)
+    gcnt = counts.numpy("int64", count=maxg)
+    same = ng == len(G["groups"])
+    for g, want in enumerate(G["groups"][:ng]):
+        same = same and int(gcnt[g]) == want["count"]
+        same = same and all(bytes(got[g * ne + k]).hex() == want["sums"][k] for k in range(ne))
+        same = same and all(bytes(got[maxg * ne + g * 3 + j]).hex() == want["avgs"][j] for j in range(3))
+    res["gpu_equals_fixture"] = bool(same)
     # CPU baseline: the oracle's restatement of the reference's host Decimal
     # path over the same rows (all of them by default), built -march=native on
     # this host; its group images and counts are then compared byte for byte
@@ -686,7 +697,8 @@ def main():
                       f"is the thread count)",
             "cpu_model": cpu["cpu_model"],
         },
-        "exact": {"headline_gpu_equals_cpu_all_rows": h.get("gpu_equals_cpu_all_rows")},
+        "exact": {"headline_gpu_equals_cpu_all_rows": h.get("gpu_equals_cpu_all_rows"),
+                  "headline_gpu_equals_full_size_fixture": h.get("gpu_equals_fixture")},
         "device": info["name"],
         "extra": extra,
     }
