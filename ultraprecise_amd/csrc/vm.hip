@@ -599,6 +599,13 @@ int updec_batch_add(updec_batch_t b, updec_vm_t v, const uint8_t* const* d_cols,
     // the class's tuned rows per item, raised (in steps of 64 rows) for a
     // query that would otherwise need more items than the whole batch holds
     int64_t ch = b->rows_per_item[pr.wclass];
+    // a large query takes items of at least rows / (4 x the class's resident
+    // blocks): four items per block keep the load balanced while the
+    // per-item work (fetch, block reduction, partial write) stays small next
+    // to the item's rows
+    const int64_t slots = 4 * (int64_t)api_device_cus() * b->blocks_per_cu[pr.wclass];
+    const int64_t big = (rows + slots - 1) / slots;
+    if (big > ch) ch = (big + 63) / 64 * 64;
     if ((rows + ch - 1) / ch > b->max_items) ch = ((rows + b->max_items - 1) / b->max_items + 63) / 64 * 64;
     const uint32_t items = (uint32_t)std::max<int64_t>(1, (rows + ch - 1) / ch);
     const bool new_prog = b->pend_programs.find(&pr) == b->pend_programs.end();
