@@ -147,6 +147,17 @@ template <int L, int TPI> UPD_DEV u32 limb_len(const MMag<L>& a, const Grp<TPI>&
     return top * (u32)L + g.from(loc, top);
 }
 
+// lanes of the group up to the highest one holding a nonzero limb (0 for
+// zero), on every lane
+template <int L, int TPI> UPD_DEV u32 rank_len(const MMag<L>& a, const Grp<TPI>& g)
+{
+    u32 o = 0;
+#pragma unroll
+    for (int i = 0; i < L; i++) o |= a.w[i];
+    const u64 nz = g.bits(__ballot(o != 0u ? 1 : 0));
+    return nz == 0ull ? 0u : 64u - (u32)__clzll(nz);
+}
+
 // limb k of the group's value (k uniform in the group; 0 past the value)
 template <int L, int TPI> UPD_DEV u32 limb(const MMag<L>& a, u32 k, const Grp<TPI>& g)
 {
@@ -276,14 +287,25 @@ template <int L, int TPI> UPD_DEV MMag<L> mul(const MMag<L>& a, const MMag<L>& b
         lo[t] = 0ull;
         hi[t] = 0u;
     }
-    for (u32 ir = 0; ir < (u32)TPI; ir++) {
+    // the operand with fewer nonzero lanes is broadcast, and only its
+    // nonzero lanes: the ranks above them add nothing to any column
+    const u32 ra = rank_len<L, TPI>(a, g), rb = rank_len<L, TPI>(b, g);
+    const bool swap = rb < ra;
+    const u32 n = swap ? rb : ra;
+    MMag<L> x, y;
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        x.w[i] = swap ? b.w[i] : a.w[i];
+        y.w[i] = swap ? a.w[i] : b.w[i];
+    }
+    for (u32 ir = 0; ir < n; ir++) {
         u32 av[L], b0[L], b1[L];
         const u32 s0 = (g.rank - ir) & (TPI - 1u), s1 = (g.rank - ir - 1u) & (TPI - 1u);
 #pragma unroll
         for (int i = 0; i < L; i++) {
-            av[i] = g.from(a.w[i], ir);
-            b0[i] = g.from(b.w[i], s0);
-            b1[i] = g.from(b.w[i], s1);
+            av[i] = g.from(x.w[i], ir);
+            b0[i] = g.from(y.w[i], s0);
+            b1[i] = g.from(y.w[i], s1);
         }
         const bool v0 = g.rank >= ir, v1 = g.rank >= ir + 1u;
 #pragma unroll
