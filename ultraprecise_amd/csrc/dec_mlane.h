@@ -61,9 +61,12 @@ template <int L> UPD_DEV MMag<L> mzero()
 // compile-time constant
 template <int L> UPD_DEV u32 sel(const MMag<L>& a, u32 t)
 {
-    u32 v = a.w[0];
+    // masks rather than a select chain: the compiler turns a chain of
+    // selects over the array into one load at a run-time index, which moves
+    // the array to scratch
+    u32 v = 0u;
 #pragma unroll
-    for (int i = 1; i < L; i++) v = t == (u32)i ? a.w[i] : v;
+    for (int i = 0; i < L; i++) v |= a.w[i] & (0u - (u32)(t == (u32)i));
     return v;
 }
 
