@@ -41,12 +41,22 @@ template <int TPI> struct Grp {
         if constexpr (TPI == 64) return ballot;
         else return (ballot >> base) & ((1ull << TPI) - 1ull);
     }
+    // v of the group's lane src_rank (any source per lane)
     UPD_DEV u32 from(u32 v, u32 src_rank) const { return (u32)__shfl((int)v, (int)(base + src_rank), 64); }
-    UPD_DEV u64 from64(u64 v, u32 src_rank) const
+    // v of lane src_rank, src_rank the same on every lane of the group: a
+    // scalar read of the lane when the group is the whole wave
+    UPD_DEV u32 bcast(u32 v, u32 src_rank) const
     {
-        const u32 lo = from((u32)v, src_rank), hi = from((u32)(v >> 32), src_rank);
-        return ((u64)hi << 32) | lo;
+        if constexpr (TPI == 64) return (u32)__builtin_amdgcn_readlane((int)v, (int)src_rank);
+        else return from(v, src_rank);
     }
+    // v of the next lower lane of the wave (DPP wave_shr:1); rank 0 of a
+    // group reads the group below (callers ignore it)
+    UPD_DEV u32 below(u32 v) const { return (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false); }
+    // v of the next higher lane of the wave (DPP wave_shl:1); the top rank of
+    // a group reads the group above (callers ignore it)
+    UPD_DEV u32 above(u32 v) const { return (u32)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false); }
+    UPD_DEV u64 below64(u64 v) const { return ((u64)below((u32)(v >> 32)) << 32) | below((u32)v); }
 };
 
 template <int L> UPD_DEV MMag<L> mzero()
@@ -104,7 +114,7 @@ template <int L, int TPI> UPD_DEV u32 add_mag(MMag<L>& a, const MMag<L>& b, u32 
         cc = (u32)(s >> 32);
     }
     const u32 out = (gen || (prop && cin)) ? 1u : 0u;
-    return g.from(out, TPI - 1u);
+    return g.bcast(out, TPI - 1u);
 }
 
 // a -= b (modulo 2^(32 W)); returns the borrow out of the group
@@ -134,7 +144,7 @@ template <int L, int TPI> UPD_DEV int cmp(const MMag<L>& a, const MMag<L>& b, co
     const u64 nz = g.bits(__ballot(loc != 0 ? 1 : 0));
     if (nz == 0ull) return 0;
     const u32 top = 63u - (u32)__clzll(nz);
-    return (int)g.from((u32)loc, top);
+    return (int)g.bcast((u32)loc, top);
 }
 
 // number of limbs up to the top nonzero one (0 for zero), on every lane
@@ -147,7 +157,7 @@ template <int L, int TPI> UPD_DEV u32 limb_len(const MMag<L>& a, const Grp<TPI>&
     const u64 nz = g.bits(__ballot(loc != 0 ? 1 : 0));
     if (nz == 0ull) return 0u;
     const u32 top = 63u - (u32)__clzll(nz);
-    return top * (u32)L + g.from(loc, top);
+    return top * (u32)L + g.bcast(loc, top);
 }
 
 // lanes of the group up to the highest one holding a nonzero limb (0 for
@@ -165,14 +175,14 @@ template <int L, int TPI> UPD_DEV u32 rank_len(const MMag<L>& a, const Grp<TPI>&
 template <int L, int TPI> UPD_DEV u32 limb(const MMag<L>& a, u32 k, const Grp<TPI>& g)
 {
     if (k >= (u32)(L * TPI)) return 0u;
-    return g.from(sel<L>(a, k % (u32)L), k / (u32)L);
+    return g.bcast(sel<L>(a, k % (u32)L), k / (u32)L);
 }
 
 // add a multi-bit carry word cw (< 2^(32 L - 1)) from the lane below into
 // every lane's slice and resolve the one-bit carries that follows
 template <int L, int TPI> UPD_DEV void carry_up(MMag<L>& a, u64 cw_out, const Grp<TPI>& g)
 {
-    const u64 below = g.from64(cw_out, (g.rank - 1u) & (TPI - 1u));
+    const u64 below = g.below64(cw_out);
     u64 c = g.rank == 0 ? 0ull : below;
     u32 all1 = 0xffffffffu;
 #pragma unroll
@@ -205,8 +215,8 @@ template <int L, int TPI> UPD_DEV u32 mul_word(MMag<L>& a, u32 m, const Grp<TPI>
     }
     // the top lane's word leaves the group, plus a one-bit carry the
     // resolution below may push out of it
-    const u32 top_c = g.from((u32)c, TPI - 1u);
-    const u64 below = g.from64(c, (g.rank - 1u) & (TPI - 1u));
+    const u32 top_c = g.bcast((u32)c, TPI - 1u);
+    const u64 below = g.below64(c);
     u64 cw = g.rank == 0 ? 0ull : below;
     u32 all1 = 0xffffffffu;
 #pragma unroll
@@ -226,7 +236,7 @@ template <int L, int TPI> UPD_DEV u32 mul_word(MMag<L>& a, u32 m, const Grp<TPI>
         cc = (u32)(s >> 32);
     }
     const u32 top_out = (gen || (prop && cin)) ? 1u : 0u;
-    return top_c + g.from(top_out, TPI - 1u);
+    return top_c + g.bcast(top_out, TPI - 1u);
 }
 
 // a * 10^n modulo 2^(32 W), n at run time (uniform in the group)
@@ -260,7 +270,7 @@ template <int L, int TPI> UPD_DEV void div_word(MMag<L>& a, u32 d, const Grp<TPI
             hm = (hm * om) % d;
         }
     }
-    const u32 above = g.from((u32)ha, (g.rank + 1u) & (TPI - 1u));
+    const u32 above = g.above((u32)ha);
     u64 rem = g.rank + 1u < (u32)TPI ? above : 0u;
 #pragma unroll
     for (int i = L - 1; i >= 0; i--) {
@@ -306,7 +316,7 @@ template <int L, int TPI> UPD_DEV MMag<L> mul(const MMag<L>& a, const MMag<L>& b
         const u32 s0 = (g.rank - ir) & (TPI - 1u), s1 = (g.rank - ir - 1u) & (TPI - 1u);
 #pragma unroll
         for (int i = 0; i < L; i++) {
-            av[i] = g.from(x.w[i], ir);
+            av[i] = g.bcast(x.w[i], ir);
             b0[i] = g.from(y.w[i], s0);
             b1[i] = g.from(y.w[i], s1);
         }
@@ -340,9 +350,9 @@ template <int L, int TPI> UPD_DEV MMag<L> mul(const MMag<L>& a, const MMag<L>& b
 // of the group's top limb (on every lane)
 template <int L, int TPI> UPD_DEV u32 shl_bits(MMag<L>& a, u32 s, const Grp<TPI>& g)
 {
-    const u32 topw = g.from(a.w[L - 1], TPI - 1u);
+    const u32 topw = g.bcast(a.w[L - 1], TPI - 1u);
     const u32 out = s ? topw >> (32u - s) : 0u;
-    const u32 below = g.from(a.w[L - 1], (g.rank - 1u) & (TPI - 1u));
+    const u32 below = g.below(a.w[L - 1]);
     u32 prev = g.rank == 0 ? 0u : below;
     if (s) {
 #pragma unroll
@@ -358,7 +368,7 @@ template <int L, int TPI> UPD_DEV u32 shl_bits(MMag<L>& a, u32 s, const Grp<TPI>
 // shift right by s bits (0 <= s < 32, uniform); `in` enters the top limb
 template <int L, int TPI> UPD_DEV void shr_bits(MMag<L>& a, u32 s, u32 in, const Grp<TPI>& g)
 {
-    const u32 above = g.from(a.w[0], (g.rank + 1u) & (TPI - 1u));
+    const u32 above = g.above(a.w[0]);
     u32 next = g.rank + 1u < (u32)TPI ? above : in;
     if (s) {
 #pragma unroll
@@ -390,7 +400,7 @@ template <int L, int TPI> UPD_DEV MMag<L> shl_limbs(const MMag<L>& a, u32 n, con
 // a >> 32 (one limb right)
 template <int L, int TPI> UPD_DEV void shr_limb1(MMag<L>& a, const Grp<TPI>& g)
 {
-    const u32 above = g.from(a.w[0], (g.rank + 1u) & (TPI - 1u));
+    const u32 above = g.above(a.w[0]);
 #pragma unroll
     for (int i = 0; i < L - 1; i++) a.w[i] = a.w[i + 1];
     a.w[L - 1] = g.rank + 1u < (u32)TPI ? above : 0u;
@@ -557,7 +567,7 @@ UPD_DEV MDec<L> load_compact(const u8* __restrict__ col, usize row, u32 B, usize
     const u32 sb = 8u * (B - 1u) + 7u;
     const u32 owner = sb / (32u * (u32)L);
     const u32 loc = (sb / 32u) % (u32)L;
-    const u32 w = g.from(sel<L>(r.m, loc), owner);
+    const u32 w = g.bcast(sel<L>(r.m, loc), owner);
     r.neg = (w >> (sb & 31u)) & 1u;
     if (g.rank == owner) {
 #pragma unroll
@@ -583,7 +593,7 @@ UPD_DEV MDec<L> load_compact_b(const u8* __restrict__ col, usize row, const Grp<
         }
         constexpr u32 sb = 8u * (B - 1u) + 7u;
         constexpr u32 owner = sb / (32u * (u32)L), loc = (sb / 32u) % (u32)L;
-        r.neg = (g.from(r.m.w[loc], owner) >> (sb & 31u)) & 1u;
+        r.neg = (g.bcast(r.m.w[loc], owner) >> (sb & 31u)) & 1u;
         if (g.rank == owner) r.m.w[loc] &= ~(1u << (sb & 31u));
     } else {
         r = load_compact<L, TPI>(col, row, (u32)B, (row + 1) * (usize)B, g);
