@@ -8,14 +8,15 @@
 // TPI holds limbs [r L, r L + L), every lane of the group holds the sign.
 // Carries and borrows between the lanes of a group are resolved with one
 // wave ballot (carry lookahead on the group's bits of the ballot mask),
-// multi-bit carries move one lane up with a shuffle, and a limb is
-// broadcast to the group with a shuffle.  Every group of a wave runs the
-// same instruction stream on its own value; all group-wide decisions are
-// uniform within a group, so groups of one wave may diverge without
-// splitting a group.
+// multi-bit carries move one lane up with a DPP wave shift, and a limb is
+// broadcast to the group with a shuffle (a scalar lane read when the group
+// is the whole wave).  Every group of a wave runs the same instruction
+// stream on its own value; all group-wide decisions are uniform within a
+// group, so groups of one wave may diverge without splitting a group.
 //
-// Used by the interpreter's wide register classes (vm.hip): W = 64, 128 and
-// 256 limbs, where one value per lane no longer fits the register file.
+// Used by the interpreter's wide register classes (vm.hip): W = 64, 128, 256
+// and 512 limbs, where one value per lane no longer fits the register file;
+// and by the generated multi-lane calc kernels (planner.cpp, gen_calc_ml).
 #ifndef UPDEC_DEC_MLANE_H
 #define UPDEC_DEC_MLANE_H
 #include "dec_device.h"
