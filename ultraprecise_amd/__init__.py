@@ -536,12 +536,19 @@ class Agg:
         max_groups * (nexprs + len(avg)) * 133 bytes, SUM image of expression k
         in group g at g * nexprs + k, AVG j at max_groups * nexprs + g * len(avg)
         + j; counts: DeviceBuffer of max_groups int64."""
-        ptrs = (ctypes.c_void_p * max(1, len(columns)))(*[c.ptr for c in columns])
-        ng = ctypes.c_int32(0)
-        keys = (ctypes.c_uint16 * max(1, max_groups))()
+        # the argument arrays are kept between calls with the same columns
+        # and group bound (a serving or benchmark loop repeats them)
+        cols = tuple(c.ptr for c in columns)
+        cached = getattr(self, "_run_args", None)
+        if cached is None or cached[0] != cols or cached[1] != max_groups:
+            cached = (cols, max_groups, (ctypes.c_void_p * max(1, len(cols)))(*cols), ctypes.c_int32(0),
+                      (ctypes.c_uint16 * max(1, max_groups))())
+            self._run_args = cached
+        ptrs, ng, keys = cached[2], cached[3], cached[4]
         _check(_lib.updec_agg_run(self._h, ptrs, _ptr(filter_col), _ptr(key0), _ptr(key1), rows, _ptr(images),
                                   _ptr(counts), max_groups, ctypes.byref(ng), keys, _stream(stream)))
-        return ng.value, [keys[i] for i in range(ng.value)]
+        n = ng.value
+        return n, list(keys[:n])
 
     def last_kernel_ms(self):
         ms = ctypes.c_float(0)

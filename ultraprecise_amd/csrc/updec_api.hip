@@ -1176,9 +1176,25 @@ static int64_t agg_grid(hipFunction_t f, int64_t rows, int T, int rps)
 {
     const int64_t steps = (rows + rps - 1) / rps;
     int64_t tiles = std::max<int64_t>(1, (steps + T - 1) / T);
+    // the occupancy query is made once per kernel and block size (a
+    // repeated run would otherwise pay it on every call); a handle reused by
+    // a later module can only make the grid a poor size, never wrong, since
+    // the per-lane row bound below does not depend on it
+    static std::mutex occ_mu;
+    static std::map<std::pair<hipFunction_t, int>, int> occ;
     int blocks_per_cu = 0;
-    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, f, T, 0) != hipSuccess || blocks_per_cu < 1)
-        blocks_per_cu = 2;
+    {
+        std::lock_guard<std::mutex> g(occ_mu);
+        auto it = occ.find({f, T});
+        if (it != occ.end()) {
+            blocks_per_cu = it->second;
+        } else {
+            if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, f, T, 0) != hipSuccess ||
+                blocks_per_cu < 1)
+                blocks_per_cu = 2;
+            occ[{f, T}] = blocks_per_cu;
+        }
+    }
     int64_t grid = std::min<int64_t>(tiles, (int64_t)device_cus() * blocks_per_cu);
     // every lane <= 2^14 rows: at most 2^12 quads (or 2^14 single rows)
     const int64_t per_block = (int64_t)T << (kAggRowsPerLaneLog2 - 2);
