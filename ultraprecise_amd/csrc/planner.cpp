@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -1883,15 +1884,33 @@ int agg_rows_per_step(const AggSpec& s, const std::vector<ColumnType>& cols, con
 // rows that can reach one slot copy: 2^14 per lane times the lanes sharing it
 static int share_log_for(int split) { return kAggRowsPerLaneLog2 + (split >= 64 ? 0 : 6 - __builtin_ctz((unsigned)split)); }
 
-// Accumulator layout of the grouped kernels: u64 slots [copy][slot][plan][AWR]
-// where copy = (wave, lane group).  AWR = 1 holds the signed 64-bit sum
-// directly; otherwise AWR = AW2 slots hold the 32-bit limbs of the two's
-// complement value separately ("redundant" limbs: each slot collects its
-// limb's column sum, carries are resolved once at the end), so every update
-// is a plain ds_add_u64 with no carry chain.
+// Accumulator layout of the grouped kernels: u64 words [copy][slot][plan
+// words] where copy = (wave, lane group).  AWR = 1: one word per plan holds
+// the signed 64-bit sum directly.  Otherwise plan i has W_i + 1 words (W_i
+// the limbs of its stored value, at most AW2) holding the 32-bit limbs of
+// the two's complement value separately ("redundant" limbs: each word
+// collects its limb's column sum, carries are resolved once at the end), so
+// every update is a plain ds_add_u64 with no carry chain; a block's sum of
+// values below 2^(32 W_i) over at most 2^20 rows fits W_i + 1 words.
 struct AccCfg {
     int T = 256, SPLIT = 4, AWR = 1;
 };
+
+static int plan_slot_words(const Plan& p, int AWR, int AW2) { return AWR == 1 ? 1 : std::min(AW2, p.n[p.root].W + 1); }
+
+// words of one slot over all plans, and plan i's first word
+static int slot_words_total(const AggSpec& s, int AWR, int AW2)
+{
+    int w = 0;
+    for (const Plan& p : s.plans) w += plan_slot_words(p, AWR, AW2);
+    return w;
+}
+static int slot_word_offset(const AggSpec& s, int i, int AWR, int AW2)
+{
+    int w = 0;
+    for (int k = 0; k < i; k++) w += plan_slot_words(s.plans[k], AWR, AW2);
+    return w;
+}
 
 static bool choose_acc(const AggSpec& s, int nslots, const GenOptions& o, AccCfg* c)
 {
@@ -1900,7 +1919,11 @@ static bool choose_acc(const AggSpec& s, int nslots, const GenOptions& o, AccCfg
     for (auto& cd : cands) {
         const int T = cd[0], SP = std::max(1, std::min(64, cd[1]));
         const int AWR = (vbits + share_log_for(SP) + 1 <= 64) ? 1 : AW2;
-        const long bytes = (long)(T / 64) * SP * nslots * ((long)NP * AWR * 8 + 4);
+        // multi-word slots take at most two copies per wave: more copies
+        // cost more LDS than the contention they save (Q1 shape without
+        // column statistics: 0.539 against 0.564 ms at four)
+        if (AWR > 1 && SP > 2) continue;
+        const long bytes = (long)(T / 64) * SP * nslots * ((long)slot_words_total(s, AWR, AW2) * 8 + 4);
         if (bytes <= 60 * 1024) {
             c->T = T;
             c->SPLIT = SP;
@@ -1929,7 +1952,19 @@ static void emit_agg_prelude(Emit& E, const AggSpec& s, const AccCfg& c, int nsl
     for (int i = 0; i < NP; i++) E.os << "// plan " << i << ": " << s.plans[i].text(false) << "\n";
     E.os << "#define NP " << NP << "\n#define AW2 " << agg_acc_words(s) << "\n#define T " << c.T << "\n#define SPLIT "
          << c.SPLIT << "\n#define NWAVE (T / 64)\n#define NCOPY (NWAVE * SPLIT)\n#define NS " << nslots
-         << "\n#define AWR " << c.AWR << "\n";
+         << "\n#define AWR " << c.AWR << "\n#define SLOTW " << slot_words_total(s, c.AWR, agg_acc_words(s)) << "\n";
+    // plan pp's first word and word count in a slot, at run time and as
+    // compile-time constants
+    E.os << "UPD_DEV u32 slot_off(u32 pp) {\n  u32 r = 0u;\n";
+    for (int i = 0; i < NP; i++)
+        E.os << "  if (pp == " << i << "u) r = " << slot_word_offset(s, i, c.AWR, agg_acc_words(s)) << "u;\n";
+    E.os << "  return r;\n}\nUPD_DEV u32 slot_words(u32 pp) {\n  u32 r = 1u;\n";
+    for (int i = 0; i < NP; i++)
+        E.os << "  if (pp == " << i << "u) r = " << plan_slot_words(s.plans[i], c.AWR, agg_acc_words(s)) << "u;\n";
+    E.os << "  return r;\n}\n";
+    for (int i = 0; i < NP; i++)
+        E.os << "#define SLOT_OFF" << i << " " << slot_word_offset(s, i, c.AWR, agg_acc_words(s)) << "\n#define SLOT_WORDS" << i
+             << " " << plan_slot_words(s.plans[i], c.AWR, agg_acc_words(s)) << "\n";
     // round trip through the materialized compact column (the reference sums
     // the stored compact result, AriesExprCalcNode.cpp:1623-1700)
     E.os << "template <int B, int W> UPD_DEV Dec<W> stored(const Dec<W>& v) {\n"
@@ -1947,7 +1982,7 @@ static void emit_agg_prelude(Emit& E, const AggSpec& s, const AccCfg& c, int nsl
             "  return r;\n"
             "}\n";
     // add one signed value into a slot of AWR u64 words
-    E.os << "template <int W> UPD_DEV void slot_add(u64* sl, const Dec<W>& v) {\n"
+    E.os << "template <int W, int N> UPD_DEV void slot_add(u64* sl, const Dec<W>& v) {\n"
             "  if constexpr (AWR == 1) {\n"
             "    u64 m = v.m.w[0];\n"
             "    if constexpr (W > 1) m |= (u64)v.m.w[1] << 32;\n"
@@ -1955,14 +1990,14 @@ static void emit_agg_prelude(Emit& E, const AggSpec& s, const AccCfg& c, int nsl
             "  } else {\n"
             "    const u32 x = v.neg ? 0xffffffffu : 0u;\n"
             "#pragma unroll\n"
-            "    for (int i = 0; i < AWR; i++) {\n"
+            "    for (int i = 0; i < N; i++) {\n"
             "      const u64 add = (u64)((i < W ? v.m.w[i] : 0u) ^ x) + (i == 0 ? (u64)(v.neg & 1u) : 0ull);\n"
             "      if (add) atomicAdd(sl + i, add);\n"
             "    }\n"
             "  }\n"
             "}\n";
     // slot -> AW2-limb two's complement value
-    E.os << "UPD_DEV Mag<AW2> slot_value(const u64* sl) {\n"
+    E.os << "UPD_DEV Mag<AW2> slot_value(const u64* sl, u32 n) {\n"
             "  Mag<AW2> r;\n"
             "  if constexpr (AWR == 1) {\n"
             "    const u64 x = sl[0];\n"
@@ -1970,12 +2005,19 @@ static void emit_agg_prelude(Emit& E, const AggSpec& s, const AccCfg& c, int nsl
             "#pragma unroll\n"
             "    for (int w = 0; w < AW2; w++) r.w[w] = w == 0 ? (u32)x : (w == 1 ? (u32)(x >> 32) : ext);\n"
             "  } else {\n"
+            "    // n words: the value modulo 2^(32 n), then its sign extended\n"
             "    u64 carry = 0;\n"
+            "    u32 ext = 0u;\n"
             "#pragma unroll\n"
             "    for (int w = 0; w < AW2; w++) {\n"
-            "      const u64 lo = (sl[w] & 0xffffffffull) + (carry & 0xffffffffull);\n"
-            "      r.w[w] = (u32)lo;\n"
-            "      carry = (sl[w] >> 32) + (carry >> 32) + (lo >> 32);\n"
+            "      if ((u32)w < n) {\n"
+            "        const u64 lo = (sl[w] & 0xffffffffull) + (carry & 0xffffffffull);\n"
+            "        r.w[w] = (u32)lo;\n"
+            "        carry = (sl[w] >> 32) + (carry >> 32) + (lo >> 32);\n"
+            "        ext = (r.w[w] >> 31) ? 0xffffffffu : 0u;\n"
+            "      } else {\n"
+            "        r.w[w] = ext;\n"
+            "      }\n"
             "    }\n"
             "  }\n"
             "  return r;\n"
@@ -1998,7 +2040,8 @@ static void emit_agg_row(Emit& E, const AggSpec& s, const std::vector<ColumnType
         // the iterator form skips a NULL value (agg_sum_t of nullable_type,
         // operators.hxx:1175-1195)
         E.os << ind << (E.iter ? "if (p" + std::to_string(i) + "_nn) " : std::string()) << "slot_add<" << W
-             << ">(&aslot[sbase + " << i << " * AWR], stored<" << p.outB << ", " << W << ">(" << rv << "));\n";
+             << ", SLOT_WORDS" << i << ">(&aslot[sbase + SLOT_OFF" << i << "], stored<" << p.outB << ", " << W << ">(" << rv
+             << "));\n";
     }
 }
 
@@ -2111,11 +2154,11 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     const StagePlan sp;
     emit_stage_decl(E, sp, c.T / 64);
     E.os << "  __shared__ u32 htab[HT];\n  __shared__ u32 nslot;\n  __shared__ u32 skey[MAXS];\n";
-    E.os << "  __shared__ u64 aslot[NCOPY * NS * NP * AWR];\n  __shared__ u32 acnt[NCOPY * NS];\n";
+    E.os << "  __shared__ u64 aslot[NCOPY * NS * SLOTW];\n  __shared__ u32 acnt[NCOPY * NS];\n";
     E.os << "  for (u32 i = tid; i < HT; i += T) htab[i] = 0u;\n";
     E.os << "  for (u32 i = tid; i < MAXS; i += T) skey[i] = 0u;\n";
     E.os << "  if (tid == 0) nslot = 0u;\n";
-    E.os << "  for (u32 i = tid; i < NCOPY * NS * NP * AWR; i += T) aslot[i] = 0ull;\n";
+    E.os << "  for (u32 i = tid; i < NCOPY * NS * SLOTW; i += T) aslot[i] = 0ull;\n";
     E.os << "  for (u32 i = tid; i < NCOPY * NS; i += T) acnt[i] = 0u;\n";
     E.os << "  __syncthreads();\n";
     E.os << "  const u32 copy = wave * SPLIT + lane / (64 / SPLIT);\n";
@@ -2127,7 +2170,7 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     if (s.filter_col >= 0) E.os << bi << "if (!(fv " << fops[s.filter_op] << " (i64)" << s.filter_val << "LL)) continue;\n";
     E.os << bi << "const u32 g = slot_of(key, htab, &nslot, skey);\n";
     E.os << bi << "if (g >= MAXS) { ovf = 1u; continue; }\n";
-    E.os << bi << "const u32 sbase = (copy * NS + g) * NP * AWR;\n";
+    E.os << bi << "const u32 sbase = (copy * NS + g) * SLOTW;\n";
     emit_agg_row(E, s, cols, loaded, bi);
     E.os << bi << "atomicAdd(&acnt[copy * NS + g], 1u);\n";
     emit_row_loop_close(E, rps);
@@ -2148,7 +2191,7 @@ static std::string gen_agg_onepass(const AggSpec& s, const std::vector<ColumnTyp
     E.os << "    u32* e = a.partials + (usize)sl * EW * nb + blockIdx.x;\n";
     E.os << "    if (pp == 0) e[0] = sl < ns ? skey[sl] : 0u;\n";
     E.os << "    if (pp < NP) {\n      Mag<AW2> sm = mzero<AW2>();\n";
-    E.os << "      if (sl < ns) for (u32 cc = 0; cc < NCOPY; cc++) add_to<AW2>(sm, sm, slot_value(&aslot[((cc * NS + sl) * NP + pp) * AWR]));\n";
+    E.os << "      if (sl < ns) for (u32 cc = 0; cc < NCOPY; cc++) add_to<AW2>(sm, sm, slot_value(&aslot[(cc * NS + sl) * SLOTW + slot_off(pp)], slot_words(pp)));\n";
     E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) e[(usize)(1 + pp * AW2 + w) * nb] = sm.w[w];\n";
     E.os << "    } else {\n      u64 cnt = 0;\n";
     E.os << "      if (sl < ns) for (u32 cc = 0; cc < NCOPY; cc++) cnt += acnt[cc * NS + sl];\n";
@@ -2281,8 +2324,8 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
     const StagePlan sp = (reg && !o.blocks && agg_rows_per_step(s, cols, o) == 1) ? stage_plan(s, cols) : StagePlan();
     emit_stage_decl(E, sp, c.T / 64);
     if (!reg) {
-        E.os << "  __shared__ u64 aslot[NCOPY * NS * NP * AWR];\n  __shared__ u32 acnt[NCOPY * NS];\n";
-        E.os << "  for (u32 i = tid; i < NCOPY * NS * NP * AWR; i += T) aslot[i] = 0ull;\n";
+        E.os << "  __shared__ u64 aslot[NCOPY * NS * SLOTW];\n  __shared__ u32 acnt[NCOPY * NS];\n";
+        E.os << "  for (u32 i = tid; i < NCOPY * NS * SLOTW; i += T) aslot[i] = 0ull;\n";
         E.os << "  for (u32 i = tid; i < NCOPY * NS; i += T) acnt[i] = 0u;\n";
         E.os << "  __syncthreads();\n";
         E.os << "  const u32 copy = wave * SPLIT + lane / (64 / SPLIT);\n";
@@ -2306,7 +2349,7 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
         E.os << bi << "cnt += " << (o.blocks ? iter_count_expr(s) : std::string("1u")) << ";\n";
     } else {
         E.os << bi << "const u32 g = a.lut[key];\n";
-        E.os << bi << "const u32 sbase = (copy * NS + g) * NP * AWR;\n";
+        E.os << bi << "const u32 sbase = (copy * NS + g) * SLOTW;\n";
         emit_agg_row(E, s, cols, loaded, bi);
         E.os << bi << "atomicAdd(&acnt[copy * NS + g], " << (o.blocks ? iter_count_expr(s) : std::string("1u")) << ");\n";
     }
@@ -2318,7 +2361,7 @@ std::string gen_agg_kernels(const AggSpec& s, const std::vector<ColumnType>& col
         E.os << "  for (u32 k = tid; k < NP * NG + NG; k += T) {\n";
         E.os << "    u32* dst = a.partials + (usize)blockIdx.x * PW;\n";
         E.os << "    if (k < NP * NG) {\n      const u32 gg = k / NP, pp = k % NP;\n      Mag<AW2> sm = mzero<AW2>();\n";
-        E.os << "      for (u32 cc = 0; cc < NCOPY; cc++) add_to<AW2>(sm, sm, slot_value(&aslot[((cc * NS + gg) * NP + pp) * AWR]));\n";
+        E.os << "      for (u32 cc = 0; cc < NCOPY; cc++) add_to<AW2>(sm, sm, slot_value(&aslot[(cc * NS + gg) * SLOTW + slot_off(pp)], slot_words(pp)));\n";
         E.os << "#pragma unroll\n      for (int w = 0; w < AW2; w++) dst[k * AW2 + w] = sm.w[w];\n";
         E.os << "    } else {\n      const u32 gg = k - NP * NG;\n      u64 cn = 0;\n";
         E.os << "      for (u32 cc = 0; cc < NCOPY; cc++) cn += acnt[cc * NS + gg];\n";
